@@ -1,0 +1,302 @@
+/*
+ * fpmhip.h — C ABI of the MI355X (gfx950) implementation of fpm-rs's
+ * image-plane forward model and per-LED reconstruction updates
+ * (AlternatingProjection, Fpie, Mpie, Epry).
+ *
+ * Conventions shared with fpm-rs (reference citations are path:line in
+ * hgrecco/fpm-rs):
+ *   - arrays are C-contiguous row-major (README.md:59-75, src/array_layout.rs);
+ *   - complex arrays are interleaved (re, im) pairs of double, the memory
+ *     layout of num_complex::Complex64;
+ *   - object spectra are centred (fftshift by floor(n/2)); the forward 2-D FFT
+ *     is scaled by 1/N and the inverse is unscaled (src/backend/cpu.rs:162-172);
+ *   - every function returns an fpmhip_status; on failure a thread-local
+ *     message is available from fpmhip_last_error() and, for
+ *     FPMHIP_INVALID_PARAMETER, the stable parameter name from
+ *     fpmhip_last_error_name() (src/error.rs:14-174).
+ *
+ * Ownership: handles are created and destroyed explicitly. A model or a
+ * measurement set may be shared by several sessions; a session must not be
+ * used from two threads at once (a session owns its scratch, as a
+ * ReconstructionState does, src/reconstruction/state.rs:159-172).
+ * Host pointers passed in are read during the call only.
+ */
+#ifndef FPMHIP_H
+#define FPMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: one per fpm_rs::Error variant on this path (src/error.rs:14-174) ---- */
+typedef enum fpmhip_status {
+  FPMHIP_OK = 0,
+  FPMHIP_INVALID_SHAPE = 1,         /* Error::InvalidShape */
+  FPMHIP_INVALID_PARAMETER = 2,     /* Error::InvalidParameter{name, reason} */
+  FPMHIP_INVALID_MODEL = 3,         /* Error::InvalidModel */
+  FPMHIP_INVALID_MEASUREMENTS = 4,  /* Error::InvalidMeasurements */
+  FPMHIP_LENGTH_MISMATCH = 5,       /* Error::LengthMismatch */
+  FPMHIP_SHAPE_OVERFLOW = 6,        /* Error::ShapeOverflow */
+  FPMHIP_NON_STANDARD_LAYOUT = 7,   /* Error::NonStandardLayout */
+  FPMHIP_FRAME_OUT_OF_RANGE = 8,    /* Error::FrameOutOfRange */
+  FPMHIP_NUMERICAL = 9,             /* Error::Numerical */
+  FPMHIP_UNSUPPORTED = 10,          /* Error::Unsupported */
+  FPMHIP_DEVICE_ERROR = 11,         /* HIP runtime failure (no reference counterpart) */
+  FPMHIP_OUT_OF_MEMORY = 12         /* device allocation failure (no reference counterpart) */
+} fpmhip_status;
+
+/* Message of the last failure on the calling thread ("" after success). */
+const char* fpmhip_last_error(void);
+/* Stable parameter name of the last FPMHIP_INVALID_PARAMETER ("" otherwise),
+ * e.g. "object_step", "momentum_friction" (tests/reconstruction_tests.rs:907). */
+const char* fpmhip_last_error_name(void);
+/* Library version string. */
+const char* fpmhip_version(void);
+
+/* ---- enums ---- */
+typedef enum fpmhip_direction { FPMHIP_FORWARD = 0, FPMHIP_INVERSE = 1 } fpmhip_direction;
+
+/* Algorithms whose step is projection_update (src/algorithms/common.rs:57-306). */
+typedef enum fpmhip_algorithm_kind {
+  FPMHIP_ALTERNATING_PROJECTION = 0, /* src/algorithms/alternating_projection.rs:37 */
+  FPMHIP_FPIE = 1,                   /* src/algorithms/fpie.rs:36 */
+  FPMHIP_MPIE = 2,                   /* src/algorithms/mpie.rs:58 */
+  FPMHIP_EPRY = 3                    /* src/algorithms/epry.rs:55 */
+} fpmhip_algorithm_kind;
+
+/* ObjectDenominator (src/algorithms/common.rs:18-22). */
+typedef enum fpmhip_denominator {
+  FPMHIP_DENOMINATOR_LOCAL = 0,
+  FPMHIP_DENOMINATOR_RPIE = 1,
+  FPMHIP_DENOMINATOR_GLOBAL = 2
+} fpmhip_denominator;
+
+/* LossType (src/algorithms/objective.rs:9-19). */
+typedef enum fpmhip_loss {
+  FPMHIP_LOSS_AMPLITUDE_MSE = 0,
+  FPMHIP_LOSS_INTENSITY_MSE = 1,
+  FPMHIP_LOSS_POISSON_NLL = 2,
+  FPMHIP_LOSS_HUBER_AMPLITUDE = 3
+} fpmhip_loss;
+
+/* ReconstructionShape (src/model/image_plane_fpm.rs:287-348). */
+typedef enum fpmhip_shape_policy {
+  FPMHIP_SHAPE_EXACT = 0,
+  FPMHIP_SHAPE_MINIMUM = 1,
+  FPMHIP_SHAPE_SMOOTH = 2,
+  FPMHIP_SHAPE_POWER_OF_TWO = 3
+} fpmhip_shape_policy;
+
+/* Arithmetic of a session.
+ *   PLAIN_C128: every buffer and operation in complex128/f64, the reference's
+ *               arithmetic, one LED step at a time.
+ *   MIXED_F32:  FFTs and pointwise updates in complex64; object-spectrum
+ *               accumulation, step-size denominators (max|P|^2, max|patch|^2)
+ *               and objective reductions in f64. */
+typedef enum fpmhip_precision {
+  FPMHIP_PLAIN_C128 = 0,
+  FPMHIP_MIXED_F32 = 1
+} fpmhip_precision;
+
+/* FrameSchedule (src/reconstruction/schedule.rs:10-27). RandomShuffle needs
+ * rand's StdRng stream and is expressed by passing an explicit order instead. */
+typedef enum fpmhip_schedule {
+  FPMHIP_SCHEDULE_SEQUENTIAL = 0,
+  FPMHIP_SCHEDULE_BRIGHTFIELD_FIRST = 1,
+  FPMHIP_SCHEDULE_SPIRAL_OUT = 2,
+  FPMHIP_SCHEDULE_SNR_WEIGHTED = 3
+} fpmhip_schedule;
+
+/* ---- physical inputs of ImagePlaneModel::from_experiment ---- */
+
+/* Optics (src/experiment/optics.rs:44-60); has_defocus/has_aberration mark
+ * the Option fields. */
+typedef struct fpmhip_optics {
+  double wavelength_vacuum_m;
+  double objective_na;
+  double magnification;
+  double camera_pixel_size;
+  double illumination_refractive_index;
+  double objective_medium_refractive_index;
+  int32_t has_defocus;
+  int32_t has_aberration;
+  double defocus_distance;
+  double astigmatism, coma, spherical, edge_apodization; /* PupilAberration */
+} fpmhip_optics;
+
+/* PlanarLedArray (src/experiment/led_array.rs:120-313) with an ArrayPose:
+ * shape (rows, columns), pitch (x, y), reference index (column, row),
+ * translation (x, y, z) and active extrinsic XYZ rotation in radians. */
+typedef struct fpmhip_planar_array {
+  uint64_t rows, columns;
+  double pitch_x_m, pitch_y_m;
+  double reference_column, reference_row;
+  double translation_m[3];
+  double rotation_rad[3];
+} fpmhip_planar_array;
+
+/* ---- compiled model (ImagePlaneModel fields, src/model/image_plane_fpm.rs:58-73) ---- */
+typedef struct fpmhip_model_desc {
+  uint64_t image_h, image_w;        /* low-resolution grid n_h x n_w */
+  uint64_t recon_h, recon_w;        /* high-resolution grid R_h x R_w */
+  uint64_t source_count;            /* S individual sources */
+  uint64_t frame_count;             /* F measured frames (== S without multiplexing) */
+  const uint64_t* crop_start;       /* [S][2] (row, column) crop origins */
+  const double* offset;             /* [S][2] (row, column) fractional offsets, or NULL */
+  const double* pupil;              /* [N] complex128, interleaved */
+  const uint8_t* support;           /* [N] 0/1 */
+  const double* frame_gain;         /* [F] or NULL */
+  const double* background;         /* [N] or [F*N] or NULL */
+  uint64_t background_len;          /* 0, N or F*N */
+  const uint64_t* mux_row_ptr;      /* [F+1] CSR row pointers, or NULL (identity) */
+  const uint64_t* mux_source;       /* [nnz] source per entry */
+  const double* mux_weight;         /* [nnz] incoherent weight per entry */
+  const double* k_vectors;          /* [S][2] (kx, ky) rad/m, or NULL; used by schedules */
+} fpmhip_model_desc;
+
+/* Sizes of a compile: fill image shape and policy, get S and R back. */
+typedef struct fpmhip_compiled_shape {
+  uint64_t source_count;
+  uint64_t recon_h, recon_w;
+} fpmhip_compiled_shape;
+
+/* ImagePlaneModel::from_experiment (src/model/image_plane_fpm.rs:167-253) for a
+ * planar LED array, in two calls: first with all outputs NULL to get the
+ * shape, then with caller-allocated outputs. `exact_h/exact_w` are read for
+ * FPMHIP_SHAPE_EXACT only. Outputs: crop_start [S][2], offset [S][2],
+ * k_vectors [S][2], pupil [N] complex, support [N], frame_gain [S] (unit
+ * gains). Host-only computation; no device needed. */
+int fpmhip_compile_planar(const fpmhip_optics* optics, const fpmhip_planar_array* array,
+                          uint64_t image_h, uint64_t image_w, int32_t policy,
+                          uint64_t exact_h, uint64_t exact_w, fpmhip_compiled_shape* shape,
+                          uint64_t* crop_start, double* offset, double* k_vectors,
+                          double* pupil, uint8_t* support, double* frame_gain);
+
+/* ---- algorithm configuration (UpdateConfiguration, src/algorithms/common.rs:24-55) ---- */
+typedef struct fpmhip_algorithm {
+  int32_t kind;               /* fpmhip_algorithm_kind */
+  int32_t loss_type;          /* fpmhip_loss */
+  int32_t recover_pupil;      /* EPRY recover_pupil */
+  int32_t constrain_pupil;    /* EPRY constrain_pupil_support */
+  int32_t recover_gain;       /* EPRY recover_frame_gains */
+  int32_t recover_background; /* EPRY recover_background */
+  uint64_t iterations;
+  uint64_t batch_size;
+  double object_step;
+  double pupil_step;          /* EPRY */
+  double stability;           /* FPIE/mPIE rPIE alpha */
+  double epsilon;
+  uint64_t momentum_interval; /* mPIE */
+  double momentum_friction;
+  double momentum_feedback;
+  double gain_step, minimum_gain, maximum_gain;
+  double background_step, minimum_background, maximum_background;
+} fpmhip_algorithm;
+
+/* Fills the reference defaults of `kind` (alternating_projection.rs:51-60,
+ * fpie.rs:53-62, mpie.rs:81-95, epry.rs:91-111). */
+int fpmhip_algorithm_default(int32_t kind, fpmhip_algorithm* out);
+/* Validates as ReconstructionAlgorithm::validate does, with the same stable
+ * parameter names (mpie.rs:229-272, epry.rs:187-238, fpie.rs:111-131,
+ * alternating_projection.rs:108-132). */
+int fpmhip_algorithm_validate(const fpmhip_algorithm* algorithm);
+
+/* ---- opaque handles ---- */
+typedef struct fpmhip_context fpmhip_context;
+typedef struct fpmhip_model fpmhip_model;
+typedef struct fpmhip_measurements fpmhip_measurements;
+typedef struct fpmhip_session fpmhip_session;
+typedef struct fpmhip_batch fpmhip_batch;
+typedef struct fpmhip_server fpmhip_server;
+
+/* One HIP stream per context on `device`. */
+int fpmhip_context_create(int32_t device, fpmhip_context** out);
+int fpmhip_context_destroy(fpmhip_context* context);
+int fpmhip_context_synchronize(fpmhip_context* context);
+
+/* ---- FFT primitive: Backend::fft2 (src/backend/traits.rs:123, cpu.rs:130-174) ----
+ * In-place 2-D complex128 FFT of a host array, forward scaled by 1/(h*w),
+ * inverse unscaled. Any h, w >= 1 (mixed radix). */
+int fpmhip_fft2_host(fpmhip_context* context, double* values, uint64_t h, uint64_t w,
+                     int32_t direction);
+
+/* ---- model and measurement upload ---- */
+/* Validates like ImagePlaneModel::validate (image_plane_fpm.rs:841-972) and
+ * uploads pupil, support, crops, offsets, gains, background, multiplexing. */
+int fpmhip_model_create(fpmhip_context* context, const fpmhip_model_desc* desc,
+                        fpmhip_model** out);
+int fpmhip_model_destroy(fpmhip_model* model);
+
+/* MeasurementStack (src/measurements/stack.rs:47-105): F frames of N pixels.
+ * Exactly one of `counts_u16` (exact detector counts) and `frames_f64` is
+ * non-NULL. `weights` [F] (>= 0; NULL = all 1), `masks` [F*N] 0/1 (NULL = none). */
+int fpmhip_measurements_create(fpmhip_context* context, const fpmhip_model* model,
+                               const uint16_t* counts_u16, const double* frames_f64,
+                               const double* weights, const uint8_t* masks,
+                               fpmhip_measurements** out);
+int fpmhip_measurements_destroy(fpmhip_measurements* measurements);
+
+/* ---- forward model: ForwardModel::forward_intensity_stack_into (src/model/forward.rs:300-372) ----
+ * spectrum [R] complex128 centred, pupil [N] complex128 (NULL = model pupil),
+ * out [F*N] f64 = gain * sum_s w_s |IFFT(ifftshift(patch_s * P))|^2 + background. */
+int fpmhip_forward_stack(fpmhip_context* context, const fpmhip_model* model,
+                         const double* spectrum, const double* pupil, double* out);
+
+/* ---- reconstruction session: ReconstructionState + one algorithm ---- */
+int fpmhip_session_create(const fpmhip_model* model, const fpmhip_measurements* measurements,
+                          int32_t precision, const fpmhip_algorithm* algorithm,
+                          fpmhip_session** out);
+int fpmhip_session_destroy(fpmhip_session* session);
+/* ReconstructionState::initialize_with_backend (src/reconstruction/state.rs:280-355)
+ * followed by the algorithm's auxiliary state (mpie.rs:152-172) and, for
+ * EPRY with pupil recovery, canonicalize_state (runner.rs:122). */
+int fpmhip_session_initialize(fpmhip_session* session);
+/* Restores state (ReconstructionState::from_checkpoint_with_backend, state.rs:398-419):
+ * spectrum [R] and pupil [N] complex128; mPIE velocity/anchor [R] (NULL = zero
+ * velocity, anchor = spectrum) and counter; gains [F] / background (NULL = model). */
+int fpmhip_session_set_state(fpmhip_session* session, const double* spectrum,
+                             const double* pupil, const double* velocity, const double* anchor,
+                             uint64_t counter, const double* frame_gains, const double* background,
+                             uint64_t background_len);
+/* One iteration: projection_update over `order` (F_order frame indices; NULL =
+ * sequential 0..F-1), then canonicalize_state (runner.rs:156-227). Writes the
+ * per-frame objective in order position (may be NULL) and the iteration
+ * objective Σ w_f L_f / Σ w_f (NaN when Σ w_f = 0; metrics.rs:33). */
+int fpmhip_session_sweep(fpmhip_session* session, const uint64_t* order, uint64_t order_len,
+                         double* per_frame_objective, double* objective);
+/* Frame order of FrameSchedule::order_for_problem (schedule.rs:35-104) for the
+ * session's model and measurements, written to `order` [F]. */
+int fpmhip_session_schedule(fpmhip_session* session, int32_t schedule, uint64_t* order);
+/* Runner::run body (runner.rs:76-285) for `iterations` sweeps in `schedule`
+ * order; trace [iterations] of objectives (may be NULL). */
+int fpmhip_session_run(fpmhip_session* session, uint64_t iterations, int32_t schedule,
+                       double* trace);
+/* Downloads (complex arrays interleaved). object = state_object (result.rs:397-413). */
+int fpmhip_session_get_spectrum(fpmhip_session* session, double* out);
+int fpmhip_session_get_pupil(fpmhip_session* session, double* out);
+int fpmhip_session_get_object(fpmhip_session* session, double* out);
+int fpmhip_session_get_momentum(fpmhip_session* session, double* velocity, double* anchor,
+                                uint64_t* counter);
+int fpmhip_session_get_calibration(fpmhip_session* session, double* frame_gains,
+                                   double* background, uint64_t* background_len);
+int fpmhip_session_shape(const fpmhip_session* session, uint64_t* image_h, uint64_t* image_w,
+                         uint64_t* recon_h, uint64_t* recon_w, uint64_t* frame_count);
+
+/* ---- batched form: many independent tiles (each its own model, pupil and
+ * measurements, uneven sizes) advanced together, one launch per sweep for
+ * all tiles, captured in a HIP graph. Sessions must be MIXED_F32, sequential
+ * schedule, and all use the same algorithm kind. ---- */
+int fpmhip_batch_create(fpmhip_context* context, fpmhip_session* const* sessions, uint64_t count,
+                        fpmhip_batch** out);
+int fpmhip_batch_destroy(fpmhip_batch* batch);
+/* Initialise every tile, run `iterations` sweeps (graph replays) and finish
+ * each tile's result; trace [count][iterations] objectives (may be NULL). */
+int fpmhip_batch_run(fpmhip_batch* batch, uint64_t iterations, double* trace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FPMHIP_H */
