@@ -1,0 +1,163 @@
+"""Throughput of fpm-rs's mPIE reconstruction on one MI355X.
+
+Workload (one "step" = one complete tile reconstruction: init, 50 mPIE
+sweeps over 225 LEDs, result): 15x15 planar LED array, 4 mm pitch, 90 mm,
+532 nm, NA 0.10, 4x, 6.5 um pixels; 128^2 images -> 375^2 ('smooth');
+object = smoothed Gaussian random field (sigma 2 px, seed 7) mapped to
+amplitude [0.3, 1] and phase [-pi, pi]; Poisson counts at 1e4 peak photons
+stored as uint16; mPIE with the default step 0.2, stability 0.05 and
+interval 30, friction/feedback 0.5 (fpm-rs's 0.9/0.9 diverges on 225 LEDs).
+
+Prints one JSON line (rank 0). Only the cpu_baseline leg uses the oracle.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fpm-rs_amd"))
+
+import numpy as np  # noqa: E402
+
+from fpmhip import MeasurementStack, Mpie, ReconstructionProblem, abi, runtime  # noqa: E402
+from fpmhip import synthetic as syn  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(n=128, k=15, seed=7, peak=1e4):
+    model = runtime.compile_model(syn.base_optics(), syn.planar_array(k), (n, n))
+    obj = syn.random_field_object(model.reconstruction_shape, seed)
+    I = runtime.forward_stack(model, syn.object_spectrum(obj))
+    counts, ppp = syn.poisson_counts(I, peak, seed)
+    model.with_frame_gains(np.full(model.frame_count, ppp))
+    return model, MeasurementStack(counts=counts)
+
+
+def algorithm(iterations):
+    return Mpie(iterations=iterations, momentum_friction=0.5, momentum_feedback=0.5)
+
+
+def tile_bytes(model, iterations, interval, s_m, s_o, s_p):
+    """Algorithmic HBM bytes of one tile (per-LED window traffic, momentum
+    events, init read and the R^2 init/result passes)."""
+    N = model.image_len
+    M = model.reconstruction_shape[0] * model.reconstruction_shape[1]
+    F = model.frame_count
+    b_led = N * (s_m + 2 * s_o + s_p)
+    events = (iterations * F) // interval
+    return iterations * F * b_led + events * 6 * M * s_o + F * N * s_m + 4 * M * s_o
+
+
+def run_plain(model, meas, iterations, steps, warmup, precision):
+    prob = ReconstructionProblem(meas, model)
+    ctx = runtime.default_context()
+    dm = runtime.DeviceModel(ctx, model)
+    dme = runtime.DeviceMeasurements(ctx, dm, meas)
+    alg = algorithm(iterations)
+
+    def one():
+        s = runtime.Session(prob, alg, precision=precision, context=ctx, dmodel=dm, dmeas=dme)
+        s.initialize()
+        s.run(iterations)
+        return s.object()
+
+    for i in range(warmup):
+        one()
+        log(f"warmup tile {i + 1}/{warmup} done")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        one()
+        log(f"tile {i + 1}/{steps} done")
+    ctx.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(model, meas, iterations, sample_iterations, threads):
+    """The oracle (scalar C++ FP64 restatement of the reference's serial
+    per-tile path) on `threads` host threads, one tile per thread, each
+    running `sample_iterations` of the tile's sweeps; scaled to full tiles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    alg = algorithm(sample_iterations).to_c()
+    times = [0.0] * threads
+
+    def work(i):
+        t = time.perf_counter()
+        oracle.run(model, meas, alg, sample_iterations)
+        times[i] = time.perf_counter() - t
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    per_tile = wall * iterations / sample_iterations
+    return {"value": threads / per_tile, "unit": "tiles/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} tiles x {sample_iterations} of {iterations} mPIE sweeps on {threads} threads "
+                      f"({wall:.1f} s wall), scaled linearly to full tiles"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iterations", type=int, default=50)
+    ap.add_argument("--mode", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample-iterations", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (tiles are independent; "
+              "multi-GPU placement is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    log("building workload")
+    model, meas = workload()
+    log("workload ready", model.reconstruction_shape)
+    precision = "c128" if args.mode == "plain" else "f32"
+    secs = run_plain(model, meas, args.iterations, args.steps, args.warmup, precision)
+    tiles_per_s = args.steps / secs
+    s_o = 16
+    s_p = 16 if precision == "c128" else 8
+    B = tile_bytes(model, args.iterations, 30, 2, s_o, s_p)
+    achieved = B * tiles_per_s / 1e9
+    line = {
+        "metric": "mPIE tiles/sec, 225 LEDs x 128^2 tiles (375^2 object), 50 sweeps",
+        "value": tiles_per_s,
+        "unit": "tiles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "c128" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "225 LEDs 128^2 -> 375^2 mPIE 50 sweeps, uint16 Poisson 1e4", "path": args.mode},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "whole-tile algorithmic bytes / wall time"},
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(model, meas, args.iterations, args.cpu_sample_iterations, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

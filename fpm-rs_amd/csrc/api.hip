@@ -1,0 +1,830 @@
+// api.hip — device-facing C ABI of libfpmhip: contexts, the Backend::fft2
+// primitive, model and measurement upload, the forward stack and the
+// reconstruction session (ReconstructionState + Runner loop).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+#include "session.h"
+
+using namespace fpmhip;
+
+namespace {
+
+template <class F>
+int guarded(F&& body) {
+  clear_error();
+  try {
+    body();
+    return FPMHIP_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  } catch (const std::bad_alloc&) {
+    return capture_bad_alloc();
+  }
+}
+
+void require(bool ok, const char* name, const char* reason) {
+  if (!ok) fail(FPMHIP_INVALID_PARAMETER, reason, name);
+}
+
+template <class T>
+void upload(DeviceBuffer& b, const T* src, size_t n, hipStream_t st) {
+  b.allocate(n * sizeof(T));
+  if (n) HIP_CHECK(hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+void set_device(const Context* c) { HIP_CHECK(hipSetDevice(c->device)); }
+
+inline bool off_zero(double r, double c) { return std::fabs(r) <= 1e-12 && std::fabs(c) <= 1e-12; }
+
+// ---------------------------------------------------------------------------
+// host FP64 restatement of canonicalize_object_pupil (src/algorithms/gauge.rs:20-282)
+// over downloaded state; complex values interleaved.
+struct Cd {
+  double re, im;
+};
+inline Cd cm(Cd a, Cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline Cd cj(Cd a) { return {a.re, -a.im}; }
+inline double cabs(Cd a) { return std::hypot(a.re, a.im); }
+inline double carg(Cd a) { return std::atan2(a.im, a.re); }
+inline Cd cpol(double r, double t) { return {r * std::cos(t), r * std::sin(t)}; }
+
+double phase_slope(const Model& m, const Cd* P, const Cd* Pref, bool row, double tol) {
+  const uint64_t h = m.nh, w = m.nw;
+  Cd res{0, 0}, strongest{0, 0};
+  double total = 0.0, sw = 0.0;
+  const uint64_t rl = row ? h - 1 : h, cl = row ? w : w - 1;
+  for (uint64_t r = 0; r < rl; ++r)
+    for (uint64_t c = 0; c < cl; ++c) {
+      const uint64_t cur = r * w + c, nxt = row ? (r + 1) * w + c : cur + 1;
+      if (!m.support[cur] || !m.support[nxt]) continue;
+      const Cd contribution = cm(cm(P[nxt], cj(P[cur])), cj(cm(Pref[nxt], cj(Pref[cur]))));
+      const double weight = cabs(contribution);
+      if (!std::isfinite(weight)) continue;
+      res = {res.re + contribution.re, res.im + contribution.im};
+      total += weight;
+      if (weight > sw) {
+        strongest = contribution;
+        sw = weight;
+      }
+    }
+  if (total == 0.0) return 0.0;
+  if (cabs(res) > tol * total) return carg(res);
+  return sw > 0.0 ? carg(strongest) : 0.0;
+}
+
+void canonicalize_host(const Model& m, std::vector<Cd>& spec, std::vector<Cd>& P) {
+  const Cd* Pref = reinterpret_cast<const Cd*>(m.pupil.data());
+  if (m.support_count == 0) fail(FPMHIP_INVALID_MODEL, "compiled reference pupil support is empty");
+  const double tol = 64.0 * std::numeric_limits<double>::epsilon() * (double)m.support_count;
+  bool row_exact = true, col_exact = true;
+  for (uint64_t s = 0; s < m.S; ++s) {
+    row_exact = row_exact && std::fabs(m.off_r[s]) <= 1e-12;
+    col_exact = col_exact && std::fabs(m.off_c[s]) <= 1e-12;
+  }
+  const double rs = row_exact ? phase_slope(m, P.data(), Pref, true, tol) : 0.0;
+  const double cs = col_exact ? phase_slope(m, P.data(), Pref, false, tol) : 0.0;
+  if (std::fabs(rs) > tol || std::fabs(cs) > tol) {
+    for (uint64_t r = 0; r < m.nh; ++r)
+      for (uint64_t c = 0; c < m.nw; ++c) {
+        const double y = (double)r - (double)(m.nh / 2), x = (double)c - (double)(m.nw / 2);
+        P[r * m.nw + c] = cm(P[r * m.nw + c], cpol(1.0, -(rs * y + cs * x)));
+      }
+    for (uint64_t r = 0; r < m.rh; ++r)
+      for (uint64_t c = 0; c < m.rw; ++c) {
+        const double y = (double)r - (double)(m.rh / 2), x = (double)c - (double)(m.rw / 2);
+        spec[r * m.rw + c] = cm(spec[r * m.rw + c], cpol(1.0, rs * y + cs * x));
+      }
+  }
+  double eref = 0.0, ecur = 0.0;
+  for (uint64_t p = 0; p < m.N(); ++p)
+    if (m.support[p]) {
+      eref += Pref[p].re * Pref[p].re + Pref[p].im * Pref[p].im;
+      ecur += P[p].re * P[p].re + P[p].im * P[p].im;
+    }
+  if (!std::isfinite(eref) || eref <= 0.0)
+    fail(FPMHIP_INVALID_MODEL, "compiled reference pupil has invalid supported energy");
+  if (!std::isfinite(ecur) || ecur <= 0.0)
+    fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization requires positive finite recovered pupil energy");
+  const double computed = std::sqrt(eref / ecur);
+  if (!std::isfinite(computed) || computed <= 0.0)
+    fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization produced an invalid pupil scale");
+  const double scale = std::fabs(computed - 1.0) <= tol ? 1.0 : computed;
+  Cd ov{0, 0}, st{0, 0};
+  double ow = 0.0, sw = 0.0;
+  for (uint64_t p = 0; p < m.N(); ++p) {
+    if (!m.support[p]) continue;
+    const Cd contribution = cm(cj(Pref[p]), P[p]);
+    const double weight = cabs(contribution);
+    ov = {ov.re + contribution.re, ov.im + contribution.im};
+    ow += weight;
+    if (weight > sw) {
+      st = contribution;
+      sw = weight;
+    }
+  }
+  double phase;
+  if (ow > 0.0 && cabs(ov) > tol * ow)
+    phase = carg(ov);
+  else if (sw > 0.0)
+    phase = carg(st);
+  else
+    fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization could not anchor recovered pupil phase");
+  if (!std::isfinite(phase)) fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization produced a non-finite pupil phase");
+  if (std::fabs(phase) <= tol) phase = 0.0;
+  const Cd pf = cpol(scale, -phase);
+  const double n2 = pf.re * pf.re + pf.im * pf.im;
+  const Cd of{(1.0 * pf.re + 0.0 * pf.im) / n2, (0.0 * pf.re - 1.0 * pf.im) / n2};
+  if (!(pf.re == 1.0 && pf.im == 0.0)) {
+    for (auto& v : P) v = cm(v, pf);
+    for (auto& v : spec) v = cm(v, of);
+  }
+  Cd strongest{0, 0};
+  double smag = 0.0;
+  for (const Cd& v : spec) {
+    if (!std::isfinite(v.re) || !std::isfinite(v.im))
+      fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization requires a finite object spectrum");
+    const double mag = cabs(v);
+    if (mag > smag) {
+      strongest = v;
+      smag = mag;
+    }
+  }
+  if (smag == 0.0) return;
+  const Cd dc = spec[(m.rh / 2) * m.rw + m.rw / 2];
+  const double oph = carg(cabs(dc) > tol * smag ? dc : strongest);
+  if (!std::isfinite(oph)) fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization produced a non-finite object phase");
+  if (std::fabs(oph) <= tol) return;
+  const Cd corr = cpol(1.0, -oph);
+  for (auto& v : spec) v = cm(v, corr);
+}
+
+// ---- session helpers ----------------------------------------------------------------
+
+void download_pupil64(fpmhip_session& s, double* out) {
+  const Model& m = *s.model;
+  const uint64_t N = m.N();
+  if (s.precision == FPMHIP_PLAIN_C128) {
+    HIP_CHECK(hipMemcpyAsync(out, s.b.pupil.ptr, N * sizeof(double2), hipMemcpyDeviceToHost, s.ctx->stream));
+  } else {
+    launch_pupil_to_f64(s.b.pupil.ptr, true, s.b.work_m.as<double2>(), N, s.ctx->stream);
+    HIP_CHECK(hipMemcpyAsync(out, s.b.work_m.ptr, N * sizeof(double2), hipMemcpyDeviceToHost, s.ctx->stream));
+  }
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+}
+
+void upload_pupil64(fpmhip_session& s, const double* src) {
+  const uint64_t N = s.model->N();
+  if (s.precision == FPMHIP_PLAIN_C128) {
+    HIP_CHECK(hipMemcpyAsync(s.b.pupil.ptr, src, N * sizeof(double2), hipMemcpyHostToDevice, s.ctx->stream));
+  } else {
+    HIP_CHECK(hipMemcpyAsync(s.b.work_m.ptr, src, N * sizeof(double2), hipMemcpyHostToDevice, s.ctx->stream));
+    launch_convert_pupil(s.b.work_m.as<double2>(), s.b.pupil.ptr, true, N, s.ctx->stream);
+  }
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+}
+
+void refresh_max_pupil(fpmhip_session& s) {
+  launch_max_abs2(s, s.b.pupil.ptr, s.precision == FPMHIP_PLAIN_C128, s.model->N(), 0);
+}
+
+void canonicalize_session(fpmhip_session& s) {
+  const Model& m = *s.model;
+  std::vector<Cd> spec(m.M()), P(m.N());
+  HIP_CHECK(hipMemcpyAsync(spec.data(), s.b.spec.ptr, m.M() * sizeof(double2), hipMemcpyDeviceToHost, s.ctx->stream));
+  download_pupil64(s, reinterpret_cast<double*>(P.data()));
+  canonicalize_host(m, spec, P);
+  HIP_CHECK(hipMemcpyAsync(s.b.spec.ptr, spec.data(), m.M() * sizeof(double2), hipMemcpyHostToDevice, s.ctx->stream));
+  upload_pupil64(s, reinterpret_cast<const double*>(P.data()));
+  refresh_max_pupil(s);
+}
+
+void check_flags(fpmhip_session& s) {
+  int flags[4];
+  HIP_CHECK(hipMemcpyAsync(flags, s.b.flag.ptr, sizeof(flags), hipMemcpyDeviceToHost, s.ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+  if (flags[0]) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
+  if (flags[1]) fail(FPMHIP_INVALID_MEASUREMENTS, "no positive-weight, unmasked measurements are available");
+}
+
+void init_aux(fpmhip_session& s) {
+  if (!s.cfg.momentum) return;
+  const uint64_t M = s.model->M();
+  launch_fill_zero(s.b.vel.as<double2>(), M, s.ctx->stream);
+  HIP_CHECK(hipMemcpyAsync(s.b.anc.ptr, s.b.spec.ptr, M * sizeof(double2), hipMemcpyDeviceToDevice, s.ctx->stream));
+  s.counter = 0;
+}
+
+std::vector<uint64_t> schedule_order(const fpmhip_session& s, int schedule) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  std::vector<uint64_t> order(m.F);
+  std::iota(order.begin(), order.end(), 0);
+  if (schedule == FPMHIP_SCHEDULE_SEQUENTIAL) return order;
+  if (schedule == FPMHIP_SCHEDULE_SNR_WEIGHTED) {  // schedule.rs:84-128
+    std::vector<double> score(m.F);
+    const uint64_t N = m.N();
+    for (uint64_t f = 0; f < m.F; ++f) {
+      if (me.weight[f] == 0.0) {
+        score[f] = -std::numeric_limits<double>::infinity();
+        continue;
+      }
+      double sum = 0.0;
+      uint64_t valid = 0;
+      for (uint64_t p = 0; p < N; ++p)
+        if (me.masks.empty() || me.masks[f * N + p]) {
+          const double b = m.background.empty() ? 0.0 : m.background[m.background.size() == N ? p : f * N + p];
+          sum += std::max(me.value(f, p) - b, 0.0);
+          ++valid;
+        }
+      score[f] = valid ? me.weight[f] * std::sqrt(sum / (double)valid) : -std::numeric_limits<double>::infinity();
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return score[a] > score[b]; });
+    return order;
+  }
+  if (!m.has_k) fail(FPMHIP_INVALID_MODEL, "schedule requires k-vectors");
+  std::vector<double> rad(m.F), ang(m.F);
+  for (uint64_t f = 0; f < m.F; ++f) {  // frame_vector (schedule.rs:130-146)
+    double kx = 0.0, ky = 0.0;
+    if (m.mux) {
+      double tw = 0.0;
+      for (uint64_t e = m.row_ptr[f]; e < m.row_ptr[f + 1]; ++e) {
+        kx += m.mw[e] * m.kx[m.msrc[e]];
+        ky += m.mw[e] * m.ky[m.msrc[e]];
+        tw += m.mw[e];
+      }
+      kx /= tw;
+      ky /= tw;
+    } else {
+      kx = m.kx[f];
+      ky = m.ky[f];
+    }
+    rad[f] = std::hypot(kx, ky);
+    double a = std::fmod(std::atan2(ky, kx), 6.283185307179586);
+    ang[f] = a < 0.0 ? a + 6.283185307179586 : a;
+  }
+  if (schedule == FPMHIP_SCHEDULE_BRIGHTFIELD_FIRST)
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return rad[a] < rad[b]; });
+  else if (schedule == FPMHIP_SCHEDULE_SPIRAL_OUT)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint64_t a, uint64_t b) { return rad[a] != rad[b] ? rad[a] < rad[b] : ang[a] < ang[b]; });
+  else
+    fail(FPMHIP_UNSUPPORTED, "unknown frame schedule");
+  return order;
+}
+
+// One iteration of projection_update over `order` (common.rs:57-306), frame
+// by frame on the session stream; objectives reduced on the host afterwards.
+void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_frame, double* objective) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  const uint64_t N = m.N();
+  launch_flag_reset(s);
+  if (!s.cfg.has_pupil_step) refresh_max_pupil(s);
+  std::vector<double> fw(n, 0.0);
+  std::vector<bool> ran(n, false);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t f = order[i];
+    if (f >= m.F) fail(FPMHIP_FRAME_OUT_OF_RANGE, "frame index out of range");
+    const double w = me.weight[f];
+    fw[i] = w;
+    if (w == 0.0) continue;
+    const double gain = s.has_gain ? s.gain[f] : 1.0;
+    if (!std::isfinite(gain) || gain <= 0.0) fail(FPMHIP_INVALID_MODEL, "state frame has invalid gain");
+    uint64_t e0 = f, e1 = f + 1;
+    if (m.mux) {
+      e0 = m.row_ptr[f];
+      e1 = m.row_ptr[f + 1];
+    }
+    const uint32_t K = (uint32_t)(e1 - e0);
+    double sw_sum = 0.0;
+    if (m.mux)
+      for (uint64_t e = e0; e < e1; ++e) sw_sum += m.mw[e];
+    else
+      sw_sum = 1.0;
+    if (s.cfg.has_pupil_step) refresh_max_pupil(s);
+    for (uint32_t k = 0; k < K; ++k) launch_extract(s, m.mux ? m.msrc[e0 + k] : f, k);
+    if (s.precision == FPMHIP_PLAIN_C128)
+      fft2_f64(*m.fft_low, s.b.field.as<double2>(), K, true, s.ctx->stream);
+    else
+      fft2_f32(*m.fft_low, s.b.field.as<float2>(), K, true, s.ctx->stream);
+    launch_project(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, sw_sum, gain, i);
+    ran[i] = true;
+    for (uint32_t k = 0; k < K; ++k) {
+      const uint64_t src = m.mux ? m.msrc[e0 + k] : f;
+      const double swk = m.mux ? m.mw[e0 + k] : 1.0;
+      if (k > 0) launch_difference_frame(s, k, f);
+      const size_t vb = s.vbytes();
+      if (s.precision == FPMHIP_PLAIN_C128)
+        fft2_f64(*m.fft_low, reinterpret_cast<double2*>(s.b.field.as<char>() + k * N * vb), 1, false, s.ctx->stream);
+      else
+        fft2_f32(*m.fft_low, reinterpret_cast<float2*>(s.b.field.as<char>() + k * N * vb), 1, false, s.ctx->stream);
+      if (s.cfg.has_pupil_step)
+        launch_max_abs2(s, s.b.patch.as<char>() + k * N * vb, s.precision == FPMHIP_PLAIN_C128, N, 1);
+      const double nsw = swk / sw_sum;
+      launch_update(s, k, nsw, k == 0);
+      launch_insert(s, src, k, w * s.cfg.object_step * nsw);
+    }
+    if (s.cfg.has_pupil_step) launch_pupil_step(s, w * s.cfg.pupil_step);
+    if (s.cfg.momentum) {  // apply_object_momentum (common.rs:308-353)
+      s.counter += 1;
+      if (s.counter >= s.cfg.interval) {
+        if (s.counter != s.cfg.interval)
+          fail(FPMHIP_INVALID_MODEL, "mPIE frame counter exceeds the configured momentum interval");
+        launch_momentum(s);
+        s.counter = 0;
+      }
+    }
+  }
+  std::vector<double> partials(n * s.nblk * 2);
+  if (n)
+    HIP_CHECK(hipMemcpyAsync(partials.data(), s.b.partials.ptr, partials.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s.ctx->stream));
+  check_flags(s);
+  double osum = 0.0, wsum = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    double loss = 0.0;
+    if (ran[i]) {
+      double ls = 0.0, lc = 0.0;
+      for (uint32_t b = 0; b < s.nblk; ++b) {
+        ls += partials[(i * s.nblk + b) * 2];
+        lc += partials[(i * s.nblk + b) * 2 + 1];
+      }
+      if (lc == 0.0) fail(FPMHIP_INVALID_MEASUREMENTS, "frame has no unmasked pixels");
+      loss = ls / lc;
+    }
+    if (per_frame) per_frame[i] = loss;
+    osum += fw[i] * loss;  // StepSummary::push_frame (metrics.rs:25-31)
+    wsum += fw[i];
+  }
+  if (s.cfg.gauge) canonicalize_session(s);
+  if (objective) *objective = wsum > 0.0 ? osum / wsum : std::numeric_limits<double>::quiet_NaN();
+}
+
+}  // namespace
+
+extern "C" {
+
+int fpmhip_context_create(int32_t device, fpmhip_context** out) {
+  return guarded([&] {
+    require(out != nullptr, "context", "output is null");
+    int count = 0;
+    HIP_CHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) fail(FPMHIP_DEVICE_ERROR, "no such HIP device");
+    auto c = std::make_unique<fpmhip_context>();
+    c->device = device;
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipGetDeviceProperties(&c->props, device));
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+  });
+}
+
+int fpmhip_context_destroy(fpmhip_context* c) {
+  return guarded([&] {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+  });
+}
+
+int fpmhip_context_synchronize(fpmhip_context* c) {
+  return guarded([&] {
+    require(c != nullptr, "context", "context is null");
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int fpmhip_fft2_host(fpmhip_context* c, double* values, uint64_t h, uint64_t w, int32_t direction) {
+  return guarded([&] {
+    require(c && values, "values", "null argument");
+    if (h == 0 || w == 0) fail(FPMHIP_INVALID_SHAPE, "FFT dimensions must be non-zero");
+    set_device(c);
+    auto plan = make_fft2((uint32_t)h, (uint32_t)w);
+    DeviceBuffer d(h * w * sizeof(double2));
+    HIP_CHECK(hipMemcpyAsync(d.ptr, values, d.bytes, hipMemcpyHostToDevice, c->stream));
+    fft2_f64(*plan, d.as<double2>(), 1, direction == FPMHIP_INVERSE, c->stream);
+    HIP_CHECK(hipMemcpyAsync(values, d.ptr, d.bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+// ImagePlaneModel::validate (image_plane_fpm.rs:841-972) + upload
+int fpmhip_model_create(fpmhip_context* c, const fpmhip_model_desc* d, fpmhip_model** out) {
+  return guarded([&] {
+    require(c && d && out, "model", "null argument");
+    set_device(c);
+    auto m = std::make_unique<fpmhip_model>();
+    m->ctx = c;
+    if (d->image_h == 0 || d->image_w == 0 || d->source_count == 0)
+      fail(FPMHIP_INVALID_MODEL, "source k-vectors and crops; counts must be equal and non-zero");
+    if (d->recon_h < d->image_h || d->recon_w < d->image_w)
+      fail(FPMHIP_INVALID_MODEL, "reconstruction shape must contain a low-resolution crop");
+    if (d->image_h > 4096 || d->image_w > 4096 || d->recon_h > 65536 || d->recon_w > 65536)
+      fail(FPMHIP_UNSUPPORTED, "grid is larger than this build supports");
+    m->nh = (uint32_t)d->image_h;
+    m->nw = (uint32_t)d->image_w;
+    m->rh = (uint32_t)d->recon_h;
+    m->rw = (uint32_t)d->recon_w;
+    m->S = d->source_count;
+    m->mux = d->mux_row_ptr != nullptr;
+    m->F = m->mux ? d->frame_count : d->source_count;
+    if (m->F == 0) fail(FPMHIP_INVALID_MODEL, "multiplexing matrix must contain at least one measured frame");
+    const uint64_t N = m->N();
+    std::vector<SrcParam> sp(m->S);
+    for (uint64_t s = 0; s < m->S; ++s) {
+      const uint64_t r0 = d->crop_start[2 * s], c0 = d->crop_start[2 * s + 1];
+      if (r0 + m->nh > m->rh || c0 + m->nw > m->rw)
+        fail(FPMHIP_INVALID_MODEL, "crop lies outside reconstruction shape");
+      const double orow = d->offset ? d->offset[2 * s] : 0.0, ocol = d->offset ? d->offset[2 * s + 1] : 0.0;
+      m->crop_r.push_back(r0);
+      m->crop_c.push_back(c0);
+      m->off_r.push_back(orow);
+      m->off_c.push_back(ocol);
+      const Axis ar = interpolation_axis(r0, m->nh, m->rh, orow);
+      const Axis ac = interpolation_axis(c0, m->nw, m->rw, ocol);
+      sp[s] = SrcParam{r0, c0, ar, ac, off_zero(orow, ocol) ? 1 : 0, 0};
+    }
+    m->pupil.assign(d->pupil, d->pupil + 2 * N);
+    for (double v : m->pupil)
+      if (!std::isfinite(v)) fail(FPMHIP_INVALID_MODEL, "pupil support or numeric values are invalid");
+    m->support.assign(d->support, d->support + N);
+    for (uint8_t v : m->support) {
+      if (v > 1) fail(FPMHIP_INVALID_MODEL, "pupil support values must be exactly zero or one");
+      m->support_count += v;
+    }
+    std::vector<double> gains(m->F, 1.0);
+    if (d->frame_gain) {
+      m->has_gain = true;
+      m->gain.assign(d->frame_gain, d->frame_gain + m->F);
+      for (double g : m->gain)
+        if (!std::isfinite(g) || g < 0.0) fail(FPMHIP_INVALID_MODEL, "frame gains must be finite and non-negative");
+      gains = m->gain;
+    }
+    if (d->background && d->background_len) {
+      if (d->background_len != N && d->background_len != N * m->F)
+        fail(FPMHIP_INVALID_MODEL, "background must be one image or one image per frame");
+      m->background.assign(d->background, d->background + d->background_len);
+      for (double v : m->background)
+        if (!std::isfinite(v)) fail(FPMHIP_INVALID_MODEL, "background values must be finite");
+    }
+    if (m->mux) {
+      m->row_ptr.assign(d->mux_row_ptr, d->mux_row_ptr + m->F + 1);
+      const uint64_t nnz = m->row_ptr[m->F];
+      m->msrc.assign(d->mux_source, d->mux_source + nnz);
+      m->mw.assign(d->mux_weight, d->mux_weight + nnz);
+      for (uint64_t f = 0; f < m->F; ++f) {
+        std::vector<bool> seen(m->S, false);
+        bool bad = m->row_ptr[f + 1] <= m->row_ptr[f];
+        for (uint64_t e = m->row_ptr[f]; e < m->row_ptr[f + 1] && !bad; ++e) {
+          const uint64_t s = m->msrc[e];
+          bad = s >= m->S || !std::isfinite(m->mw[e]) || m->mw[e] < 0.0 || seen[s];
+          if (s < m->S) seen[s] = true;
+        }
+        if (bad)
+          fail(FPMHIP_INVALID_MODEL, "multiplexing row is empty or contains a duplicate/invalid source or weight");
+      }
+      std::vector<uint32_t> rp(m->row_ptr.begin(), m->row_ptr.end()), ms(m->msrc.begin(), m->msrc.end());
+      upload(m->d_mux_ptr, rp.data(), rp.size(), c->stream);
+      upload(m->d_mux_src, ms.data(), ms.size(), c->stream);
+      upload(m->d_mux_w, m->mw.data(), m->mw.size(), c->stream);
+    }
+    if (d->k_vectors) {
+      m->has_k = true;
+      for (uint64_t s = 0; s < m->S; ++s) {
+        m->kx.push_back(d->k_vectors[2 * s]);
+        m->ky.push_back(d->k_vectors[2 * s + 1]);
+      }
+    }
+    upload(m->d_src, sp.data(), sp.size(), c->stream);
+    upload(m->d_gains, gains.data(), gains.size(), c->stream);
+    upload(m->d_pupil64, m->pupil.data(), m->pupil.size(), c->stream);
+    upload(m->d_support, m->support.data(), m->support.size(), c->stream);
+    if (!m->background.empty()) upload(m->d_background, m->background.data(), m->background.size(), c->stream);
+    m->fft_low = make_fft2(m->nh, m->nw);
+    m->fft_high = make_fft2(m->rh, m->rw);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = m.release();
+  });
+}
+
+int fpmhip_model_destroy(fpmhip_model* m) {
+  return guarded([&] {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    delete m;
+  });
+}
+
+// MeasurementStack::from_vec validation (src/measurements/stack.rs:60-105)
+int fpmhip_measurements_create(fpmhip_context* c, const fpmhip_model* m, const uint16_t* counts,
+                               const double* frames, const double* weights, const uint8_t* masks,
+                               fpmhip_measurements** out) {
+  return guarded([&] {
+    require(c && m && out, "measurements", "null argument");
+    if ((counts == nullptr) == (frames == nullptr))
+      fail(FPMHIP_INVALID_MEASUREMENTS, "exactly one of uint16 counts and f64 frames is required");
+    set_device(c);
+    auto me = std::make_unique<fpmhip_measurements>();
+    me->ctx = c;
+    me->model = m;
+    me->F = m->F;
+    me->N = m->N();
+    const uint64_t FN = me->F * me->N;
+    me->weight.assign(me->F, 1.0);
+    if (weights)
+      for (uint64_t f = 0; f < me->F; ++f) {
+        if (!std::isfinite(weights[f]) || weights[f] < 0.0)
+          fail(FPMHIP_INVALID_MEASUREMENTS, "frame weights must be finite and non-negative");
+        me->weight[f] = weights[f];
+      }
+    if (counts) {
+      me->is_u16 = true;
+      me->counts_host.assign(counts, counts + FN);
+      upload(me->d_frames, counts, FN, c->stream);
+    } else {
+      for (uint64_t i = 0; i < FN; ++i)
+        if (!std::isfinite(frames[i])) fail(FPMHIP_INVALID_MEASUREMENTS, "measurements must be finite");
+      me->frames_host.assign(frames, frames + FN);
+      upload(me->d_frames, frames, FN, c->stream);
+    }
+    if (masks) {
+      me->masks.assign(masks, masks + FN);
+      for (uint64_t f = 0; f < me->F; ++f) {
+        uint64_t valid = 0;
+        for (uint64_t p = 0; p < me->N; ++p) valid += masks[f * me->N + p] != 0;
+        if (valid == 0 && me->weight[f] > 0.0)
+          fail(FPMHIP_INVALID_MEASUREMENTS, "a positive-weight frame has no unmasked pixels");
+      }
+      upload(me->d_masks, masks, FN, c->stream);
+    }
+    upload(me->d_weights, me->weight.data(), me->F, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = me.release();
+  });
+}
+
+int fpmhip_measurements_destroy(fpmhip_measurements* me) {
+  return guarded([&] {
+    if (!me) return;
+    (void)hipSetDevice(me->ctx->device);
+    delete me;
+  });
+}
+
+int fpmhip_forward_stack(fpmhip_context* c, const fpmhip_model* m, const double* spectrum, const double* pupil,
+                         double* out) {
+  return guarded([&] {
+    require(c && m && spectrum && out, "forward_stack", "null argument");
+    set_device(c);
+    for (uint64_t f = 0; f < m->F; ++f) {
+      const double g = m->has_gain ? m->gain[f] : 1.0;
+      if (!std::isfinite(g) || g <= 0.0) fail(FPMHIP_INVALID_MODEL, "frame has invalid gain");
+    }
+    const uint64_t N = m->N(), M = m->M();
+    DeviceBuffer ds(M * sizeof(double2)), dp, fields(m->S * N * sizeof(double2)), dout(m->F * N * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(ds.ptr, spectrum, ds.bytes, hipMemcpyHostToDevice, c->stream));
+    const double2* pp = m->d_pupil64.as<double2>();
+    if (pupil) {
+      dp.allocate(N * sizeof(double2));
+      HIP_CHECK(hipMemcpyAsync(dp.ptr, pupil, dp.bytes, hipMemcpyHostToDevice, c->stream));
+      pp = dp.as<double2>();
+    }
+    launch_forward_stack(*m, ds.as<double2>(), pp, fields.as<double2>(), dout.as<double>(), c->stream);
+    HIP_CHECK(hipMemcpyAsync(out, dout.ptr, dout.bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, int32_t precision,
+                          const fpmhip_algorithm* alg, fpmhip_session** out) {
+  return guarded([&] {
+    require(m && me && alg && out, "session", "null argument");
+    if (me->model != m && (me->F != m->F || me->N != m->N()))
+      fail(FPMHIP_INVALID_MEASUREMENTS, "measurements do not match the model");
+    if (precision != FPMHIP_PLAIN_C128 && precision != FPMHIP_MIXED_F32)
+      fail(FPMHIP_INVALID_PARAMETER, "unknown precision", "precision");
+    if (int rc = fpmhip_algorithm_validate(alg)) throw Failure{rc};
+    if (alg->kind == FPMHIP_EPRY && (alg->recover_gain || alg->recover_background))
+      fail(FPMHIP_UNSUPPORTED, "EPRY gain/background recovery is not implemented on the device path");
+    set_device(m->ctx);
+    auto s = std::make_unique<fpmhip_session>();
+    s->ctx = m->ctx;
+    s->model = m;
+    s->meas = me;
+    s->precision = precision;
+    s->alg = *alg;
+    s->cfg = make_config(*alg);
+    s->has_gain = m->has_gain;
+    s->gain = m->gain;
+    uint32_t kmax = 1;
+    if (m->mux)
+      for (uint64_t f = 0; f < m->F; ++f) kmax = std::max<uint32_t>(kmax, (uint32_t)(m->row_ptr[f + 1] - m->row_ptr[f]));
+    s->kmax = kmax;
+    const uint64_t N = m->N(), M = m->M();
+    s->nblk = (uint32_t)((N + 255) / 256);
+    const size_t vb = s->vbytes();
+    SessionBuffers& b = s->b;
+    b.spec.allocate(M * sizeof(double2));
+    if (s->cfg.momentum) {
+      b.vel.allocate(M * sizeof(double2));
+      b.anc.allocate(M * sizeof(double2));
+    }
+    b.pupil.allocate(N * vb);
+    b.patch.allocate(kmax * N * vb);
+    b.field.allocate(kmax * N * vb);
+    b.pred.allocate(N * sizeof(double));
+    b.proj.allocate(N * vb);
+    b.U.allocate(N * vb);
+    b.grad.allocate(N * vb);
+    b.scal.allocate(8 * sizeof(double));
+    b.partials.allocate(std::max<uint64_t>(m->F, 1) * s->nblk * 2 * sizeof(double));
+    b.flag.allocate(4 * sizeof(int));
+    b.work_m.allocate(std::max(M, N) * sizeof(double2));
+    b.avg.allocate(N * sizeof(double));
+    b.aw.allocate(N * sizeof(double));
+    launch_convert_pupil(m->d_pupil64.as<double2>(), b.pupil.ptr, precision == FPMHIP_MIXED_F32, N, m->ctx->stream);
+    launch_flag_reset(*s);
+    HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
+    *out = s.release();
+  });
+}
+
+int fpmhip_session_destroy(fpmhip_session* s) {
+  return guarded([&] {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    delete s;
+  });
+}
+
+int fpmhip_session_initialize(fpmhip_session* s) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    set_device(s->ctx);
+    for (uint64_t f = 0; f < s->model->F; ++f) {
+      const double g = s->model->has_gain ? s->model->gain[f] : 1.0;
+      if (!std::isfinite(g) || g <= 0.0) fail(FPMHIP_INVALID_MODEL, "frame has invalid gain");
+    }
+    launch_flag_reset(*s);
+    launch_init(*s);
+    launch_convert_pupil(s->model->d_pupil64.as<double2>(), s->b.pupil.ptr, s->precision == FPMHIP_MIXED_F32,
+                         s->model->N(), s->ctx->stream);
+    s->has_gain = s->model->has_gain;
+    s->gain = s->model->gain;
+    check_flags(*s);
+    init_aux(*s);
+    refresh_max_pupil(*s);
+    if (s->cfg.gauge) canonicalize_session(*s);
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    s->initialized = true;
+  });
+}
+
+int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const double* pupil, const double* velocity,
+                             const double* anchor, uint64_t counter, const double* frame_gains,
+                             const double* background, uint64_t background_len) {
+  return guarded([&] {
+    require(s && spectrum && pupil, "state", "null argument");
+    if (background) fail(FPMHIP_UNSUPPORTED, "a restored background must equal the model background on the device path");
+    set_device(s->ctx);
+    const uint64_t M = s->model->M();
+    HIP_CHECK(hipMemcpyAsync(s->b.spec.ptr, spectrum, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+    upload_pupil64(*s, pupil);
+    if (frame_gains) {
+      s->has_gain = true;
+      s->gain.assign(frame_gains, frame_gains + s->model->F);
+    }
+    if (s->cfg.momentum) {
+      if (velocity && anchor) {
+        if (counter >= s->cfg.interval)
+          fail(FPMHIP_INVALID_MODEL, "mPIE frame counter exceeds the configured momentum interval");
+        HIP_CHECK(hipMemcpyAsync(s->b.vel.ptr, velocity, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(s->b.anc.ptr, anchor, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+        s->counter = counter;
+      } else {
+        init_aux(*s);
+      }
+    }
+    (void)background_len;
+    refresh_max_pupil(*s);
+    if (s->cfg.gauge) canonicalize_session(*s);
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    s->initialized = true;
+  });
+}
+
+int fpmhip_session_sweep(fpmhip_session* s, const uint64_t* order, uint64_t order_len, double* per_frame,
+                         double* objective) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    if (!s->initialized) fail(FPMHIP_INVALID_MODEL, "session state is not initialized");
+    set_device(s->ctx);
+    std::vector<uint64_t> seq;
+    if (!order) {
+      seq.resize(s->model->F);
+      std::iota(seq.begin(), seq.end(), 0);
+      order = seq.data();
+      order_len = seq.size();
+    }
+    if (order_len > s->model->F) {
+      // partial buffers hold one slot per frame of an iteration
+      fail(FPMHIP_INVALID_PARAMETER, "order is longer than the frame count", "order");
+    }
+    sweep(*s, order, order_len, per_frame, objective);
+  });
+}
+
+int fpmhip_session_schedule(fpmhip_session* s, int32_t schedule, uint64_t* order) {
+  return guarded([&] {
+    require(s && order, "schedule", "null argument");
+    std::vector<uint64_t> o = schedule_order(*s, schedule);
+    std::copy(o.begin(), o.end(), order);
+  });
+}
+
+int fpmhip_session_run(fpmhip_session* s, uint64_t iterations, int32_t schedule, double* trace) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    set_device(s->ctx);
+    if (!s->initialized) {
+      if (int rc = fpmhip_session_initialize(s)) throw Failure{rc};
+    }
+    std::vector<uint64_t> order = schedule_order(*s, schedule);
+    for (uint64_t it = 0; it < iterations; ++it) {
+      double obj = 0.0;
+      sweep(*s, order.data(), order.size(), nullptr, &obj);
+      if (trace) trace[it] = obj;
+    }
+  });
+}
+
+int fpmhip_session_get_spectrum(fpmhip_session* s, double* out) {
+  return guarded([&] {
+    require(s && out, "spectrum", "null argument");
+    set_device(s->ctx);
+    HIP_CHECK(hipMemcpyAsync(out, s->b.spec.ptr, s->model->M() * sizeof(double2), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int fpmhip_session_get_pupil(fpmhip_session* s, double* out) {
+  return guarded([&] {
+    require(s && out, "pupil", "null argument");
+    set_device(s->ctx);
+    download_pupil64(*s, out);
+  });
+}
+
+int fpmhip_session_get_object(fpmhip_session* s, double* out) {
+  return guarded([&] {
+    require(s && out, "object", "null argument");
+    set_device(s->ctx);
+    launch_object(*s, s->b.work_m.as<double2>());
+    HIP_CHECK(hipMemcpyAsync(out, s->b.work_m.ptr, s->model->M() * sizeof(double2), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int fpmhip_session_get_momentum(fpmhip_session* s, double* velocity, double* anchor, uint64_t* counter) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    if (!s->cfg.momentum) fail(FPMHIP_UNSUPPORTED, "session has no mPIE momentum state");
+    set_device(s->ctx);
+    const size_t bytes = s->model->M() * sizeof(double2);
+    if (velocity) HIP_CHECK(hipMemcpyAsync(velocity, s->b.vel.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+    if (anchor) HIP_CHECK(hipMemcpyAsync(anchor, s->b.anc.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    if (counter) *counter = s->counter;
+  });
+}
+
+int fpmhip_session_get_calibration(fpmhip_session* s, double* frame_gains, double* background,
+                                   uint64_t* background_len) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    const Model& m = *s->model;
+    if (frame_gains)
+      for (uint64_t f = 0; f < m.F; ++f) frame_gains[f] = s->has_gain ? s->gain[f] : 1.0;
+    if (background) std::copy(m.background.begin(), m.background.end(), background);
+    if (background_len) *background_len = m.background.size();
+  });
+}
+
+int fpmhip_session_shape(const fpmhip_session* s, uint64_t* nh, uint64_t* nw, uint64_t* rh, uint64_t* rw,
+                         uint64_t* F) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    if (nh) *nh = s->model->nh;
+    if (nw) *nw = s->model->nw;
+    if (rh) *rh = s->model->rh;
+    if (rw) *rw = s->model->rw;
+    if (F) *F = s->model->F;
+  });
+}
+
+}  // extern "C"

@@ -133,6 +133,9 @@ struct Model {
   bool has_k = false;
   std::vector<double> kx, ky;
   DeviceBuffer d_pupil64, d_pupil32, d_support, d_background;
+  DeviceBuffer d_src;                          // SrcParam[S]
+  DeviceBuffer d_gains;                        // double[F] (ones when absent)
+  DeviceBuffer d_mux_w, d_mux_src, d_mux_ptr;  // CSR multiplexing (double, uint32, uint32)
   std::unique_ptr<Fft2> fft_low, fft_high;
   uint64_t N() const { return (uint64_t)nh * nw; }
   uint64_t M() const { return (uint64_t)rh * rw; }
@@ -149,6 +152,7 @@ struct Measurements {
   std::vector<double> frames_host;
   DeviceBuffer d_frames;  // uint16_t[F*N] or double[F*N]
   DeviceBuffer d_masks;   // uint8_t[F*N] or empty
+  DeviceBuffer d_weights; // double[F]
   double value(uint64_t f, uint64_t p) const {
     return is_u16 ? (double)counts_host[f * N + p] : frames_host[f * N + p];
   }

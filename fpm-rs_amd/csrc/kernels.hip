@@ -1,0 +1,562 @@
+// kernels.hip — per-stage HIP kernels of one LED update (projection_update,
+// src/algorithms/common.rs:57-306), state initialisation (state.rs:280-355),
+// mPIE momentum (common.rs:355-387), the result transform (result.rs:397-413)
+// and the forward stack (forward.rs:245-372).
+//
+// Templated on the session precision R: PLAIN_C128 keeps every buffer in
+// complex128; MIXED_F32 keeps patches, fields, pupil and updates in complex64
+// and runs the FFTs in FP32, while the object spectrum, every pointwise
+// formula and all reductions stay FP64. Pointwise formulas follow the
+// reference term by term (num-complex operation order; no FMA contraction).
+#include "session.h"
+
+namespace fpmhip {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline double meas_value(const void* frames, int is_u16, uint64_t idx) {
+  return is_u16 ? (double)static_cast<const uint16_t*>(frames)[idx] : static_cast<const double*>(frames)[idx];
+}
+
+__device__ inline double bg_value(const double* bg, uint64_t bg_len, uint64_t N, uint64_t f, uint64_t p) {
+  if (bg_len == 0) return 0.0;
+  return bg[bg_len == N ? p : f * N + p];
+}
+
+// point_loss (src/algorithms/objective.rs:47-71)
+__device__ inline double point_loss(double predicted, double measured, int loss) {
+  if (loss == FPMHIP_LOSS_INTENSITY_MSE) {
+    double r = predicted - measured;
+    return r * r;
+  }
+  if (loss == FPMHIP_LOSS_POISSON_NLL) {
+    double mean = fmax(predicted, 1e-12);
+    return mean - fmax(measured, 0.0) * log(mean);
+  }
+  double r = sqrt(fmax(predicted, 0.0)) - sqrt(fmax(measured, 0.0));
+  if (loss == FPMHIP_LOSS_HUBER_AMPLITUDE) {
+    double a = fabs(r);
+    return a <= 1.0 ? 0.5 * r * r : a - 0.5;
+  }
+  return r * r;
+}
+
+template <int BLOCK>
+__device__ inline double block_sum(double v, double* sh) {
+  // fixed-order tree reduction: deterministic for a given launch shape
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+using D = cx<double>;
+
+// bilinear extract (crop.rs:129-167) of one source, times pupil, ifftshift
+// (forward.rs:538-549) into the field buffer.
+template <class V>
+__global__ void k_extract(const double2* __restrict__ spec, uint32_t rw, const SrcParam* __restrict__ src,
+                          uint64_t source, uint32_t h, uint32_t w, const V* __restrict__ pupil,
+                          V* __restrict__ patch, V* __restrict__ field) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= h * w) return;
+  const SrcParam sp = src[source];
+  const uint32_t r = p / w, c = p % w;
+  double2 v;
+  if (sp.zero) {
+    v = spec[(sp.sr + r) * rw + sp.sc + c];
+  } else {
+    const uint64_t lr = sp.ar.lower + r, ur = sp.ar.upper + r, lc = sp.ac.lower + c, uc = sp.ac.upper + c;
+    const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+                 wuu = sp.ar.wu * sp.ac.wu;
+    v = D::mulr(spec[lr * rw + lc], wll);
+    v = D::add(v, D::mulr(spec[lr * rw + uc], wlu));
+    v = D::add(v, D::mulr(spec[ur * rw + lc], wul));
+    v = D::add(v, D::mulr(spec[ur * rw + uc], wuu));
+  }
+  const V pv = from_d2<V>(v);
+  patch[p] = pv;
+  const double2 e = D::mul(to_d2(pv), to_d2(pupil[p]));
+  field[((r + (h + 1) / 2) % h) * w + (c + (w + 1) / 2) % w] = from_d2<V>(e);
+}
+
+// predicted intensity, objective terms and amplitude ratio (common.rs:106-196);
+// the first mode's difference field (:219-228) is written in place.
+template <class V>
+__global__ void __launch_bounds__(kBlock) k_project(V* __restrict__ fields, uint32_t K, const double* __restrict__ mode_w,
+                                                    double sw_sum, uint32_t N, const void* __restrict__ frames,
+                                                    int is_u16, uint64_t frame, const uint8_t* __restrict__ masks,
+                                                    double gain, const double* __restrict__ bg, uint64_t bg_len,
+                                                    double eps, int loss, double* __restrict__ pred_out,
+                                                    V* __restrict__ proj_out, double* __restrict__ partials) {
+  __shared__ double sh[kBlock];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  double lsum = 0.0, lcnt = 0.0;
+  if (p < N) {
+    double pred = 0.0;
+    for (uint32_t k = 0; k < K; ++k) {
+      const double2 f = to_d2(fields[(uint64_t)k * N + p]);
+      pred += (mode_w ? mode_w[k] : 1.0) * D::norm2(f);
+    }
+    const bool masked = masks && masks[frame * N + p] == 0;
+    double2 proj = make_double2(1.0, 0.0);
+    if (!masked) {
+      const double b = bg_value(bg, bg_len, N, frame, p);
+      const double y = meas_value(frames, is_u16, frame * N + p);
+      lsum = point_loss(gain * pred + b, y, loss);
+      lcnt = 1.0;
+      const double target = fmax((y - b) / gain, 0.0);
+      proj = pred > eps ? make_double2(sqrt(target / pred), 0.0) : make_double2(0.0, sqrt(target / sw_sum));
+    }
+    pred_out[p] = pred;
+    if (K > 1) proj_out[p] = from_d2<V>(proj);
+    const double2 f0 = to_d2(fields[p]);
+    double2 d = masked ? f0 : (pred > eps ? D::mulr(f0, proj.x) : make_double2(proj.y, 0.0));
+    fields[p] = from_d2<V>(d);
+  }
+  const double s = block_sum<kBlock>(lsum, sh);
+  const double c = block_sum<kBlock>(lcnt, sh);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = s;
+    partials[2 * blockIdx.x + 1] = c;
+  }
+}
+
+// difference field of a further multiplexed mode (common.rs:219-228)
+template <class V>
+__global__ void k_difference(V* __restrict__ field, const V* __restrict__ proj, const double* __restrict__ pred,
+                             uint32_t N, const uint8_t* __restrict__ mask, double eps) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const double2 f = to_d2(field[p]);
+  const double2 pr = to_d2(proj[p]);
+  const bool masked = mask && mask[p] == 0;
+  double2 d = masked ? f : (pred[p] > eps ? D::mulr(f, pr.x) : make_double2(pr.y, 0.0));
+  field[p] = from_d2<V>(d);
+}
+
+// fftshift of the projected spectrum, exit-wave difference, object update
+// U = conj(P) diff / denominator and the EPRY pupil gradient (common.rs:235-285)
+template <class V>
+__global__ void k_update(const V* __restrict__ fftdiff, const V* __restrict__ patch, const V* __restrict__ pupil,
+                         uint32_t h, uint32_t w, int denom, double alpha, double eps,
+                         const double* __restrict__ scal, double nsw, int pupil_step, int first_mode,
+                         V* __restrict__ U, V* __restrict__ grad) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= h * w) return;
+  const uint32_t r = p / w, c = p % w;
+  const double2 ps = to_d2(fftdiff[((r + (h + 1) / 2) % h) * w + (c + (w + 1) / 2) % w]);
+  const double2 pa = to_d2(patch[p]);
+  const double2 P = to_d2(pupil[p]);
+  const double2 diff = D::sub(ps, D::mul(pa, P));
+  const double local = D::norm2(P);
+  const double maxp = fmax(scal[0], eps);
+  double den;
+  if (denom == FPMHIP_DENOMINATOR_LOCAL)
+    den = local + eps;
+  else if (denom == FPMHIP_DENOMINATOR_GLOBAL)
+    den = maxp + eps;
+  else
+    den = (1.0 - alpha) * local + alpha * maxp + eps;
+  U[p] = from_d2<V>(D::divr(D::mul(D::conj(P), diff), den));
+  if (pupil_step) {
+    const double maxo = fmax(scal[1], eps);
+    const double2 g = D::divr(D::mul(D::scale(nsw, D::conj(pa)), diff), maxo + eps);
+    const double2 prev = first_mode ? make_double2(0.0, 0.0) : to_d2(grad[p]);
+    grad[p] = from_d2<V>(D::add(prev, g));
+  }
+}
+
+// exact adjoint of the bilinear extract (crop.rs:188-256), written as a
+// gather over the destination window: each destination pixel adds its taps
+// in the reference's order (source row-major; taps LL, LU, UL, UU), so no
+// two threads write the same element.
+template <class V>
+__global__ void k_insert(double2* __restrict__ spec, uint32_t rw, const SrcParam* __restrict__ src, uint64_t source,
+                         uint32_t h, uint32_t w, const V* __restrict__ U, double scale) {
+  const SrcParam sp = src[source];
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp.zero) {
+    if (q >= h * w) return;
+    const uint32_t r = q / w, c = q % w;
+    double2& d = spec[(sp.sr + r) * rw + sp.sc + c];
+    d = D::add(d, D::scale(scale, to_d2(U[q])));
+    return;
+  }
+  const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
+  const uint32_t wh = h + (fr ? 1 : 0), ww = w + (fc ? 1 : 0);
+  if (q >= wh * ww) return;
+  const int a = (int)(q / ww), b = (int)(q % ww);
+  double2& dst = spec[(sp.ar.lower + a) * (uint64_t)rw + sp.ac.lower + b];
+  double2 acc = dst;
+  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+               wuu = sp.ar.wu * sp.ac.wu;
+  auto tap = [&](int sr_, int sc_, double wt) {
+    if (sr_ >= 0 && sr_ < (int)h && sc_ >= 0 && sc_ < (int)w) {
+      const double2 v = D::scale(scale, to_d2(U[sr_ * w + sc_]));
+      acc = D::add(acc, D::mulr(v, wt));
+    }
+  };
+  if (fr && fc) {
+    tap(a - 1, b - 1, wuu);
+    tap(a - 1, b, wul);
+    tap(a, b - 1, wlu);
+    tap(a, b, wll);
+  } else if (fr) {
+    tap(a - 1, b, wul);
+    tap(a, b, wll);
+  } else if (fc) {
+    tap(a, b - 1, wlu);
+    tap(a, b, wll);
+  } else {
+    tap(a, b, wll);
+  }
+  dst = acc;
+}
+
+// max |v|^2 over n values (single block; max is order independent)
+template <class V>
+__global__ void __launch_bounds__(1024) k_max_abs2(const V* __restrict__ v, uint64_t n, double* __restrict__ out) {
+  __shared__ double sh[1024];
+  double m = 0.0;
+  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, D::norm2(to_d2(v[i])));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// P += w_f beta g; support projection (common.rs:287-300, pupil.rs:171-182)
+template <class V>
+__global__ void k_pupil_step(V* __restrict__ pupil, const V* __restrict__ grad, double step,
+                             const uint8_t* __restrict__ support, int constrain, uint32_t N) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double2 P = D::add(to_d2(pupil[p]), D::scale(step, to_d2(grad[p])));
+  if (constrain && support[p] == 0) P = make_double2(0.0, 0.0);
+  pupil[p] = from_d2<V>(P);
+}
+
+// apply_momentum_event (common.rs:355-387)
+__global__ void k_momentum(double2* __restrict__ obj, double2* __restrict__ vel, double2* __restrict__ anc,
+                           uint64_t n, double friction, double feedback, int* __restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 o = obj[i];
+  const double2 nv = D::add(D::scale(friction, vel[i]), D::sub(o, anc[i]));
+  const double2 no = feedback == 0.0 ? o : D::add(o, D::scale(feedback, nv));
+  if (!isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y)) flag[0] = 1;
+  vel[i] = nv;
+  obj[i] = no;
+  anc[i] = no;
+}
+
+__global__ void k_flag_reset(int* flag) {
+  if (threadIdx.x < 4) flag[threadIdx.x] = 0;
+}
+
+// weighted mean measured amplitude per low-resolution pixel (state.rs:288-313)
+__global__ void k_init_average(const void* __restrict__ frames, int is_u16, const double* __restrict__ weights,
+                               const uint8_t* __restrict__ masks, const double* __restrict__ gains,
+                               const double* __restrict__ bg, uint64_t bg_len, uint64_t F, uint32_t N,
+                               double* __restrict__ avg, double* __restrict__ aw) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double a = 0.0, w = 0.0;
+  for (uint64_t f = 0; f < F; ++f) {
+    const double fw = weights[f];
+    if (fw == 0.0) continue;
+    if (masks && masks[f * N + p] == 0) continue;
+    const double b = bg_value(bg, bg_len, N, f, p);
+    const double amp = sqrt(fmax((meas_value(frames, is_u16, f * N + p) - b) / gains[f], 0.0));
+    a += fw * amp;
+    w += fw;
+  }
+  avg[p] = a;
+  aw[p] = w;
+}
+
+// fallback amplitude (state.rs:314-326) and nearest-neighbour upsample (:327-336)
+__global__ void __launch_bounds__(1024) k_init_object(const double* __restrict__ avg, const double* __restrict__ aw,
+                                                      uint32_t nh, uint32_t nw, uint32_t rh, uint32_t rw,
+                                                      double2* __restrict__ object, int* __restrict__ flag) {
+  __shared__ double sa[1024], sw[1024];
+  const uint32_t N = nh * nw;
+  double ta = 0.0, tw = 0.0;
+  for (uint32_t p = threadIdx.x; p < N; p += blockDim.x) {
+    ta += avg[p];
+    tw += aw[p];
+  }
+  sa[threadIdx.x] = ta;
+  sw[threadIdx.x] = tw;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sw[threadIdx.x] += sw[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  const double total_w = sw[0];
+  const double fallback = total_w > 0.0 ? sa[0] / total_w : 0.0;
+  if (threadIdx.x == 0 && total_w == 0.0) flag[1] = 1;
+  const uint64_t M = (uint64_t)rh * rw;
+  for (uint64_t i = threadIdx.x; i < M; i += blockDim.x) {
+    const uint32_t r = (uint32_t)(i / rw), c = (uint32_t)(i % rw);
+    const uint32_t lr = (uint32_t)((uint64_t)r * nh / rh), lc = (uint32_t)((uint64_t)c * nw / rw);
+    const uint32_t lp = lr * nw + lc;
+    const double v = aw[lp] > 0.0 ? avg[lp] / aw[lp] : fallback;
+    object[i] = make_double2(v, 0.0);
+  }
+}
+
+__global__ void k_shift(const double2* __restrict__ src, double2* __restrict__ dst, uint32_t h, uint32_t w,
+                        uint32_t rs, uint32_t cs) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)h * w) return;
+  const uint32_t r = (uint32_t)(i / w), c = (uint32_t)(i % w);
+  dst[(uint64_t)((r + rs) % h) * w + (c + cs) % w] = src[i];
+}
+
+__global__ void k_fill_zero(double2* dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = make_double2(0.0, 0.0);
+}
+
+template <class V>
+__global__ void k_convert_from_d2(const double2* __restrict__ src, V* __restrict__ dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = from_d2<V>(src[i]);
+}
+template <class V>
+__global__ void k_convert_to_d2(const V* __restrict__ src, double2* __restrict__ dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = to_d2(src[i]);
+}
+
+// forward stack: every source's field (extract x pupil, ifftshift) ...
+__global__ void k_extract_all(const double2* __restrict__ spec, uint32_t rw, const SrcParam* __restrict__ src,
+                              uint32_t h, uint32_t w, const double2* __restrict__ pupil,
+                              double2* __restrict__ fields) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t s = blockIdx.y;
+  if (p >= h * w) return;
+  const SrcParam sp = src[s];
+  const uint32_t r = p / w, c = p % w;
+  double2 v;
+  if (sp.zero) {
+    v = spec[(sp.sr + r) * rw + sp.sc + c];
+  } else {
+    const uint64_t lr = sp.ar.lower + r, ur = sp.ar.upper + r, lc = sp.ac.lower + c, uc = sp.ac.upper + c;
+    const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+                 wuu = sp.ar.wu * sp.ac.wu;
+    v = D::mulr(spec[lr * rw + lc], wll);
+    v = D::add(v, D::mulr(spec[lr * rw + uc], wlu));
+    v = D::add(v, D::mulr(spec[ur * rw + lc], wul));
+    v = D::add(v, D::mulr(spec[ur * rw + uc], wuu));
+  }
+  fields[s * h * w + ((r + (h + 1) / 2) % h) * w + (c + (w + 1) / 2) % w] = D::mul(v, pupil[p]);
+}
+
+// ... then |field|^2 per frame (incoherent sum when multiplexed), x gain + background
+__global__ void k_intensity(const double2* __restrict__ fields, uint32_t N, const uint32_t* __restrict__ mux_ptr,
+                            const uint32_t* __restrict__ mux_src, const double* __restrict__ mux_w,
+                            const double* __restrict__ gains, const double* __restrict__ bg, uint64_t bg_len,
+                            double* __restrict__ out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t f = blockIdx.y;
+  if (p >= N) return;
+  double v;
+  if (mux_ptr) {
+    v = 0.0;
+    for (uint32_t e = mux_ptr[f]; e < mux_ptr[f + 1]; ++e) v += mux_w[e] * D::norm2(fields[(uint64_t)mux_src[e] * N + p]);
+  } else {
+    v = D::norm2(fields[f * N + p]);
+  }
+  out[f * N + p] = gains[f] * v + bg_value(bg, bg_len, N, f, p);
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---- launchers ------------------------------------------------------------------
+
+#define DISPATCH_V(s, EXPR)                          \
+  do {                                               \
+    if ((s).precision == FPMHIP_PLAIN_C128) {        \
+      using V = double2;                             \
+      EXPR;                                          \
+    } else {                                         \
+      using V = float2;                              \
+      EXPR;                                          \
+    }                                                \
+  } while (0)
+
+void launch_extract(const fpmhip_session& s, uint64_t source, uint32_t mode) {
+  const Model& m = *s.model;
+  const uint64_t N = m.N();
+  DISPATCH_V(s, hipLaunchKernelGGL(k_extract<V>, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.spec.as<double2>(), m.rw, m.d_src.as<SrcParam>(), source, m.nh,
+                                   m.nw, s.b.pupil.as<V>(), s.b.patch.as<V>() + mode * N,
+                                   s.b.field.as<V>() + mode * N));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_project(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev, double sw_sum,
+                    double gain, uint64_t slot) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  const uint64_t N = m.N();
+  DISPATCH_V(s, hipLaunchKernelGGL(k_project<V>, dim3(s.nblk), dim3(kBlock), 0, s.ctx->stream, s.b.field.as<V>(),
+                                   K, mode_w_dev, sw_sum, (uint32_t)N, me.d_frames.ptr, me.is_u16 ? 1 : 0, frame,
+                                   me.d_masks.as<uint8_t>(), gain, m.d_background.as<double>(),
+                                   (uint64_t)m.background.size(), s.cfg.epsilon, s.cfg.loss, s.b.pred.as<double>(),
+                                   s.b.proj.as<V>(), s.b.partials.as<double>() + slot * s.nblk * 2));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_update(const fpmhip_session& s, uint32_t mode, double nsw, bool first_mode) {
+  const Model& m = *s.model;
+  const uint64_t N = m.N();
+  DISPATCH_V(s, hipLaunchKernelGGL(k_update<V>, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.field.as<V>() + mode * N, s.b.patch.as<V>() + mode * N, s.b.pupil.as<V>(),
+                                   m.nh, m.nw, s.cfg.denom, s.cfg.alpha, s.cfg.epsilon, s.b.scal.as<double>(), nsw,
+                                   s.cfg.has_pupil_step ? 1 : 0, first_mode ? 1 : 0, s.b.U.as<V>(),
+                                   s.b.grad.as<V>()));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_insert(const fpmhip_session& s, uint64_t source, uint32_t mode, double scale) {
+  (void)mode;
+  const Model& m = *s.model;
+  const uint64_t window = (uint64_t)(m.nh + 1) * (m.nw + 1);
+  DISPATCH_V(s, hipLaunchKernelGGL(k_insert<V>, dim3(blocks_for(window)), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.spec.as<double2>(), m.rw, m.d_src.as<SrcParam>(), source, m.nh,
+                                   m.nw, s.b.U.as<V>(), scale));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_max_abs2(const fpmhip_session& s, const void* v, bool f64v, uint64_t n, uint32_t slot) {
+  if (f64v)
+    hipLaunchKernelGGL(k_max_abs2<double2>, dim3(1), dim3(1024), 0, s.ctx->stream, static_cast<const double2*>(v),
+                       n, s.b.scal.as<double>() + slot);
+  else
+    hipLaunchKernelGGL(k_max_abs2<float2>, dim3(1), dim3(1024), 0, s.ctx->stream, static_cast<const float2*>(v), n,
+                       s.b.scal.as<double>() + slot);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_pupil_step(const fpmhip_session& s, double step) {
+  const Model& m = *s.model;
+  const uint64_t N = m.N();
+  DISPATCH_V(s, hipLaunchKernelGGL(k_pupil_step<V>, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.pupil.as<V>(), s.b.grad.as<V>(), step, m.d_support.as<uint8_t>(),
+                                   s.cfg.constrain ? 1 : 0, (uint32_t)N));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_momentum(const fpmhip_session& s) {
+  const uint64_t M = s.model->M();
+  hipLaunchKernelGGL(k_momentum, dim3(blocks_for(M)), dim3(kBlock), 0, s.ctx->stream, s.b.spec.as<double2>(),
+                     s.b.vel.as<double2>(), s.b.anc.as<double2>(), M, s.cfg.friction, s.cfg.feedback,
+                     s.b.flag.as<int>());
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_flag_reset(const fpmhip_session& s) {
+  hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(64), 0, s.ctx->stream, s.b.flag.as<int>());
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_init(const fpmhip_session& s) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  const uint64_t N = m.N();
+  hipStream_t st = s.ctx->stream;
+  hipLaunchKernelGGL(k_init_average, dim3(blocks_for(N)), dim3(kBlock), 0, st, me.d_frames.ptr, me.is_u16 ? 1 : 0,
+                     me.d_weights.as<double>(), me.d_masks.as<uint8_t>(), m.d_gains.as<double>(),
+                     m.d_background.as<double>(), (uint64_t)m.background.size(), m.F, (uint32_t)N,
+                     s.b.avg.as<double>(), s.b.aw.as<double>());
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_init_object, dim3(1), dim3(1024), 0, st, s.b.avg.as<double>(), s.b.aw.as<double>(), m.nh,
+                     m.nw, m.rh, m.rw, s.b.work_m.as<double2>(), s.b.flag.as<int>());
+  HIP_CHECK(hipGetLastError());
+  fft2_f64(*m.fft_high, s.b.work_m.as<double2>(), 1, false, st);
+  launch_shift(s.b.work_m.as<double2>(), s.b.spec.as<double2>(), m.rh, m.rw, m.rh / 2, m.rw / 2, st);
+}
+
+void launch_object(const fpmhip_session& s, double2* out_dev) {
+  const Model& m = *s.model;
+  launch_shift(s.b.spec.as<double2>(), out_dev, m.rh, m.rw, (m.rh + 1) / 2, (m.rw + 1) / 2, s.ctx->stream);
+  fft2_f64(*m.fft_high, out_dev, 1, true, s.ctx->stream);
+}
+
+void launch_shift(const double2* src, double2* dst, uint32_t h, uint32_t w, uint32_t rs, uint32_t cs,
+                  hipStream_t stream) {
+  hipLaunchKernelGGL(k_shift, dim3(blocks_for((uint64_t)h * w)), dim3(kBlock), 0, stream, src, dst, h, w, rs, cs);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_convert_pupil(const double2* src, void* dst, bool to_f32, uint64_t n, hipStream_t stream) {
+  if (to_f32)
+    hipLaunchKernelGGL(k_convert_from_d2<float2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, src,
+                       static_cast<float2*>(dst), n);
+  else
+    hipLaunchKernelGGL(k_convert_from_d2<double2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, src,
+                       static_cast<double2*>(dst), n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream) {
+  if (from_f32)
+    hipLaunchKernelGGL(k_convert_to_d2<float2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream,
+                       static_cast<const float2*>(src), dst, n);
+  else
+    hipLaunchKernelGGL(k_convert_to_d2<double2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream,
+                       static_cast<const double2*>(src), dst, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(n)), dim3(kBlock), 0, stream, dst, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields, double* out,
+                          hipStream_t stream) {
+  const uint64_t N = m.N();
+  hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(N), (uint32_t)m.S), dim3(kBlock), 0, stream, spec, m.rw,
+                     m.d_src.as<SrcParam>(), m.nh, m.nw, pupil, fields);
+  HIP_CHECK(hipGetLastError());
+  fft2_f64(*m.fft_low, fields, (uint32_t)m.S, true, stream);
+  hipLaunchKernelGGL(k_intensity, dim3(blocks_for(N), (uint32_t)m.F), dim3(kBlock), 0, stream, fields,
+                     (uint32_t)N, m.mux ? m.d_mux_ptr.as<uint32_t>() : nullptr, m.d_mux_src.as<uint32_t>(),
+                     m.d_mux_w.as<double>(), m.d_gains.as<double>(), m.d_background.as<double>(),
+                     (uint64_t)m.background.size(), out);
+  HIP_CHECK(hipGetLastError());
+}
+
+// difference of mode `mode` of frame `frame` (multiplexed frames only)
+void launch_difference_frame(const fpmhip_session& s, uint32_t mode, uint64_t frame) {
+  const Model& m = *s.model;
+  const uint64_t N = m.N();
+  const uint8_t* mask = s.meas->d_masks.ptr ? s.meas->d_masks.as<uint8_t>() + frame * N : nullptr;
+  DISPATCH_V(s, hipLaunchKernelGGL(k_difference<V>, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.field.as<V>() + mode * N, s.b.proj.as<V>(), s.b.pred.as<double>(),
+                                   (uint32_t)N, mask, s.cfg.epsilon));
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace fpmhip

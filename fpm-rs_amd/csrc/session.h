@@ -1,0 +1,96 @@
+// session.h — reconstruction session state and the launchers of the
+// per-stage kernels (kernels.hip). Internal to libfpmhip.
+#pragma once
+
+#include "cx.h"
+#include "internal.h"
+
+namespace fpmhip {
+
+// Per-source crop parameters, uploaded once per model.
+struct SrcParam {
+  uint64_t sr, sc;   // crop origin
+  Axis ar, ac;       // interpolation axes (crop.rs:266-313)
+  int32_t zero;      // FourierOffset::is_zero (crop.rs:28)
+  int32_t pad;
+};
+
+
+// Internal configuration (UpdateConfiguration, common.rs:24-55).
+struct Config {
+  double object_step = 1.0;
+  bool has_pupil_step = false;
+  double pupil_step = 0.0;
+  double epsilon = 1e-10;
+  int loss = 0;
+  int denom = 0;
+  double alpha = 0.0;
+  bool constrain = true;
+  bool momentum = false;
+  uint64_t interval = 0;
+  double friction = 0.0, feedback = 0.0;
+  bool gauge = false;  // canonicalize_state after init and each iteration (EPRY)
+};
+Config make_config(const fpmhip_algorithm& a);
+
+struct SessionBuffers {
+  DeviceBuffer spec;      // double2[M] centred object spectrum (FP64 accumulation)
+  DeviceBuffer vel, anc;  // double2[M] mPIE velocity/anchor
+  DeviceBuffer pupil;     // V[N]
+  DeviceBuffer patch;     // V[K*N]
+  DeviceBuffer field;     // V[K*N]
+  DeviceBuffer pred;      // double[N]
+  DeviceBuffer proj;      // V[N]
+  DeviceBuffer U;         // V[N]
+  DeviceBuffer grad;      // V[N]
+  DeviceBuffer scal;      // double[8]: [0] max|P|^2, [1] max|patch|^2
+  DeviceBuffer partials;  // double[F * nblk * 2]
+  DeviceBuffer flag;      // int[4]
+  DeviceBuffer work_m;    // double2[M] (init / result scratch)
+  DeviceBuffer avg, aw;   // double[N] (init)
+};
+
+}  // namespace fpmhip
+
+struct fpmhip_session {
+  fpmhip::Context* ctx = nullptr;
+  const fpmhip::Model* model = nullptr;
+  const fpmhip::Measurements* meas = nullptr;
+  int precision = FPMHIP_PLAIN_C128;
+  fpmhip_algorithm alg{};
+  fpmhip::Config cfg;
+  bool initialized = false;
+  uint64_t counter = 0;
+  uint32_t kmax = 1;   // largest multiplexing row
+  uint32_t nblk = 0;   // blocks of the projection kernel
+  bool has_gain = false;
+  std::vector<double> gain;
+  fpmhip::SessionBuffers b;
+  size_t vbytes() const { return precision == FPMHIP_PLAIN_C128 ? sizeof(double2) : sizeof(float2); }
+};
+
+namespace fpmhip {
+
+// ---- launchers (kernels.hip) ------------------------------------------------
+// All take the precision of the session (double2 for PLAIN_C128, float2 for MIXED_F32).
+void launch_extract(const fpmhip_session& s, uint64_t source, uint32_t mode);
+void launch_project(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
+                    double sw_sum, double gain, uint64_t slot);
+void launch_difference_frame(const fpmhip_session& s, uint32_t mode, uint64_t frame);
+void launch_update(const fpmhip_session& s, uint32_t mode, double nsw, bool first_mode);
+void launch_insert(const fpmhip_session& s, uint64_t source, uint32_t mode, double scale);
+void launch_max_abs2(const fpmhip_session& s, const void* v, bool f64v, uint64_t n, uint32_t slot);
+void launch_pupil_step(const fpmhip_session& s, double step);
+void launch_momentum(const fpmhip_session& s);
+void launch_flag_reset(const fpmhip_session& s);
+void launch_init(const fpmhip_session& s);
+void launch_object(const fpmhip_session& s, double2* out_dev);
+void launch_shift(const double2* src, double2* dst, uint32_t h, uint32_t w, uint32_t rs, uint32_t cs,
+                  hipStream_t stream);
+void launch_convert_pupil(const double2* src, void* dst, bool to_f32, uint64_t n, hipStream_t stream);
+void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream);
+void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream);
+void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields,
+                          double* out, hipStream_t stream);
+
+}  // namespace fpmhip

@@ -1,0 +1,309 @@
+"""ctypes binding of libfpmhip.so (include/fpmhip.h) and the host-side mirror
+of fpm-rs's Python surface: compile_model, simulate's forward stack, and
+`Algorithm.run(problem)` (python/src/reconstruction.rs:1107 run_algorithm).
+
+There is no CPU fallback: a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .model import ImagePlaneModel, MeasurementStack, ReconstructionProblem
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfpmhip.so")
+
+EXPORTS = [
+    "fpmhip_last_error", "fpmhip_last_error_name", "fpmhip_version", "fpmhip_compile_planar",
+    "fpmhip_algorithm_default", "fpmhip_algorithm_validate", "fpmhip_context_create", "fpmhip_context_destroy",
+    "fpmhip_context_synchronize", "fpmhip_fft2_host", "fpmhip_model_create", "fpmhip_model_destroy",
+    "fpmhip_measurements_create", "fpmhip_measurements_destroy", "fpmhip_forward_stack", "fpmhip_session_create",
+    "fpmhip_session_destroy", "fpmhip_session_initialize", "fpmhip_session_set_state", "fpmhip_session_sweep",
+    "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
+    "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
+    "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
+]
+
+
+class FpmError(RuntimeError):
+    """An fpm_rs::Error variant returned through the C ABI."""
+
+    def __init__(self, code, message, name=""):
+        self.code = code
+        self.kind = abi.STATUS_NAMES.get(code, str(code))
+        self.message = message
+        self.name = name
+        super().__init__(f"{self.kind}: {message}" + (f" [{name}]" if name else ""))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libfpmhip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ct.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name).restype = ct.c_int
+        L.fpmhip_last_error.restype = ct.c_char_p
+        L.fpmhip_last_error_name.restype = ct.c_char_p
+        L.fpmhip_version.restype = ct.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        L = lib()
+        raise FpmError(code, L.fpmhip_last_error().decode(), L.fpmhip_last_error_name().decode())
+
+
+def _cd(a):
+    return a.ctypes.data_as(abi.P_dbl)
+
+
+def compile_model(optics, array, image_shape, reconstruction_shape="smooth") -> ImagePlaneModel:
+    """fpm_rs.compile_model (python/src/model.rs:163-172; default 'smooth'),
+    computed by libfpmhip's host code (no device needed)."""
+    if isinstance(reconstruction_shape, str):
+        policy, eh, ew = abi.SHAPE[reconstruction_shape], 0, 0
+    else:
+        policy = abi.SHAPE["exact"]
+        eh, ew = reconstruction_shape
+    oc, ac = optics.to_c(), array.to_c()
+    shape = abi.CompiledShape()
+    L = lib()
+    args = (ct.byref(oc), ct.byref(ac), ct.c_uint64(image_shape[0]), ct.c_uint64(image_shape[1]),
+            ct.c_int32(policy), ct.c_uint64(eh), ct.c_uint64(ew), ct.byref(shape))
+    check(L.fpmhip_compile_planar(*args, None, None, None, None, None, None))
+    S = shape.source_count
+    crop = np.zeros((S, 2), np.uint64)
+    off = np.zeros((S, 2), np.float64)
+    kv = np.zeros((S, 2), np.float64)
+    pupil = np.zeros(image_shape, np.complex128)
+    support = np.zeros(image_shape, np.uint8)
+    gains = np.zeros(S, np.float64)
+    check(L.fpmhip_compile_planar(*args, abi.ptr(crop, ct.c_uint64), _cd(off), _cd(kv), _cd(pupil),
+                                  abi.ptr(support, ct.c_uint8), _cd(gains)))
+    return ImagePlaneModel(tuple(image_shape), (shape.recon_h, shape.recon_w), crop, off, pupil, support,
+                           frame_gain=gains, k_vectors=kv)
+
+
+def algorithm_default(kind) -> abi.Algorithm:
+    a = abi.Algorithm()
+    check(lib().fpmhip_algorithm_default(ct.c_int32(kind), ct.byref(a)))
+    return a
+
+
+def validate(algorithm: abi.Algorithm):
+    check(lib().fpmhip_algorithm_validate(ct.byref(algorithm)))
+
+
+class Context:
+    """One HIP stream on one device (fpmhip_context)."""
+
+    def __init__(self, device=0):
+        self.handle = ct.c_void_p()
+        check(lib().fpmhip_context_create(ct.c_int32(device), ct.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_context_destroy(self.handle)
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().fpmhip_context_synchronize(self.handle))
+
+    def fft2(self, values, direction="forward"):
+        v = np.array(values, dtype=np.complex128, order="C", copy=True)
+        check(lib().fpmhip_fft2_host(self.handle, _cd(v), ct.c_uint64(v.shape[0]), ct.c_uint64(v.shape[1]),
+                                     ct.c_int32(abi.FORWARD if direction == "forward" else abi.INVERSE)))
+        return v
+
+
+_default_context = None
+
+
+def default_context() -> Context:
+    global _default_context
+    if _default_context is None:
+        _default_context = Context(0)
+    return _default_context
+
+
+class DeviceModel:
+    def __init__(self, ctx: Context, model: ImagePlaneModel):
+        self.ctx = ctx
+        self.model = model
+        self.handle = ct.c_void_p()
+        d = model.desc()
+        check(lib().fpmhip_model_create(ctx.handle, ct.byref(d), ct.byref(self.handle)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().fpmhip_model_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class DeviceMeasurements:
+    def __init__(self, ctx: Context, dmodel: DeviceModel, meas: MeasurementStack):
+        self.ctx = ctx
+        self.dmodel = dmodel
+        self.handle = ct.c_void_p()
+        counts = meas.counts
+        frames = meas.frames
+        w = meas.weights
+        m = meas.masks
+        check(lib().fpmhip_measurements_create(
+            ctx.handle, dmodel.handle, None if counts is None else abi.ptr(counts, ct.c_uint16),
+            None if frames is None else _cd(frames), None if w is None else _cd(w),
+            None if m is None else abi.ptr(m, ct.c_uint8), ct.byref(self.handle)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().fpmhip_measurements_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def forward_stack(model: ImagePlaneModel, spectrum, pupil=None, context: Optional[Context] = None,
+                  dmodel: Optional[DeviceModel] = None):
+    """ForwardModel::forward_intensity_stack_into on the device: (F, n_h, n_w) f64."""
+    ctx = context or default_context()
+    dm = dmodel or DeviceModel(ctx, model)
+    s = np.ascontiguousarray(spectrum, dtype=np.complex128)
+    p = None if pupil is None else np.ascontiguousarray(pupil, dtype=np.complex128)
+    out = np.zeros((model.frame_count,) + tuple(model.image_shape), np.float64)
+    check(lib().fpmhip_forward_stack(ctx.handle, dm.handle, _cd(s), None if p is None else _cd(p), _cd(out)))
+    return out
+
+
+class Session:
+    """A ReconstructionState on the device plus one algorithm."""
+
+    def __init__(self, problem: ReconstructionProblem, algorithm, precision="c128", context=None,
+                 dmodel=None, dmeas=None):
+        self.ctx = context or default_context()
+        self.problem = problem
+        self.dmodel = dmodel or DeviceModel(self.ctx, problem.model)
+        self.dmeas = dmeas or DeviceMeasurements(self.ctx, self.dmodel, problem.measurements)
+        self.alg = algorithm if isinstance(algorithm, abi.Algorithm) else algorithm.to_c()
+        self.precision = abi.PLAIN_C128 if precision in ("c128", "plain", abi.PLAIN_C128) else abi.MIXED_F32
+        self.handle = ct.c_void_p()
+        check(lib().fpmhip_session_create(self.dmodel.handle, self.dmeas.handle, ct.c_int32(self.precision),
+                                          ct.byref(self.alg), ct.byref(self.handle)))
+        m = problem.model
+        self.R = tuple(m.reconstruction_shape)
+        self.N = tuple(m.image_shape)
+        self.F = m.frame_count
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().fpmhip_session_destroy(self.handle)
+        except Exception:
+            pass
+
+    def initialize(self):
+        check(lib().fpmhip_session_initialize(self.handle))
+
+    def set_state(self, spectrum, pupil, velocity=None, anchor=None, counter=0, frame_gains=None):
+        keep = []
+
+        def c(a, dt=np.complex128):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return _cd(a)
+
+        check(lib().fpmhip_session_set_state(self.handle, c(spectrum), c(pupil), c(velocity), c(anchor),
+                                             ct.c_uint64(counter), c(frame_gains, np.float64), None,
+                                             ct.c_uint64(0)))
+
+    def sweep(self, order=None):
+        per = np.zeros(self.F if order is None else len(order), np.float64)
+        obj = ct.c_double(0.0)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.uint64)
+        check(lib().fpmhip_session_sweep(self.handle, None if o is None else abi.ptr(o, ct.c_uint64),
+                                         ct.c_uint64(0 if o is None else o.size), _cd(per), ct.byref(obj)))
+        return obj.value, per
+
+    def schedule(self, name="sequential"):
+        order = np.zeros(self.F, np.uint64)
+        check(lib().fpmhip_session_schedule(self.handle, ct.c_int32(abi.SCHEDULE[name]),
+                                            abi.ptr(order, ct.c_uint64)))
+        return order
+
+    def run(self, iterations, schedule="sequential"):
+        trace = np.zeros(iterations, np.float64)
+        check(lib().fpmhip_session_run(self.handle, ct.c_uint64(iterations), ct.c_int32(abi.SCHEDULE[schedule]),
+                                       _cd(trace)))
+        return trace
+
+    def spectrum(self):
+        out = np.zeros(self.R, np.complex128)
+        check(lib().fpmhip_session_get_spectrum(self.handle, _cd(out)))
+        return out
+
+    def pupil(self):
+        out = np.zeros(self.N, np.complex128)
+        check(lib().fpmhip_session_get_pupil(self.handle, _cd(out)))
+        return out
+
+    def object(self):
+        out = np.zeros(self.R, np.complex128)
+        check(lib().fpmhip_session_get_object(self.handle, _cd(out)))
+        return out
+
+    def momentum(self):
+        v = np.zeros(self.R, np.complex128)
+        a = np.zeros(self.R, np.complex128)
+        c = ct.c_uint64(0)
+        check(lib().fpmhip_session_get_momentum(self.handle, _cd(v), _cd(a), ct.byref(c)))
+        return v, a, c.value
+
+
+@dataclass
+class ReconstructionResult:
+    """fpm_rs ReconstructionResult fields (src/reconstruction/result.rs:40-76)."""
+
+    object: np.ndarray
+    amplitude: np.ndarray
+    phase: np.ndarray
+    object_spectrum: np.ndarray
+    recovered_pupil: np.ndarray
+    trace: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+
+def run_algorithm(algorithm, problem: ReconstructionProblem, precision="c128", schedule="sequential",
+                  context=None) -> ReconstructionResult:
+    """Runner::run (src/reconstruction/runner.rs:76-285) on one device session."""
+    alg = algorithm.to_c()
+    validate(alg)
+    s = Session(problem, alg, precision=precision, context=context)
+    s.initialize()
+    trace = s.run(int(algorithm.iterations), schedule)
+    obj = s.object()
+    res = ReconstructionResult(obj, np.abs(obj), np.angle(obj), s.spectrum(), s.pupil(), trace)
+    if alg.kind == abi.MPIE:
+        v, a, c = s.momentum()
+        res.extra.update(velocity=v, anchor=a, counter=c)
+    return res
