@@ -1,10 +1,258 @@
-// batch.hip — batched form (placeholder until the fused sweep kernel lands).
-#include "session.h"
+// batch.hip — batched form: many independent tiles (own model, pupil and
+// measurements; uneven sizes) advanced by one launch of the fused sweep
+// kernel per iteration, captured once in a HIP graph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "fused.h"
+
+using namespace fpmhip;
+
+namespace {
+
+template <class F>
+int guarded(F&& body) {
+  clear_error();
+  try {
+    body();
+    return FPMHIP_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  } catch (const std::bad_alloc&) {
+    return capture_bad_alloc();
+  }
+}
+
+__global__ void k_reset_tiles(TileDesc* tiles, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    *tiles[i].iteration = 0;
+    tiles[i].status[0] = 0;
+  }
+}
+
+__global__ void k_set_counter(uint64_t* counter, uint64_t value) {
+  if (threadIdx.x == 0) *counter = value;
+}
+
+}  // namespace
+
+struct fpmhip_batch {
+  Context* ctx = nullptr;
+  std::vector<fpmhip_session*> sessions;
+  std::vector<TileDesc> host_tiles;
+  DeviceBuffer tiles;      // TileDesc[count]
+  DeviceBuffer tw;         // float2[kTwTotal]
+  DeviceBuffer per_tile;   // counters, iterations, status
+  DeviceBuffer traces;     // double[count * trace_cap]
+  DeviceBuffer losses;     // double[count * Fmax]
+  uint32_t nmax = 0;
+  uint32_t trace_cap = 0;
+  uint64_t fmax = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double ms_mean = 0.0, ms_max = 0.0;
+  uint64_t sweeps = 0;
+  ~fpmhip_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
+  const uint32_t count = (uint32_t)b.sessions.size();
+  b.trace_cap = trace_cap;
+  b.traces.allocate((size_t)count * trace_cap * sizeof(double));
+  b.losses.allocate((size_t)count * b.fmax * sizeof(double));
+  // per tile: counter (u64), iteration (u32), status (i32)
+  b.per_tile.allocate((size_t)count * 16);
+  b.host_tiles.resize(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    fpmhip_session& s = *b.sessions[i];
+    const Model& m = *s.model;
+    const Measurements& me = *s.meas;
+    TileDesc& t = b.host_tiles[i];
+    t = TileDesc{};
+    t.spec = s.b.spec.as<double2>();
+    t.vel = s.b.vel.as<double2>();
+    t.anc = s.b.anc.as<double2>();
+    t.pupil = s.b.pupil.as<float2>();
+    t.frames = me.d_frames.ptr;
+    t.masks = me.d_masks.as<uint8_t>();
+    t.weights = me.d_weights.as<double>();
+    t.gains = m.d_gains.as<double>();
+    t.bg = m.d_background.as<double>();
+    t.bg_len = m.background.size();
+    t.src = m.d_src.as<SrcParam>();
+    t.order = nullptr;
+    t.trace = b.traces.as<double>() + (size_t)i * trace_cap;
+    t.frame_loss = b.losses.as<double>() + (size_t)i * b.fmax;
+    char* base = b.per_tile.as<char>() + (size_t)i * 16;
+    t.counter = reinterpret_cast<uint64_t*>(base);
+    t.iteration = reinterpret_cast<uint32_t*>(base + 8);
+    t.status = reinterpret_cast<int32_t*>(base + 12);
+    t.trace_len = trace_cap;
+    t.n = m.nh;
+    t.rh = m.rh;
+    t.rw = m.rw;
+    t.F = (uint32_t)m.F;
+    t.interval = (uint32_t)s.cfg.interval;
+    t.is_u16 = me.is_u16 ? 1 : 0;
+    t.denom = s.cfg.denom;
+    t.momentum = s.cfg.momentum ? 1 : 0;
+    t.loss = s.cfg.loss;
+    t.object_step = s.cfg.object_step;
+    t.alpha = s.cfg.alpha;
+    t.eps = s.cfg.epsilon;
+    double maxp = 0.0;
+    for (uint64_t p = 0; p < m.N(); ++p)
+      maxp = std::max(maxp, m.pupil[2 * p] * m.pupil[2 * p] + m.pupil[2 * p + 1] * m.pupil[2 * p + 1]);
+    t.max_p = maxp;
+    t.friction = s.cfg.friction;
+    t.feedback = s.cfg.feedback;
+  }
+  b.tiles.allocate(count * sizeof(TileDesc));
+  HIP_CHECK(hipMemcpyAsync(b.tiles.ptr, b.host_tiles.data(), count * sizeof(TileDesc), hipMemcpyHostToDevice,
+                           b.ctx->stream));
+  // capture the sweep launch once
+  if (b.exec) {
+    (void)hipGraphExecDestroy(b.exec);
+    b.exec = nullptr;
+  }
+  if (b.graph) {
+    (void)hipGraphDestroy(b.graph);
+    b.graph = nullptr;
+  }
+  HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
+  // set the LDS attribute outside capture, then capture the launch
+  prepare_sweep(b.nmax);
+  HIP_CHECK(hipStreamBeginCapture(b.ctx->stream, hipStreamCaptureModeThreadLocal));
+  launch_sweep(b.tiles.as<TileDesc>(), count, b.tw.as<float2>(), b.nmax, b.ctx->stream);
+  HIP_CHECK(hipStreamEndCapture(b.ctx->stream, &b.graph));
+  HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+}
+
+}  // namespace
 
 extern "C" {
-int fpmhip_batch_create(fpmhip_context*, fpmhip_session* const*, uint64_t, fpmhip_batch**) {
-  return FPMHIP_UNSUPPORTED;
+
+int fpmhip_batch_create(fpmhip_context* c, fpmhip_session* const* sessions, uint64_t count, fpmhip_batch** out) {
+  return guarded([&] {
+    if (!c || !sessions || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
+    if (count == 0 || count > 65535) fail(FPMHIP_INVALID_PARAMETER, "tile count must be in 1..65535", "count");
+    HIP_CHECK(hipSetDevice(c->device));
+    auto b = std::make_unique<fpmhip_batch>();
+    b->ctx = c;
+    int kind = -1;
+    for (uint64_t i = 0; i < count; ++i) {
+      fpmhip_session* s = sessions[i];
+      if (!s) fail(FPMHIP_INVALID_PARAMETER, "null session", "sessions");
+      if (s->ctx != c) fail(FPMHIP_INVALID_PARAMETER, "sessions must share the batch context", "sessions");
+      if (s->precision != FPMHIP_MIXED_F32) fail(FPMHIP_UNSUPPORTED, "batched tiles must use MIXED_F32 sessions");
+      const Model& m = *s->model;
+      if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128))
+        fail(FPMHIP_UNSUPPORTED, "batched tiles must be square 32, 64 or 128 pixel images");
+      if (m.mux) fail(FPMHIP_UNSUPPORTED, "batched tiles do not support multiplexed frames");
+      if (s->alg.kind == FPMHIP_EPRY) fail(FPMHIP_UNSUPPORTED, "batched tiles do not support EPRY");
+      if (kind >= 0 && kind != s->alg.kind) fail(FPMHIP_INVALID_PARAMETER, "tiles must share one algorithm", "sessions");
+      kind = s->alg.kind;
+      b->nmax = std::max(b->nmax, m.nh);
+      b->fmax = std::max<uint64_t>(b->fmax, m.F);
+      b->sessions.push_back(s);
+    }
+    if (sweep_lds_bytes(b->nmax) > (size_t)c->props.sharedMemPerBlockOptin && c->props.sharedMemPerBlockOptin > 0)
+      fail(FPMHIP_UNSUPPORTED, "tile does not fit the workgroup's LDS");
+    // packed twiddle tables W_n^j (forward, inverse) for n = 32, 64, 128
+    std::vector<float> tw(2 * kTwTotal);
+    for (uint32_t n : {32u, 64u, 128u}) {
+      const uint32_t off = tw_offset(n);
+      for (uint32_t j = 0; j < n; ++j)
+        for (int d = 0; d < 2; ++d) {
+          const int sign = d == 0 ? -1 : 1;
+          long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+          double re, im;
+          unit_root(j, n, sign, &re, &im);
+          const bool exact = (4ull * j) % n == 0;
+          tw[2 * (off + d * n + j)] = exact ? (float)re : (float)cosl(th);
+          tw[2 * (off + d * n + j) + 1] = exact ? (float)im : (float)(sign * sinl(th));
+        }
+    }
+    b->tw.allocate(tw.size() * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(b->tw.ptr, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    build_tiles(*b, 64);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = b.release();
+  });
 }
-int fpmhip_batch_destroy(fpmhip_batch*) { return FPMHIP_OK; }
-int fpmhip_batch_run(fpmhip_batch*, uint64_t, double*) { return FPMHIP_UNSUPPORTED; }
+
+int fpmhip_batch_destroy(fpmhip_batch* b) {
+  return guarded([&] {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+  });
 }
+
+int fpmhip_batch_run(fpmhip_batch* b, uint64_t iterations, double* trace) {
+  return guarded([&] {
+    if (!b) fail(FPMHIP_INVALID_PARAMETER, "null batch", "batch");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const uint32_t count = (uint32_t)b->sessions.size();
+    if (iterations > b->trace_cap) build_tiles(*b, (uint32_t)iterations);
+    // initialise every tile (ReconstructionState::initialize + mPIE auxiliary)
+    for (fpmhip_session* s : b->sessions) {
+      if (int rc = fpmhip_session_initialize(s)) throw Failure{rc};
+    }
+    for (uint32_t i = 0; i < count; ++i)
+      hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, b->host_tiles[i].counter, b->sessions[i]->counter);
+    hipLaunchKernelGGL(k_reset_tiles, dim3((count + 255) / 256), dim3(256), 0, st, b->tiles.as<TileDesc>(), count);
+    HIP_CHECK(hipGetLastError());
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    double total = 0.0, mx = 0.0;
+    for (uint64_t it = 0; it < iterations; ++it) {
+      HIP_CHECK(hipEventRecord(e0, st));
+      HIP_CHECK(hipGraphLaunch(b->exec, st));
+      HIP_CHECK(hipEventRecord(e1, st));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      total += ms;
+      mx = std::max(mx, (double)ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->sweeps = iterations;
+    b->ms_mean = iterations ? total / (double)iterations : 0.0;
+    b->ms_max = mx;
+    std::vector<char> per(count * 16);
+    HIP_CHECK(hipMemcpyAsync(per.data(), b->per_tile.ptr, per.size(), hipMemcpyDeviceToHost, st));
+    std::vector<double> tr((size_t)count * b->trace_cap);
+    HIP_CHECK(hipMemcpyAsync(tr.data(), b->traces.ptr, tr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < count; ++i) {
+      const char* base = per.data() + (size_t)i * 16;
+      const int32_t status = *reinterpret_cast<const int32_t*>(base + 12);
+      if (status) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
+      b->sessions[i]->counter = *reinterpret_cast<const uint64_t*>(base);
+      if (trace)
+        for (uint64_t it = 0; it < iterations; ++it) trace[i * iterations + it] = tr[(size_t)i * b->trace_cap + it];
+    }
+  });
+}
+
+int fpmhip_batch_timing(const fpmhip_batch* b, double* mean, double* mx, uint64_t* sweeps) {
+  return guarded([&] {
+    if (!b) fail(FPMHIP_INVALID_PARAMETER, "null batch", "batch");
+    if (mean) *mean = b->ms_mean;
+    if (mx) *mx = b->ms_max;
+    if (sweeps) *sweeps = b->sweeps;
+  });
+}
+
+}  // extern "C"

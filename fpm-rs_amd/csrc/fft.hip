@@ -148,31 +148,29 @@ __global__ void __launch_bounds__(256) k_fft_lines(LineKernelArgs<R> a) {
       const uint32_t pp = t / s, q = t % s;
       const V* in = src + (size_t)line * n;
       V* out = dst + (size_t)line * n;
-      V b[64];
       if (p == 4) {
-        V a0 = in[q + s * pp], a1 = in[q + s * (pp + m)], a2 = in[q + s * (pp + 2 * m)],
-          a3 = in[q + s * (pp + 3 * m)];
-        V t0 = C::add(a0, a2), t1 = C::sub(a0, a2), t2 = C::add(a1, a3), d = C::sub(a1, a3);
-        V t3 = a.inverse ? C::make(-d.y, d.x) : C::make(d.y, -d.x);
-        b[0] = C::add(t0, t2);
-        b[1] = C::add(t1, t3);
-        b[2] = C::sub(t0, t2);
-        b[3] = C::sub(t1, t3);
+        const V a0 = in[q + s * pp], a1 = in[q + s * (pp + m)], a2 = in[q + s * (pp + 2 * m)],
+                a3 = in[q + s * (pp + 3 * m)];
+        const V t0 = C::add(a0, a2), t1 = C::sub(a0, a2), t2 = C::add(a1, a3), d = C::sub(a1, a3);
+        const V t3 = a.inverse ? C::make(-d.y, d.x) : C::make(d.y, -d.x);
+        out[q + s * (p * pp)] = C::add(t0, t2);
+        out[q + s * (p * pp + 1)] = C::mul(C::add(t1, t3), a.tw[(pp * 1 * s) % n]);
+        out[q + s * (p * pp + 2)] = C::mul(C::sub(t0, t2), a.tw[(pp * 2 * s) % n]);
+        out[q + s * (p * pp + 3)] = C::mul(C::sub(t1, t3), a.tw[(pp * 3 * s) % n]);
       } else if (p == 2) {
-        V a0 = in[q + s * pp], a1 = in[q + s * (pp + m)];
-        b[0] = C::add(a0, a1);
-        b[1] = C::sub(a0, a1);
+        const V a0 = in[q + s * pp], a1 = in[q + s * (pp + m)];
+        out[q + s * (p * pp)] = C::add(a0, a1);
+        out[q + s * (p * pp + 1)] = C::mul(C::sub(a0, a1), a.tw[(pp * s) % n]);
       } else {
+        // direct DFT butterfly, each output written as soon as it is summed
         const uint32_t step = n / p;
         for (uint32_t k = 0; k < p; ++k) {
           V acc = in[q + s * pp];
           for (uint32_t r = 1; r < p; ++r)
             acc = C::add(acc, C::mul(in[q + s * (pp + r * m)], a.tw[((r * k) % p) * step]));
-          b[k] = acc;
+          out[q + s * (p * pp + k)] = k == 0 ? acc : C::mul(acc, a.tw[(pp * k * s) % n]);
         }
       }
-      out[q + s * (p * pp)] = b[0];
-      for (uint32_t k = 1; k < p; ++k) out[q + s * (p * pp + k)] = C::mul(b[k], a.tw[(pp * k * s) % n]);
     }
     __syncthreads();
     V* tmp = src;

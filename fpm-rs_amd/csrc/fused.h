@@ -1,0 +1,47 @@
+// fused.h — descriptor and launcher of the fused per-tile sweep kernel.
+#pragma once
+
+#include "session.h"
+
+namespace fpmhip {
+
+// Everything one workgroup needs to run one sweep of one tile.
+struct TileDesc {
+  double2* spec;          // [R] FP64 centred object spectrum
+  double2* vel;           // [R] mPIE velocity (or null)
+  double2* anc;           // [R] mPIE anchor (or null)
+  const float2* pupil;    // [N] complex64 pupil
+  const void* frames;     // [F*N] uint16 counts or f64 frames
+  const uint8_t* masks;   // [F*N] or null
+  const double* weights;  // [F]
+  const double* gains;    // [F]
+  const double* bg;       // [N] or [F*N] or null
+  uint64_t bg_len;
+  const SrcParam* src;    // [F] per-source crop parameters
+  const uint32_t* order;  // [F] frame order, or null for sequential
+  double* trace;          // [trace_len] iteration objectives
+  double* frame_loss;     // [F] objective of each order position, last sweep
+  uint64_t* counter;      // mPIE effective-frame counter (persists across sweeps)
+  uint32_t* iteration;    // sweeps completed
+  int32_t* status;        // [0] non-finite momentum
+  uint32_t trace_len;
+  uint32_t n, rh, rw, F;
+  uint32_t interval;
+  int32_t is_u16, denom, momentum, loss;
+  double object_step, alpha, eps, max_p, friction, feedback;
+};
+
+// Offset (in float2 units) of the forward/inverse twiddle tables of length n
+// inside the packed table: [32 fwd, 32 inv, 64 fwd, 64 inv, 128 fwd, 128 inv].
+__host__ __device__ inline uint32_t tw_offset(uint32_t n) {
+  return n == 32 ? 0u : n == 64 ? 64u : 192u;
+}
+constexpr uint32_t kTwTotal = 2 * (32 + 64 + 128);
+constexpr uint32_t kFusedMaxN = 128;
+
+size_t sweep_lds_bytes(uint32_t nmax);
+// Raises the kernel's dynamic LDS limit (outside any graph capture).
+void prepare_sweep(uint32_t nmax);
+void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, hipStream_t stream);
+
+}  // namespace fpmhip

@@ -27,6 +27,7 @@ EXPORTS = [
     "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
     "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
+    "fpmhip_batch_timing",
 ]
 
 
@@ -307,3 +308,35 @@ def run_algorithm(algorithm, problem: ReconstructionProblem, precision="c128", s
         v, a, c = s.momentum()
         res.extra.update(velocity=v, anchor=a, counter=c)
     return res
+
+
+class Batch:
+    """Many MIXED_F32 sessions (tiles) advanced together: one fused sweep
+    launch per iteration for all tiles, replayed from a HIP graph."""
+
+    def __init__(self, sessions, context=None):
+        self.ctx = context or sessions[0].ctx
+        self.sessions = list(sessions)
+        arr = (ct.c_void_p * len(self.sessions))(*[s.handle.value for s in self.sessions])
+        self.handle = ct.c_void_p()
+        check(lib().fpmhip_batch_create(self.ctx.handle, arr, ct.c_uint64(len(self.sessions)),
+                                        ct.byref(self.handle)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().fpmhip_batch_destroy(self.handle)
+        except Exception:
+            pass
+
+    def run(self, iterations):
+        """Initialise every tile and run `iterations` sweeps; returns the
+        (tiles, iterations) objective trace."""
+        trace = np.zeros((len(self.sessions), iterations), np.float64)
+        check(lib().fpmhip_batch_run(self.handle, ct.c_uint64(iterations), _cd(trace)))
+        return trace
+
+    def timing(self):
+        mean, mx, n = ct.c_double(), ct.c_double(), ct.c_uint64()
+        check(lib().fpmhip_batch_timing(self.handle, ct.byref(mean), ct.byref(mx), ct.byref(n)))
+        return mean.value, mx.value, n.value

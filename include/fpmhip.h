@@ -295,6 +295,10 @@ int fpmhip_batch_destroy(fpmhip_batch* batch);
 /* Initialise every tile, run `iterations` sweeps (graph replays) and finish
  * each tile's result; trace [count][iterations] objectives (may be NULL). */
 int fpmhip_batch_run(fpmhip_batch* batch, uint64_t iterations, double* trace);
+/* Device time of the sweep launches of the last fpmhip_batch_run, measured
+ * with HIP events on the batch stream: mean and max per sweep (ms), count. */
+int fpmhip_batch_timing(const fpmhip_batch* batch, double* sweep_ms_mean, double* sweep_ms_max,
+                        uint64_t* sweeps);
 
 #ifdef __cplusplus
 }

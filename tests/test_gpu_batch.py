@@ -1,0 +1,97 @@
+"""Batched form: uneven tiles advanced by one fused sweep launch per
+iteration. Each tile is checked against the FP64 oracle (amplitude and phase
+relative L2) and is bitwise the same alone and in a mixed batch."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fpmhip import MeasurementStack, Mpie, Fpie, AlternatingProjection, ReconstructionProblem, runtime
+from fpmhip import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def tile(n, k, seed, peak=4e3):
+    model = runtime.compile_model(syn.base_optics(), syn.planar_array(k), (n, n))
+    obj = syn.random_field_object(model.reconstruction_shape, seed)
+    I = runtime.forward_stack(model, syn.object_spectrum(obj))
+    counts, ppp = syn.poisson_counts(I, peak, seed)
+    model.with_frame_gains(np.full(model.frame_count, ppp))
+    return model, MeasurementStack(counts=counts)
+
+
+def amp_phase_err(got, ref):
+    ea = np.linalg.norm(np.abs(got) - np.abs(ref)) / np.linalg.norm(np.abs(ref))
+    ep = np.linalg.norm(np.angle(got * np.conj(ref))) / np.linalg.norm(np.angle(ref))
+    return ea, ep
+
+
+def sessions_for(tiles, alg):
+    return [runtime.Session(ReconstructionProblem(m, mo), alg, precision="f32") for mo, m in tiles]
+
+
+@pytest.mark.parametrize("alg", [Mpie(iterations=3, momentum_interval=30, momentum_friction=0.5,
+                                      momentum_feedback=0.5),
+                                 Fpie(iterations=3), AlternatingProjection(iterations=3)])
+def test_batch_matches_fp64_oracle(ctx, alg):
+    tiles = [tile(64, 9, 11), tile(128, 7, 12), tile(32, 5, 13)]
+    say("batch", type(alg).__name__, [t[0].reconstruction_shape for t in tiles])
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    trace = b.run(3)
+    for (model, meas), s, tr in zip(tiles, ss, trace):
+        ref = oracle.run(model, meas, alg.to_c(), 3)
+        ea, ep = amp_phase_err(s.object(), ref["object"])
+        es = np.linalg.norm(s.spectrum() - ref["object_spectrum"]) / np.linalg.norm(ref["object_spectrum"])
+        te = np.abs(tr - ref["trace"]) / np.abs(ref["trace"])
+        say(model.image_shape, "amp", ea, "phase", ep, "spec", es, "trace", te.max())
+        assert ea <= 1e-5 and ep <= 1e-5 and es <= 1e-5
+        assert te.max() <= 1e-5
+
+
+def test_batch_tile_is_identical_alone_and_in_mix(ctx):
+    alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(64, 9, 21), tile(128, 7, 22), tile(64, 11, 23)]
+    mix = sessions_for(tiles, alg)
+    runtime.Batch(mix).run(2)
+    alone = sessions_for(tiles[1:2], alg)
+    runtime.Batch(alone).run(2)
+    assert np.array_equal(mix[1].spectrum(), alone[0].spectrum())
+
+
+def test_batch_repeatable_and_graph_replays_match(ctx):
+    """Two runs of the same batch (graph replays with eager work between them)
+    give bitwise identical spectra."""
+    alg = Mpie(iterations=3, momentum_interval=7, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(64, 9, 31), tile(32, 5, 32)]
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    b.run(3)
+    first = [s.spectrum() for s in ss]
+    b.run(3)
+    second = [s.spectrum() for s in ss]
+    for a, c in zip(first, second):
+        assert np.array_equal(a, c)
+    mean, mx, n = b.timing()
+    assert n == 3 and mean > 0
+
+
+def test_batch_vs_per_stage_mixed_session(ctx):
+    """The fused sweep and the per-stage MIXED_F32 path agree to FP32 level."""
+    alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
+    model, meas = tile(128, 15, 41, peak=1e4)
+    s = runtime.Session(ReconstructionProblem(meas, model), alg, precision="f32")
+    runtime.Batch([s]).run(2)
+    p = runtime.Session(ReconstructionProblem(meas, model), alg, precision="f32")
+    p.initialize()
+    p.run(2)
+    e = np.linalg.norm(s.spectrum() - p.spectrum()) / np.linalg.norm(p.spectrum())
+    say("fused vs per-stage", e)
+    assert e <= 1e-5
