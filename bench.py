@@ -144,7 +144,7 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed"])
     ap.add_argument("--tiles", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample-iterations", type=int, default=2)
+    ap.add_argument("--cpu-sample-iterations", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
