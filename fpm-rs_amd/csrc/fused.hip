@@ -6,9 +6,10 @@
 //   gather 4-tap bilinear patch from the FP64 object spectrum (crop.rs:129-167)
 //   x pupil -> ifftshift -> LDS
 //   inverse 2-D FFT in LDS (Stockham radix-16/8 stages, FP32)
-//   |field|^2, objective terms (FP64), amplitude replacement (common.rs:123-228)
-//   forward 2-D FFT in LDS (x 1/N), fftshift
-//   U = conj(P) (psi' - patch P) / denominator (common.rs:240-261)
+//   |field|^2, objective terms (FP64), amplitude replacement (common.rs:123-228),
+//   kept as the field change psi' - psi
+//   forward 2-D FFT in LDS (x 1/N), fftshift: the exit-wave difference
+//   U = conj(P) difference / denominator (common.rs:240-261)
 //   adjoint insert as a gather over the (n+1)^2 window, FP64 RMW (crop.rs:188-256)
 //   every `interval` counted frames: mPIE momentum over the tile's R^2 grid
 //   (common.rs:308-387)
@@ -125,37 +126,48 @@ __device__ __forceinline__ void reg_dft(float2* x) {
   dit_levels<P, INV>(x);
 }
 
+// Row pitch of the N x N field in LDS: one padding element per row makes
+// both row-wise and column-wise accesses of consecutive lanes hit distinct
+// banks (a stride of N+1 complex64 values is an odd number of 8-byte words).
+template <int N>
+__host__ __device__ constexpr int pitch() {
+  return N + 1;
+}
+
 // One Stockham stage (radix P, stride S) over all N lines of an N x N array
-// in LDS. Rows: consecutive threads walk a row; columns: consecutive threads
-// walk across columns. Loads complete (barrier) before any store, so the
-// stage runs in place in a single LDS buffer.
+// in LDS (element (row, col) at row * (N+1) + col). Consecutive lanes take
+// consecutive lines for row transforms and consecutive columns for column
+// transforms, so every LDS access of a wave is conflict free. Loads complete
+// (barrier) before any store, so the stage runs in place in one buffer.
 template <int N, int P, int S, bool ROWS, bool INV>
-__device__ __forceinline__ void stage(float2* lds, const float2* tw) {
+__device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid) {
+  asm volatile("" : "+v"(tid));
   constexpr int M = N / (P * S);
   constexpr int BPL = N / P;
   constexpr int TOTAL = N * BPL;
   constexpr int PER = (TOTAL + kThreads - 1) / kThreads;
+  constexpr int LD = pitch<N>();
   float2 a[PER][P];
   int line[PER], pp[PER], q[PER];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
-    const int b = threadIdx.x + u * kThreads;
-    line[u] = ROWS ? b / BPL : b % N;
-    const int t = ROWS ? b % BPL : b / N;
+    const int b = tid + u * kThreads;
+    line[u] = b % N;
+    const int t = b / N;
     pp[u] = t / S;
     q[u] = t % S;
     if (b < TOTAL) {
 #pragma unroll
       for (int r = 0; r < P; ++r) {
         const int e = q[u] + S * (pp[u] + r * M);
-        a[u][r] = ROWS ? lds[line[u] * N + e] : lds[e * N + line[u]];
+        a[u][r] = ROWS ? lds[line[u] * LD + e] : lds[e * LD + line[u]];
       }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
-    const int b = threadIdx.x + u * kThreads;
+    const int b = tid + u * kThreads;
     if (b < TOTAL) {
       reg_dft<P, INV>(a[u]);
 #pragma unroll
@@ -164,9 +176,9 @@ __device__ __forceinline__ void stage(float2* lds, const float2* tw) {
         float2 v = a[u][k];
         if (k > 0 && M > 1) v = fmul(v, tw[(pp[u] * k * S) % N]);
         if (ROWS)
-          lds[line[u] * N + e] = v;
+          lds[line[u] * LD + e] = v;
         else
-          lds[e * N + line[u]] = v;
+          lds[e * LD + line[u]] = v;
       }
     }
   }
@@ -190,13 +202,22 @@ struct Plan<128> {
 
 // Unscaled 2-D DFT in LDS with the direction given by `tw`.
 template <int N, bool INV>
-__device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw) {
+__device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw, int tid) {
   constexpr int P0 = Plan<N>::P0, P1 = Plan<N>::P1;
   static_assert(P0 * P1 == N, "plan");
-  stage<N, P0, 1, true, INV>(lds, tw);
-  stage<N, P1, P0, true, INV>(lds, tw);
-  stage<N, P0, 1, false, INV>(lds, tw);
-  stage<N, P1, P0, false, INV>(lds, tw);
+  stage<N, P0, 1, true, INV>(lds, tw, tid);
+  stage<N, P1, P0, true, INV>(lds, tw, tid);
+  stage<N, P0, 1, false, INV>(lds, tw, tid);
+  stage<N, P1, P0, false, INV>(lds, tw, tid);
+}
+
+// The thread index, hidden from the optimiser so that per-thread addresses
+// are recomputed inside the LED loop instead of being hoisted out of it and
+// spilled for the whole sweep.
+__device__ __forceinline__ int opaque_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
 }
 
 // Fixed-order workgroup sum of one double per thread.
@@ -213,120 +234,140 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
   return s;  // valid in thread 0
 }
 
-__device__ __forceinline__ double point_loss(double predicted, double measured, int loss) {
+// Object-update denominator of common.rs:247-258 as a + b |P|^2 with FP64
+// constants a, b rounded once: Local: (eps, 1); Global: (max + eps, 0);
+// Rpie: (alpha max + eps, 1 - alpha).
+struct Denominator {
+  float a, b;
+};
+__device__ __forceinline__ Denominator make_denominator(const TileDesc& T, double maxp) {
+  if (T.denom == FPMHIP_DENOMINATOR_LOCAL) return {(float)T.eps, 1.0f};
+  if (T.denom == FPMHIP_DENOMINATOR_GLOBAL) return {(float)(maxp + T.eps), 0.0f};
+  return {(float)(T.alpha * maxp + T.eps), (float)(1.0 - T.alpha)};
+}
+
+__device__ __forceinline__ float point_loss_f(float predicted, float measured, int loss) {
   if (loss == FPMHIP_LOSS_INTENSITY_MSE) {
-    const double r = predicted - measured;
+    const float r = predicted - measured;
     return r * r;
   }
   if (loss == FPMHIP_LOSS_POISSON_NLL) {
-    const double mean = fmax(predicted, 1e-12);
-    return mean - fmax(measured, 0.0) * log(mean);
+    const float mean = fmaxf(predicted, 1e-12f);
+    return mean - fmaxf(measured, 0.0f) * logf(mean);
   }
-  const double r = sqrt(fmax(predicted, 0.0)) - sqrt(fmax(measured, 0.0));
+  const float r = sqrtf(fmaxf(predicted, 0.0f)) - sqrtf(fmaxf(measured, 0.0f));
   if (loss == FPMHIP_LOSS_HUBER_AMPLITUDE) {
-    const double a = fabs(r);
-    return a <= 1.0 ? 0.5 * r * r : a - 0.5;
+    const float a = fabsf(r);
+    return a <= 1.0f ? 0.5f * r * r : a - 0.5f;
   }
   return r * r;
 }
 
-// Bilinear sample of the FP64 spectrum for centred patch pixel (r, c)
-// (32-bit offsets: a tile spectrum has far fewer than 2^32 elements).
-__device__ __forceinline__ double2 sample(const double2* __restrict__ spec, uint32_t rw, const SrcParam& sp,
-                                          int r, int c) {
-  if (sp.zero) return spec[((uint32_t)sp.sr + r) * rw + (uint32_t)sp.sc + c];
-  const uint32_t l0 = (sp.ar.lower + r) * rw + sp.ac.lower + c;
-  const uint32_t du = (sp.ar.upper - sp.ar.lower) * rw, dc = sp.ac.upper - sp.ac.lower;
-  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
-               wuu = sp.ar.wu * sp.ac.wu;
-  const double2 a = spec[l0], b = spec[l0 + dc], cc = spec[l0 + du], d = spec[l0 + du + dc];
-  return make_double2(fma(d.x, wuu, fma(cc.x, wul, fma(b.x, wlu, a.x * wll))),
-                      fma(d.y, wuu, fma(cc.y, wul, fma(b.y, wlu, a.y * wll))));
-}
-
-// Object-update denominator of common.rs:247-258 for pupil value P.
-__device__ __forceinline__ float inv_denominator(const TileDesc& T, float2 P, double maxp) {
-  const double local = (double)P.x * P.x + (double)P.y * P.y;
-  double den;
-  if (T.denom == FPMHIP_DENOMINATOR_LOCAL)
-    den = local + T.eps;
-  else if (T.denom == FPMHIP_DENOMINATOR_GLOBAL)
-    den = maxp + T.eps;
-  else
-    den = (1.0 - T.alpha) * local + T.alpha * maxp + T.eps;
-  return (float)(1.0 / den);
-}
-
-// One sweep of one tile. The centred pixel p = (r, c) and its swap partner
-// ((r + N/2) mod N, (c + N/2) mod N) are owned by the same thread: for even
-// N, fftshift and ifftshift are both that swap, so the shifted LDS reads and
-// writes of phases 1 and 5 stay inside the owning thread.
+// One sweep of one tile. Thread t owns a strip of centred pixels (rows
+// 8*(t/N) .. +7 of column t%N for N = 128) and their swap partners
+// ((r + N/2) mod N, (c + N/2) mod N): for even N, fftshift and ifftshift are
+// both that swap, so the shifted LDS reads and writes of phases 1 and 5 stay
+// inside the owning thread, and vertically adjacent pixels share bilinear taps.
 template <int N>
 __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi, double* red) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
-  constexpr int NPAIR = NN / 2;  // pairs: pixels with r < N/2 and their partners
-  constexpr int PR = (NPAIR + kThreads - 1) / kThreads;
-  const int tid = threadIdx.x;
+  constexpr int LD = pitch<N>();
+  constexpr int NPAIR = NN / 2;
+  constexpr int STRIP = NPAIR / N >= (kThreads / N) ? (NPAIR / N) / (kThreads / N) : 1;  // rows per thread
+  constexpr int COLS_THREADS = N * ((HALF + STRIP - 1) / STRIP);  // threads with work
   const float inv_n = 1.0f / (float)NN;
   double2* __restrict__ spec = T.spec;
   const float2* __restrict__ pupil = T.pupil;
   const double eps = T.eps;
-  const double maxp = fmax(T.max_p, eps);
+  const float eps_f = (float)eps;
+  const Denominator den = make_denominator(T, fmax(T.max_p, eps));
   uint64_t counter = *T.counter;
   double obj_sum = 0.0, w_sum = 0.0;
   for (uint32_t pos = 0; pos < T.F; ++pos) {
     const uint32_t f = T.order ? T.order[pos] : pos;
     const double fw = T.weights[f];
+    const int tid = opaque_tid();
     if (fw == 0.0) {
       if (tid == 0) T.frame_loss[pos] = 0.0;
       continue;
     }
     const SrcParam sp = T.src[f];
-    const double gain = T.gains[f];
-    // 1. patch x pupil at p and its partner, written swapped (ifftshift)
-    float2 ex0[PR], ex1[PR];
+    const float gain_f = (float)T.gains[f];
+    const float inv_gain = (float)(1.0 / T.gains[f]);
+    // 1. patch x pupil on a strip and its partner strip, written swapped (ifftshift)
+    if (tid < COLS_THREADS) {
+      const int c = tid % N, r0 = (tid / N) * STRIP;
+      const int c1 = (c + HALF) % N;
+      if (sp.zero) {
 #pragma unroll
-    for (int j = 0; j < PR; ++j) {
-      const int i = tid + j * kThreads;
-      if (i < NPAIR) {
-        const int r = i / N, c = i % N;
-        const int r1 = r + HALF, c1 = (c + HALF) % N;
-        const int p0 = r * N + c, p1 = r1 * N + c1;
-        const double2 v0 = sample(spec, T.rw, sp, r, c);
-        const double2 v1 = sample(spec, T.rw, sp, r1, c1);
-        ex0[j] = fmul(f2((float)v0.x, (float)v0.y), pupil[p0]);
-        ex1[j] = fmul(f2((float)v1.x, (float)v1.y), pupil[p1]);
-        lds[p1] = ex0[j];
-        lds[p0] = ex1[j];
+        for (int j = 0; j < STRIP; ++j) {
+          const int r = r0 + j;
+          const double2 v0 = spec[((uint32_t)sp.sr + r) * T.rw + (uint32_t)sp.sc + c];
+          const double2 v1 = spec[((uint32_t)sp.sr + r + HALF) * T.rw + (uint32_t)sp.sc + c1];
+          lds[(r + HALF) * LD + c1] = fmul(f2((float)v0.x, (float)v0.y), pupil[r * N + c]);
+          lds[r * LD + c] = fmul(f2((float)v1.x, (float)v1.y), pupil[(r + HALF) * N + c1]);
+        }
+      } else {
+        const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+                     wuu = sp.ar.wu * sp.ac.wu;
+        const uint32_t dc = sp.ac.upper - sp.ac.lower, du = (sp.ar.upper - sp.ar.lower) * T.rw;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
+          uint32_t off = (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc;
+          double2 a = spec[off], b = spec[off + dc];
+#pragma unroll
+          for (int j = 0; j < STRIP; ++j) {
+            const double2 c2 = spec[off + du], d = spec[off + du + dc];
+            const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
+                                           fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
+            const int r = rb + j;
+            const int rr = (r + HALF) % N, cs = (cc + HALF) % N;
+            lds[rr * LD + cs] = fmul(f2((float)v.x, (float)v.y), pupil[r * N + cc]);
+            off += T.rw;
+            if (du) {
+              a = c2;
+              b = d;
+            } else {
+              a = spec[off];
+              b = spec[off + dc];
+            }
+          }
+        }
       }
     }
     __syncthreads();
     // 2. inverse FFT (unscaled)
-    fft2_lds<N, true>(lds, twi);
-    // 3. amplitude replacement and objective terms (pointwise, natural order)
+    fft2_lds<N, true>(lds, twi, tid);
+    // 3. amplitude replacement and objective terms (pointwise, natural order).
+    //    The LDS field becomes psi' - psi: the forward transform is linear,
+    //    so FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
+    //    without cancelling two nearly equal spectra in FP32. Per-pixel terms
+    //    are FP32; the frame objective is accumulated in FP64.
     double lsum = 0.0, lcnt = 0.0;
     const uint64_t fbase = (uint64_t)f * NN;
     for (int p = tid; p < NN; p += kThreads) {
-      const float2 fv = lds[p];
-      const double pred = (double)fv.x * fv.x + (double)fv.y * fv.y;
+      const int lp = (p / N) * LD + p % N;
+      const float2 fv = lds[lp];
+      const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
       const bool masked = T.masks && T.masks[fbase + p] == 0;
-      float2 d = fv;
+      float2 d = f2(0.0f, 0.0f);  // masked: psi' = psi
       if (!masked) {
-        const double b = T.bg_len == 0 ? 0.0 : T.bg[T.bg_len == (uint64_t)NN ? p : fbase + p];
-        const double y = T.is_u16 ? (double)static_cast<const uint16_t*>(T.frames)[fbase + p]
-                                  : static_cast<const double*>(T.frames)[fbase + p];
-        lsum += point_loss(gain * pred + b, y, T.loss);
+        const float b = T.bg_len == 0 ? 0.0f : (float)T.bg[T.bg_len == (uint64_t)NN ? p : fbase + p];
+        const float y = T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[fbase + p]
+                                 : (float)static_cast<const double*>(T.frames)[fbase + p];
+        lsum += (double)point_loss_f(fmaf(gain_f, pred, b), y, T.loss);
         lcnt += 1.0;
-        const double target = fmax((y - b) / gain, 0.0);
-        if (pred > eps) {
-          const float ratio = (float)sqrt(target / pred);
-          d = f2(fv.x * ratio, fv.y * ratio);
+        const float target = fmaxf((y - b) * inv_gain, 0.0f);
+        if (pred > eps_f) {
+          const float rm1 = sqrtf(target / pred) - 1.0f;
+          d = f2(fv.x * rm1, fv.y * rm1);
         } else {
-          d = f2((float)sqrt(target), 0.0f);
+          d = f2(sqrtf(target) - fv.x, -fv.y);
         }
       }
-      lds[p] = d;
+      lds[lp] = d;
     }
     const double ls = wg_sum(lsum, red);
     const double lc = wg_sum(lcnt, red);
@@ -338,22 +379,22 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
     }
     __syncthreads();
     // 4. forward FFT (x 1/N applied in phase 5)
-    fft2_lds<N, false>(lds, twf);
-    // 5. U = conj(P) (fftshift(FFT)/N - patch P) / den at p and its partner
+    fft2_lds<N, false>(lds, twf, tid);
+    // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
+    if (tid < COLS_THREADS) {
+      const int c = tid % N, r0 = (tid / N) * STRIP;
+      const int c1 = (c + HALF) % N;
 #pragma unroll
-    for (int j = 0; j < PR; ++j) {
-      const int i = tid + j * kThreads;
-      if (i < NPAIR) {
-        const int r = i / N, c = i % N;
-        const int p0 = r * N + c, p1 = (r + HALF) * N + (c + HALF) % N;
-        const float2 s0 = lds[p1], s1 = lds[p0];
-        const float2 P0 = pupil[p0], P1 = pupil[p1];
-        const float2 d0 = fsub(f2(s0.x * inv_n, s0.y * inv_n), ex0[j]);
-        const float2 d1 = fsub(f2(s1.x * inv_n, s1.y * inv_n), ex1[j]);
-        const float2 u0 = fmulc(P0, d0), u1 = fmulc(P1, d1);
-        const float k0 = inv_denominator(T, P0, maxp), k1 = inv_denominator(T, P1, maxp);
-        lds[p0] = f2(u0.x * k0, u0.y * k0);
-        lds[p1] = f2(u1.x * k1, u1.y * k1);
+      for (int j = 0; j < STRIP; ++j) {
+        const int r = r0 + j, r1 = r + HALF;
+        const int l0 = r * LD + c, l1 = r1 * LD + c1;
+        const float2 s0 = lds[l1], s1 = lds[l0];
+        const float2 P0 = pupil[r * N + c], P1 = pupil[r1 * N + c1];
+        const float k0 = inv_n / fmaf(den.b, fmaf(P0.x, P0.x, P0.y * P0.y), den.a);
+        const float k1 = inv_n / fmaf(den.b, fmaf(P1.x, P1.x, P1.y * P1.y), den.a);
+        const float2 u0 = fmulc(P0, s0), u1 = fmulc(P1, s1);
+        lds[l0] = f2(u0.x * k0, u0.y * k0);
+        lds[l1] = f2(u1.x * k1, u1.y * k1);
       }
     }
     __syncthreads();
@@ -363,41 +404,58 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
       for (int p = tid; p < NN; p += kThreads) {
         const int r = p / N, c = p % N;
         double2& d = spec[(sp.sr + r) * T.rw + sp.sc + c];
-        const float2 u = lds[p];
+        const float2 u = lds[r * LD + c];
         d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
       }
     } else {
       const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
       const int wh = N + (fr ? 1 : 0), ww = N + (fc ? 1 : 0);
-      const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
-                   wuu = sp.ar.wu * sp.ac.wu;
-      for (int qd = tid; qd < wh * ww; qd += kThreads) {
-        const int a = qd / ww, b = qd % ww;
-        double2& dst = spec[(sp.ar.lower + a) * (uint64_t)T.rw + sp.ac.lower + b];
-        double ax = 0.0, ay = 0.0;
-        auto tap = [&](int sr_, int sc_, double wt) {
-          if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
-            const float2 u = lds[sr_ * N + sc_];
-            ax = fma((double)u.x, wt, ax);
-            ay = fma((double)u.y, wt, ay);
+      const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
+                  wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
+      const int total = wh * ww;
+      constexpr int UNROLL = 4;
+      for (int base = tid; base < total; base += UNROLL * kThreads) {
+        double2 old[UNROLL];
+        float2 add[UNROLL];
+        uint32_t off[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+          const int qd = base + k * kThreads;
+          if (qd < total) {
+            const int a = qd / ww, b = qd % ww;
+            off[k] = (sp.ar.lower + a) * T.rw + sp.ac.lower + b;
+            old[k] = spec[off[k]];
+            float ax = 0.0f, ay = 0.0f;
+            auto tap = [&](int sr_, int sc_, float wt) {
+              if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
+                const float2 u = lds[sr_ * LD + sc_];
+                ax = fmaf(u.x, wt, ax);
+                ay = fmaf(u.y, wt, ay);
+              }
+            };
+            if (fr && fc) {
+              tap(a - 1, b - 1, wuu);
+              tap(a - 1, b, wul);
+              tap(a, b - 1, wlu);
+              tap(a, b, wll);
+            } else if (fr) {
+              tap(a - 1, b, wul);
+              tap(a, b, wll);
+            } else if (fc) {
+              tap(a, b - 1, wlu);
+              tap(a, b, wll);
+            } else {
+              tap(a, b, wll);
+            }
+            add[k] = f2(ax, ay);
           }
-        };
-        if (fr && fc) {
-          tap(a - 1, b - 1, wuu);
-          tap(a - 1, b, wul);
-          tap(a, b - 1, wlu);
-          tap(a, b, wll);
-        } else if (fr) {
-          tap(a - 1, b, wul);
-          tap(a, b, wll);
-        } else if (fc) {
-          tap(a, b - 1, wlu);
-          tap(a, b, wll);
-        } else {
-          tap(a, b, wll);
         }
-        const double2 o = dst;
-        dst = make_double2(fma(scale, ax, o.x), fma(scale, ay, o.y));
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+          const int qd = base + k * kThreads;
+          if (qd < total)
+            spec[off[k]] = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
+        }
       }
     }
     __syncthreads();
@@ -423,7 +481,7 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
       }
     }
   }
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     const uint32_t it = *T.iteration;
     if (it < T.trace_len) T.trace[it] = w_sum > 0.0 ? obj_sum / w_sum : __builtin_nan("");
     *T.iteration = it + 1;
@@ -437,7 +495,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__
   const TileDesc T = tiles[blockIdx.x];
   const int N = (int)T.n;
   // twiddles W_N^j, forward then inverse, after the N x N field
-  float2* twf = lds + N * N;
+  float2* twf = lds + N * (N + 1);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
   const float2* src = tw_all + tw_offset(N);
@@ -461,7 +519,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__
 }  // namespace
 
 size_t sweep_lds_bytes(uint32_t nmax) {
-  return (size_t)nmax * nmax * sizeof(float2) + 2 * (size_t)nmax * sizeof(float2) + 16 * sizeof(double);
+  return (size_t)nmax * (nmax + 1) * sizeof(float2) + 2 * (size_t)nmax * sizeof(float2) + 16 * sizeof(double);
 }
 
 void prepare_sweep(uint32_t nmax) {
