@@ -246,6 +246,11 @@ __device__ __forceinline__ Denominator make_denominator(const TileDesc& T, doubl
   return {(float)(T.alpha * maxp + T.eps), (float)(1.0 - T.alpha)};
 }
 
+// Hardware square root and reciprocal (1 ulp): the FP32 per-pixel terms
+// feed FP64 accumulations, far inside the 1e-5 error budget.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ float point_loss_f(float predicted, float measured, int loss) {
   if (loss == FPMHIP_LOSS_INTENSITY_MSE) {
     const float r = predicted - measured;
@@ -255,7 +260,7 @@ __device__ __forceinline__ float point_loss_f(float predicted, float measured, i
     const float mean = fmaxf(predicted, 1e-12f);
     return mean - fmaxf(measured, 0.0f) * logf(mean);
   }
-  const float r = sqrtf(fmaxf(predicted, 0.0f)) - sqrtf(fmaxf(measured, 0.0f));
+  const float r = fsqrt(fmaxf(predicted, 0.0f)) - fsqrt(fmaxf(measured, 0.0f));
   if (loss == FPMHIP_LOSS_HUBER_AMPLITUDE) {
     const float a = fabsf(r);
     return a <= 1.0f ? 0.5f * r * r : a - 0.5f;
@@ -345,30 +350,47 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
     //    so FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
     //    without cancelling two nearly equal spectra in FP32. Per-pixel terms
     //    are FP32; the frame objective is accumulated in FP64.
-    double lsum = 0.0, lcnt = 0.0;
     const uint64_t fbase = (uint64_t)f * NN;
-    for (int p = tid; p < NN; p += kThreads) {
-      const int lp = (p / N) * LD + p % N;
-      const float2 fv = lds[lp];
-      const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
-      const bool masked = T.masks && T.masks[fbase + p] == 0;
-      float2 d = f2(0.0f, 0.0f);  // masked: psi' = psi
-      if (!masked) {
-        const float b = T.bg_len == 0 ? 0.0f : (float)T.bg[T.bg_len == (uint64_t)NN ? p : fbase + p];
-        const float y = T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[fbase + p]
-                                 : (float)static_cast<const double*>(T.frames)[fbase + p];
-        lsum += (double)point_loss_f(fmaf(gain_f, pred, b), y, T.loss);
-        lcnt += 1.0;
-        const float target = fmaxf((y - b) * inv_gain, 0.0f);
-        if (pred > eps_f) {
-          const float rm1 = sqrtf(target / pred) - 1.0f;
-          d = f2(fv.x * rm1, fv.y * rm1);
-        } else {
-          d = f2(sqrtf(target) - fv.x, -fv.y);
+    float lsum_f = 0.0f;
+    int lcnt_i = 0;
+    {
+      constexpr int PP = (NN + kThreads - 1) / kThreads;  // pixels per thread
+      float y[PP];
+      // issue every measurement load of the thread before using any of them
+#pragma unroll
+      for (int j = 0; j < PP; ++j) {
+        const int p = tid + j * kThreads;
+        y[j] = 0.0f;
+        if (p < NN)
+          y[j] = T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[fbase + p]
+                          : (float)static_cast<const double*>(T.frames)[fbase + p];
+      }
+#pragma unroll
+      for (int j = 0; j < PP; ++j) {
+        const int p = tid + j * kThreads;
+        if (p < NN) {
+          const int lp = (p / N) * LD + p % N;
+          const float2 fv = lds[lp];
+          const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
+          float2 d = f2(0.0f, 0.0f);  // masked: psi' = psi
+          if (!(T.masks && T.masks[fbase + p] == 0)) {
+            const float b = T.bg_len == 0 ? 0.0f : (float)T.bg[T.bg_len == (uint64_t)NN ? p : fbase + p];
+            lsum_f += point_loss_f(fmaf(gain_f, pred, b), y[j], T.loss);
+            lcnt_i += 1;
+            const float target = fmaxf((y[j] - b) * inv_gain, 0.0f);
+            if (pred > eps_f) {
+              const float rm1 = fsqrt(target * frcp(pred)) - 1.0f;
+              d = f2(fv.x * rm1, fv.y * rm1);
+            } else {
+              d = f2(fsqrt(target) - fv.x, -fv.y);
+            }
+          }
+          lds[lp] = d;
         }
       }
-      lds[lp] = d;
     }
+    // per-thread partials (at most 16 terms) in FP32, workgroup reduction in FP64
+    const double lsum = (double)lsum_f, lcnt = (double)lcnt_i;
     const double ls = wg_sum(lsum, red);
     const double lc = wg_sum(lcnt, red);
     if (tid == 0) {
