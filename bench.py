@@ -126,6 +126,69 @@ def run_batched(model, meas, iterations, steps, warmup, tiles):
     return time.perf_counter() - t0, float(np.mean(sweep_ms))
 
 
+def serving(requests, rate, iterations, seed=5, max_tiles=256):
+    """Poisson arrivals (mean `rate` req/s) of jobs with tile size in
+    {64^2, 128^2}, 1-3 channels from {450, 532, 630} nm (each channel one tile
+    with its own compiled pupil), k x k LEDs with k in {9, 11, 13, 15}; FP32
+    mPIE `iterations` sweeps. Inputs are built before the arrival clock
+    starts. Returns latency percentiles and throughput."""
+    rng = np.random.default_rng(seed)
+    wavelengths = (450e-9, 532e-9, 630e-9)
+    pool = {}
+
+    def problem(n, k, wl):
+        key = (n, k, wl)
+        if key not in pool:
+            model = runtime.compile_model(syn.base_optics(wl), syn.planar_array(k), (n, n))
+            obj = syn.random_field_object(model.reconstruction_shape, int(rng.integers(1 << 30)))
+            I = runtime.forward_stack(model, syn.object_spectrum(obj))
+            counts, ppp = syn.poisson_counts(I, 4e3, int(rng.integers(1 << 30)))
+            model.with_frame_gains(np.full(model.frame_count, ppp))
+            pool[key] = ReconstructionProblem(MeasurementStack(counts=counts), model)
+        return pool[key]
+
+    reqs = []
+    t = 0.0
+    for _ in range(requests):
+        t += rng.exponential(1.0 / rate)
+        n = int(rng.choice([64, 128]))
+        k = int(rng.choice([9, 11, 13, 15]))
+        ch = rng.choice(3, size=int(rng.integers(1, 4)), replace=False)
+        reqs.append((t, [problem(n, k, wavelengths[c]) for c in sorted(ch)]))
+    alg = Mpie(iterations=iterations, momentum_friction=0.5, momentum_feedback=0.5)
+    server = runtime.Server(max_tiles=max_tiles)
+    # one warm-up job so that first-use costs stay out of the measurement
+    w = server.submit(reqs[0][1][0], alg)
+    server.wait(w, 120)
+    server.result(w)
+    log(f"serving: {requests} requests, {sum(len(r[1]) for r in reqs)} tiles, rate {rate}/s")
+    ids = []
+    t0 = time.perf_counter()
+    for (ta, probs) in reqs:
+        delay = t0 + ta - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append([server.submit(p, alg) for p in probs])
+    lat = []
+    last = 0.0
+    for group in ids:
+        worst = 0.0
+        for jid in group:
+            server.wait(jid, 600)
+            _, _, _, l, _ = server.result(jid)
+            worst = max(worst, l)
+        lat.append(worst)
+    stats = server.stats()
+    wall = time.perf_counter() - t0
+    server.close()
+    tiles = sum(len(g) for g in ids)
+    lat = np.array(lat) * 1000.0
+    return {"requests": requests, "tiles": tiles, "rate_per_s": rate, "iterations": iterations,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(lat.max()), "tiles_per_s": tiles / wall, "ticks": stats["ticks"],
+            "sizes": "64^2,128^2", "channels": "1-3 of 450/532/630 nm", "leds": "9x9..15x15"}
+
+
 def cpu_baseline(model, meas, iterations, sample_iterations, threads):
     """The oracle (scalar C++ FP64 restatement of the reference's serial
     per-tile path) on `threads` host threads, one tile per thread, each
@@ -160,7 +223,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=50)
-    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving"])
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--no-extras", action="store_true", help="batched mode without the plain and serving legs")
     ap.add_argument("--tiles", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample-iterations", type=int, default=4)
@@ -170,6 +236,12 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (tiles are independent; "
               "multi-GPU placement is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "serving":
+        res = serving(args.requests, args.rate, 30)
+        print(json.dumps({"metric": "serving p99 latency, FP32 mPIE 30 sweeps", "value": res["p99_ms"],
+                          "unit": "ms", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+                          "config": {"workload": "Poisson arrivals", **res}}), flush=True)
+        return
     log("building workload")
     model, meas = workload()
     log("workload ready", model.reconstruction_shape)
@@ -194,7 +266,7 @@ def main():
         achieved = B * tiles_per_s / 1e9
         note = "whole-tile algorithmic bytes / wall time"
     line = {
-        "metric": "mPIE tiles/sec, 225 LEDs x 128^2 tiles (375^2 object), 50 sweeps",
+        "metric": "mPIE tiles/sec, 225 LEDs x 128^2 tiles (375^2 object), 50 sweeps: batched FP32 vs plain FP64; p99 latency",
         "value": tiles_per_s,
         "unit": "tiles/s",
         "n_gpus": 1,
@@ -211,10 +283,18 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "note": note},
     }
+    if args.mode == "batched" and not args.no_extras:
+        log("plain complex128 GPU leg")
+        psecs = run_plain(model, meas, args.iterations, 1, 1, "c128")
+        line["plain_c128_tiles_per_s"] = 1.0 / psecs
+        line["speedup_vs_plain_c128"] = tiles_per_s * psecs
+        log("serving leg")
+        line["serving"] = serving(args.requests, args.rate, 30)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
         line["cpu_baseline"] = cpu_baseline(model, meas, args.iterations, args.cpu_sample_iterations, threads)
+        line["speedup_vs_cpu"] = tiles_per_s / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 
 

@@ -38,6 +38,95 @@ __global__ void k_set_counter(uint64_t* counter, uint64_t value) {
 
 }  // namespace
 
+namespace fpmhip {
+
+void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame_loss, char* per16,
+                    uint32_t trace_len) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  t = TileDesc{};
+  t.spec = s.b.spec.as<double2>();
+  t.vel = s.b.vel.as<double2>();
+  t.anc = s.b.anc.as<double2>();
+  t.pupil = s.b.pupil.as<float2>();
+  t.frames = me.d_frames.ptr;
+  t.masks = me.d_masks.as<uint8_t>();
+  t.weights = me.d_weights.as<double>();
+  t.gains = m.d_gains.as<double>();
+  t.bg = m.d_background.as<double>();
+  t.bg_len = m.background.size();
+  t.src = m.d_src.as<SrcParam>();
+  t.order = nullptr;
+  t.trace = trace;
+  t.frame_loss = frame_loss;
+  t.counter = reinterpret_cast<uint64_t*>(per16);
+  t.iteration = reinterpret_cast<uint32_t*>(per16 + 8);
+  t.status = reinterpret_cast<int32_t*>(per16 + 12);
+  t.trace_len = trace_len;
+  t.n = m.nh;
+  t.rh = m.rh;
+  t.rw = m.rw;
+  t.F = (uint32_t)m.F;
+  t.interval = (uint32_t)s.cfg.interval;
+  t.is_u16 = me.is_u16 ? 1 : 0;
+  t.denom = s.cfg.denom;
+  t.momentum = s.cfg.momentum ? 1 : 0;
+  t.loss = s.cfg.loss;
+  t.object_step = s.cfg.object_step;
+  t.alpha = s.cfg.alpha;
+  t.eps = s.cfg.epsilon;
+  double maxp = 0.0;
+  for (uint64_t p = 0; p < m.N(); ++p)
+    maxp = std::max(maxp, m.pupil[2 * p] * m.pupil[2 * p] + m.pupil[2 * p + 1] * m.pupil[2 * p + 1]);
+  t.max_p = maxp;
+  t.friction = s.cfg.friction;
+  t.feedback = s.cfg.feedback;
+}
+
+void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream) {
+  // packed tables W_n^j (forward, inverse) for n = 32, 64, 128
+  std::vector<float> tw(2 * kTwTotal);
+  for (uint32_t n : {32u, 64u, 128u}) {
+    const uint32_t off = tw_offset(n);
+    for (uint32_t j = 0; j < n; ++j)
+      for (int d = 0; d < 2; ++d) {
+        const int sign = d == 0 ? -1 : 1;
+        const long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+        double re, im;
+        unit_root(j, n, sign, &re, &im);
+        const bool exact = (4ull * j) % n == 0;
+        tw[2 * (off + d * n + j)] = exact ? (float)re : (float)cosl(th);
+        tw[2 * (off + d * n + j) + 1] = exact ? (float)im : (float)(sign * sinl(th));
+      }
+  }
+  out.allocate(tw.size() * sizeof(float));
+  HIP_CHECK(hipMemcpyAsync(out.ptr, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+bool sweep_supports(const fpmhip_session& s, const char** why) {
+  const Model& m = *s.model;
+  if (s.precision != FPMHIP_MIXED_F32) {
+    *why = "batched tiles must use MIXED_F32 sessions";
+    return false;
+  }
+  if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128)) {
+    *why = "batched tiles must be square 32, 64 or 128 pixel images";
+    return false;
+  }
+  if (m.mux) {
+    *why = "batched tiles do not support multiplexed frames";
+    return false;
+  }
+  if (s.alg.kind == FPMHIP_EPRY) {
+    *why = "batched tiles do not support EPRY";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace fpmhip
+
 struct fpmhip_batch {
   Context* ctx = nullptr;
   std::vector<fpmhip_session*> sessions;
@@ -70,50 +159,9 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
   // per tile: counter (u64), iteration (u32), status (i32)
   b.per_tile.allocate((size_t)count * 16);
   b.host_tiles.resize(count);
-  for (uint32_t i = 0; i < count; ++i) {
-    fpmhip_session& s = *b.sessions[i];
-    const Model& m = *s.model;
-    const Measurements& me = *s.meas;
-    TileDesc& t = b.host_tiles[i];
-    t = TileDesc{};
-    t.spec = s.b.spec.as<double2>();
-    t.vel = s.b.vel.as<double2>();
-    t.anc = s.b.anc.as<double2>();
-    t.pupil = s.b.pupil.as<float2>();
-    t.frames = me.d_frames.ptr;
-    t.masks = me.d_masks.as<uint8_t>();
-    t.weights = me.d_weights.as<double>();
-    t.gains = m.d_gains.as<double>();
-    t.bg = m.d_background.as<double>();
-    t.bg_len = m.background.size();
-    t.src = m.d_src.as<SrcParam>();
-    t.order = nullptr;
-    t.trace = b.traces.as<double>() + (size_t)i * trace_cap;
-    t.frame_loss = b.losses.as<double>() + (size_t)i * b.fmax;
-    char* base = b.per_tile.as<char>() + (size_t)i * 16;
-    t.counter = reinterpret_cast<uint64_t*>(base);
-    t.iteration = reinterpret_cast<uint32_t*>(base + 8);
-    t.status = reinterpret_cast<int32_t*>(base + 12);
-    t.trace_len = trace_cap;
-    t.n = m.nh;
-    t.rh = m.rh;
-    t.rw = m.rw;
-    t.F = (uint32_t)m.F;
-    t.interval = (uint32_t)s.cfg.interval;
-    t.is_u16 = me.is_u16 ? 1 : 0;
-    t.denom = s.cfg.denom;
-    t.momentum = s.cfg.momentum ? 1 : 0;
-    t.loss = s.cfg.loss;
-    t.object_step = s.cfg.object_step;
-    t.alpha = s.cfg.alpha;
-    t.eps = s.cfg.epsilon;
-    double maxp = 0.0;
-    for (uint64_t p = 0; p < m.N(); ++p)
-      maxp = std::max(maxp, m.pupil[2 * p] * m.pupil[2 * p] + m.pupil[2 * p + 1] * m.pupil[2 * p + 1]);
-    t.max_p = maxp;
-    t.friction = s.cfg.friction;
-    t.feedback = s.cfg.feedback;
-  }
+  for (uint32_t i = 0; i < count; ++i)
+    fill_tile_desc(b.host_tiles[i], *b.sessions[i], b.traces.as<double>() + (size_t)i * trace_cap,
+                   b.losses.as<double>() + (size_t)i * b.fmax, b.per_tile.as<char>() + (size_t)i * 16, trace_cap);
   b.tiles.allocate(count * sizeof(TileDesc));
   HIP_CHECK(hipMemcpyAsync(b.tiles.ptr, b.host_tiles.data(), count * sizeof(TileDesc), hipMemcpyHostToDevice,
                            b.ctx->stream));
@@ -151,12 +199,9 @@ int fpmhip_batch_create(fpmhip_context* c, fpmhip_session* const* sessions, uint
       fpmhip_session* s = sessions[i];
       if (!s) fail(FPMHIP_INVALID_PARAMETER, "null session", "sessions");
       if (s->ctx != c) fail(FPMHIP_INVALID_PARAMETER, "sessions must share the batch context", "sessions");
-      if (s->precision != FPMHIP_MIXED_F32) fail(FPMHIP_UNSUPPORTED, "batched tiles must use MIXED_F32 sessions");
+      const char* why = "";
+      if (!sweep_supports(*s, &why)) fail(FPMHIP_UNSUPPORTED, why);
       const Model& m = *s->model;
-      if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128))
-        fail(FPMHIP_UNSUPPORTED, "batched tiles must be square 32, 64 or 128 pixel images");
-      if (m.mux) fail(FPMHIP_UNSUPPORTED, "batched tiles do not support multiplexed frames");
-      if (s->alg.kind == FPMHIP_EPRY) fail(FPMHIP_UNSUPPORTED, "batched tiles do not support EPRY");
       if (kind >= 0 && kind != s->alg.kind) fail(FPMHIP_INVALID_PARAMETER, "tiles must share one algorithm", "sessions");
       kind = s->alg.kind;
       b->nmax = std::max(b->nmax, m.nh);
@@ -165,23 +210,7 @@ int fpmhip_batch_create(fpmhip_context* c, fpmhip_session* const* sessions, uint
     }
     if (sweep_lds_bytes(b->nmax) > (size_t)c->props.sharedMemPerBlockOptin && c->props.sharedMemPerBlockOptin > 0)
       fail(FPMHIP_UNSUPPORTED, "tile does not fit the workgroup's LDS");
-    // packed twiddle tables W_n^j (forward, inverse) for n = 32, 64, 128
-    std::vector<float> tw(2 * kTwTotal);
-    for (uint32_t n : {32u, 64u, 128u}) {
-      const uint32_t off = tw_offset(n);
-      for (uint32_t j = 0; j < n; ++j)
-        for (int d = 0; d < 2; ++d) {
-          const int sign = d == 0 ? -1 : 1;
-          long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
-          double re, im;
-          unit_root(j, n, sign, &re, &im);
-          const bool exact = (4ull * j) % n == 0;
-          tw[2 * (off + d * n + j)] = exact ? (float)re : (float)cosl(th);
-          tw[2 * (off + d * n + j) + 1] = exact ? (float)im : (float)(sign * sinl(th));
-        }
-    }
-    b->tw.allocate(tw.size() * sizeof(float));
-    HIP_CHECK(hipMemcpyAsync(b->tw.ptr, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    make_sweep_twiddles(b->tw, c->stream);
     build_tiles(*b, 64);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     *out = b.release();

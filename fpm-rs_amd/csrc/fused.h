@@ -40,6 +40,14 @@ constexpr uint32_t kTwTotal = 2 * (32 + 64 + 128);
 constexpr uint32_t kFusedMaxN = 128;
 
 size_t sweep_lds_bytes(uint32_t nmax);
+// Descriptor of one MIXED_F32 session's tile; per16 points at 16 device bytes
+// (counter u64, iteration u32, status i32).
+void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame_loss, char* per16,
+                    uint32_t trace_len);
+// Packed FP32 twiddle tables of the sweep kernel (kTwTotal float2).
+void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream);
+// Whether the fused sweep can run this session (else *why says why not).
+bool sweep_supports(const fpmhip_session& s, const char** why);
 // Raises the kernel's dynamic LDS limit (outside any graph capture).
 void prepare_sweep(uint32_t nmax);
 void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, hipStream_t stream);

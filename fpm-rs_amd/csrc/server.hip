@@ -1,0 +1,446 @@
+// server.hip — serving form: continuous batching of reconstruction jobs.
+//
+// A worker thread owns the device. Jobs are queued by fpmhip_server_submit,
+// admitted while tile slots and the HBM budget allow, initialised on the
+// device, and then advanced one sweep per tick together with every other
+// resident tile by one launch of the fused sweep kernel. A job that reaches
+// its iteration count is finished (result transform, download) and leaves;
+// the queue is re-examined between ticks. Tiles are independent workgroups,
+// so a job's result does not depend on which other jobs share its ticks.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "fused.h"
+
+using namespace fpmhip;
+
+namespace {
+
+template <class F>
+int guarded(F&& body) {
+  clear_error();
+  try {
+    body();
+    return FPMHIP_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  } catch (const std::bad_alloc&) {
+    return capture_bad_alloc();
+  }
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+enum JobState { kQueued = 0, kResident = 1, kDone = 2, kFailed = 3 };
+
+struct Job {
+  uint64_t id = 0;
+  // host copies of the inputs
+  std::vector<uint64_t> crop, row_ptr, msrc;
+  std::vector<double> offset, pupil, gain, bg, kv, mw;
+  std::vector<uint8_t> support;
+  std::vector<uint16_t> counts;
+  std::vector<double> frames, weights;
+  fpmhip_model_desc desc{};
+  fpmhip_algorithm alg{};
+  uint64_t iterations = 0, done_iterations = 0;
+  uint64_t bytes = 0;
+  // device objects while resident
+  fpmhip_model* model = nullptr;
+  fpmhip_measurements* meas = nullptr;
+  fpmhip_session* sess = nullptr;
+  DeviceBuffer scratch;  // [16 B per-tile state][trace][frame losses]
+  // results
+  int state = kQueued;
+  int error_code = 0;
+  std::string error;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  std::vector<double> object, spectrum, trace;
+
+  void release_device() {
+    if (sess) fpmhip_session_destroy(sess);
+    if (meas) fpmhip_measurements_destroy(meas);
+    if (model) fpmhip_model_destroy(model);
+    sess = nullptr;
+    meas = nullptr;
+    model = nullptr;
+    scratch.release();
+  }
+};
+
+}  // namespace
+
+struct fpmhip_server {
+  fpmhip_context own;  // private stream: the worker never shares the caller's stream
+  Context* ctx = nullptr;
+  fpmhip_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> queue;
+  std::map<uint64_t, std::unique_ptr<Job>> jobs;
+  std::vector<Job*> resident;
+  uint64_t resident_bytes = 0;
+  uint64_t next_id = 1;
+  bool stop = false;
+  fpmhip_server_statistics stats{};
+  std::thread worker;
+  DeviceBuffer tw, tiles;
+  uint32_t prepared_nmax = 0;
+
+  void run();
+  bool admit(Job& j);
+  void finish(Job& j);
+};
+
+namespace {
+
+uint64_t job_bytes(const fpmhip_model_desc& d, uint64_t iterations) {
+  const uint64_t N = d.image_h * d.image_w, M = d.recon_h * d.recon_w;
+  const uint64_t F = d.mux_row_ptr ? d.frame_count : d.source_count;
+  // spectrum, velocity, anchor, scratch (FP64 complex) + per-pixel session
+  // buffers + counts + model arrays + per-tile scratch
+  return 4 * M * 16 + 12 * N * 16 + F * N * 2 + N * 25 + F * 64 + 16 + (iterations + F) * 8;
+}
+
+}  // namespace
+
+bool fpmhip_server::admit(Job& j) {
+  int rc = fpmhip_model_create(&own, &j.desc, &j.model);
+  if (!rc)
+    rc = fpmhip_measurements_create(&own, j.model, j.counts.empty() ? nullptr : j.counts.data(),
+                                    j.frames.empty() ? nullptr : j.frames.data(),
+                                    j.weights.empty() ? nullptr : j.weights.data(), nullptr, &j.meas);
+  if (!rc) rc = fpmhip_session_create(j.model, j.meas, FPMHIP_MIXED_F32, &j.alg, &j.sess);
+  const char* why = "";
+  if (!rc && !sweep_supports(*j.sess, &why)) {
+    rc = FPMHIP_UNSUPPORTED;
+    j.error = why;
+  }
+  if (!rc) rc = fpmhip_session_initialize(j.sess);
+  if (rc) {
+    j.error_code = rc;
+    if (j.error.empty()) j.error = fpmhip_last_error();
+    j.release_device();
+    return false;
+  }
+  j.scratch.allocate(16 + (j.iterations + j.sess->model->F) * sizeof(double));
+  HIP_CHECK(hipMemsetAsync(j.scratch.ptr, 0, j.scratch.bytes, ctx->stream));
+  // the counter starts from the session's (zero after initialisation)
+  return true;
+}
+
+void fpmhip_server::finish(Job& j) {
+  fpmhip_session& s = *j.sess;
+  const uint64_t M = s.model->M();
+  j.object.resize(2 * M);
+  j.spectrum.resize(2 * M);
+  j.trace.resize(j.iterations);
+  int rc = fpmhip_session_get_spectrum(&s, j.spectrum.data());
+  if (!rc) rc = fpmhip_session_get_object(&s, j.object.data());
+  if (!rc) {
+    HIP_CHECK(hipMemcpyAsync(j.trace.data(), j.scratch.as<char>() + 16, j.iterations * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream));
+    int32_t status = 0;
+    HIP_CHECK(hipMemcpyAsync(&status, j.scratch.as<char>() + 12, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (status) {
+      rc = FPMHIP_NUMERICAL;
+      j.error = "mPIE momentum produced a non-finite object or velocity";
+    }
+  } else {
+    j.error = fpmhip_last_error();
+  }
+  j.error_code = rc;
+  j.release_device();
+}
+
+void fpmhip_server::run() {
+  (void)hipSetDevice(ctx->device);
+  double busy_since = -1.0;
+  for (;;) {
+    std::vector<Job*> to_admit;
+    {
+      std::unique_lock<std::mutex> lock(mu);
+      cv_work.wait(lock, [&] { return stop || !queue.empty() || !resident.empty(); });
+      if (stop) break;
+      // admission by slots and HBM footprint (a job larger than the whole
+      // budget is still admitted when nothing else is resident)
+      uint64_t bytes = resident_bytes;
+      size_t slots = resident.size();
+      while (!queue.empty() && slots < cfg.max_tiles) {
+        Job* j = jobs[queue.front()].get();
+        if (slots > 0 && bytes + j->bytes > cfg.hbm_budget_bytes) break;
+        queue.pop_front();
+        to_admit.push_back(j);
+        bytes += j->bytes;
+        ++slots;
+      }
+    }
+    for (Job* j : to_admit) {
+      bool ok = false;
+      try {
+        ok = admit(*j);
+      } catch (const Failure& f) {
+        j->error_code = f.code;
+        j->error = fpmhip_last_error();
+        j->release_device();
+      }
+      std::lock_guard<std::mutex> lock(mu);
+      if (ok) {
+        j->state = kResident;
+        j->t_admit = now_s();
+        resident.push_back(j);
+        resident_bytes += j->bytes;
+        stats.admitted++;
+      } else {
+        j->state = kFailed;
+        j->t_done = now_s();
+        stats.failed++;
+        cv_done.notify_all();
+      }
+    }
+    std::vector<Job*> active;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      active = resident;
+    }
+    if (active.empty()) {
+      if (busy_since >= 0.0) {
+        std::lock_guard<std::mutex> lock(mu);
+        stats.busy_seconds += now_s() - busy_since;
+        busy_since = -1.0;
+      }
+      continue;
+    }
+    if (busy_since < 0.0) busy_since = now_s();
+    // one tick: one sweep of every resident tile
+    try {
+      std::vector<TileDesc> desc(active.size());
+      uint32_t nmax = 0;
+      for (size_t i = 0; i < active.size(); ++i) {
+        Job& j = *active[i];
+        char* base = j.scratch.as<char>();
+        fill_tile_desc(desc[i], *j.sess, reinterpret_cast<double*>(base + 16),
+                       reinterpret_cast<double*>(base + 16 + j.iterations * sizeof(double)), base,
+                       (uint32_t)j.iterations);
+        nmax = std::max(nmax, desc[i].n);
+      }
+      if (tiles.bytes < desc.size() * sizeof(TileDesc)) tiles.allocate(cfg.max_tiles * sizeof(TileDesc));
+      HIP_CHECK(hipMemcpyAsync(tiles.ptr, desc.data(), desc.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
+                               ctx->stream));
+      if (nmax != prepared_nmax) {
+        prepare_sweep(nmax);
+        prepared_nmax = nmax;
+      }
+      launch_sweep(tiles.as<TileDesc>(), (uint32_t)desc.size(), tw.as<float2>(), nmax, ctx->stream);
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const Failure& f) {
+      std::lock_guard<std::mutex> lock(mu);
+      for (Job* j : active) {
+        j->error_code = f.code;
+        j->error = "device failure during a sweep";
+        j->release_device();
+        j->state = kFailed;
+        j->t_done = now_s();
+        stats.failed++;
+      }
+      resident.clear();
+      resident_bytes = 0;
+      cv_done.notify_all();
+      continue;
+    }
+    std::vector<Job*> finished;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      stats.ticks++;
+      for (Job* j : active)
+        if (++j->done_iterations >= j->iterations) finished.push_back(j);
+    }
+    for (Job* j : finished) {
+      try {
+        finish(*j);
+      } catch (const Failure& f) {
+        j->error_code = f.code;
+        j->error = fpmhip_last_error();
+        j->release_device();
+      }
+      std::lock_guard<std::mutex> lock(mu);
+      j->t_done = now_s();
+      j->state = j->error_code ? kFailed : kDone;
+      if (j->error_code)
+        stats.failed++;
+      else
+        stats.completed++;
+      resident.erase(std::find(resident.begin(), resident.end(), j));
+      resident_bytes -= j->bytes;
+      cv_done.notify_all();
+    }
+  }
+  // drop resident device state on shutdown
+  std::lock_guard<std::mutex> lock(mu);
+  for (Job* j : resident) j->release_device();
+  resident.clear();
+}
+
+extern "C" {
+
+int fpmhip_server_create(fpmhip_context* c, const fpmhip_server_config* config, fpmhip_server** out) {
+  return guarded([&] {
+    if (!c || !config || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "server");
+    if (config->max_tiles == 0 || config->max_tiles > 65535)
+      fail(FPMHIP_INVALID_PARAMETER, "max_tiles must be in 1..65535", "max_tiles");
+    HIP_CHECK(hipSetDevice(c->device));
+    auto s = std::make_unique<fpmhip_server>();
+    s->own.device = c->device;
+    s->own.props = c->props;
+    HIP_CHECK(hipStreamCreateWithFlags(&s->own.stream, hipStreamNonBlocking));
+    s->ctx = &s->own;
+    s->cfg = *config;
+    make_sweep_twiddles(s->tw, s->own.stream);
+    s->tiles.allocate(config->max_tiles * sizeof(TileDesc));
+    fpmhip_server* raw = s.get();
+    s->worker = std::thread([raw] { raw->run(); });
+    *out = s.release();
+  });
+}
+
+int fpmhip_server_destroy(fpmhip_server* s) {
+  return guarded([&] {
+    if (!s) return;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& kv : s->jobs) kv.second->release_device();
+    hipStream_t st = s->own.stream;
+    delete s;
+    if (st) (void)hipStreamDestroy(st);
+  });
+}
+
+int fpmhip_server_submit(fpmhip_server* s, const fpmhip_model_desc* d, const uint16_t* counts, const double* frames,
+                         const double* weights, const fpmhip_algorithm* alg, uint64_t* job_id) {
+  return guarded([&] {
+    if (!s || !d || !alg || !job_id) fail(FPMHIP_INVALID_PARAMETER, "null argument", "submit");
+    if ((counts == nullptr) == (frames == nullptr))
+      fail(FPMHIP_INVALID_MEASUREMENTS, "exactly one of uint16 counts and f64 frames is required");
+    if (int rc = fpmhip_algorithm_validate(alg)) throw Failure{rc};
+    if (alg->iterations == 0) fail(FPMHIP_INVALID_PARAMETER, "a served job needs at least one iteration", "iterations");
+    auto j = std::make_unique<Job>();
+    const uint64_t N = d->image_h * d->image_w, S = d->source_count;
+    const uint64_t F = d->mux_row_ptr ? d->frame_count : S;
+    j->crop.assign(d->crop_start, d->crop_start + 2 * S);
+    if (d->offset) j->offset.assign(d->offset, d->offset + 2 * S);
+    j->pupil.assign(d->pupil, d->pupil + 2 * N);
+    j->support.assign(d->support, d->support + N);
+    if (d->frame_gain) j->gain.assign(d->frame_gain, d->frame_gain + F);
+    if (d->background && d->background_len) j->bg.assign(d->background, d->background + d->background_len);
+    if (d->k_vectors) j->kv.assign(d->k_vectors, d->k_vectors + 2 * S);
+    if (d->mux_row_ptr) {
+      j->row_ptr.assign(d->mux_row_ptr, d->mux_row_ptr + F + 1);
+      j->msrc.assign(d->mux_source, d->mux_source + j->row_ptr.back());
+      j->mw.assign(d->mux_weight, d->mux_weight + j->row_ptr.back());
+    }
+    if (counts) j->counts.assign(counts, counts + F * N);
+    if (frames) j->frames.assign(frames, frames + F * N);
+    if (weights) j->weights.assign(weights, weights + F);
+    j->desc = *d;
+    j->desc.crop_start = j->crop.data();
+    j->desc.offset = j->offset.empty() ? nullptr : j->offset.data();
+    j->desc.pupil = j->pupil.data();
+    j->desc.support = j->support.data();
+    j->desc.frame_gain = j->gain.empty() ? nullptr : j->gain.data();
+    j->desc.background = j->bg.empty() ? nullptr : j->bg.data();
+    j->desc.background_len = j->bg.size();
+    j->desc.k_vectors = j->kv.empty() ? nullptr : j->kv.data();
+    j->desc.mux_row_ptr = j->row_ptr.empty() ? nullptr : j->row_ptr.data();
+    j->desc.mux_source = j->msrc.empty() ? nullptr : j->msrc.data();
+    j->desc.mux_weight = j->mw.empty() ? nullptr : j->mw.data();
+    j->alg = *alg;
+    j->iterations = alg->iterations;
+    j->bytes = job_bytes(*d, alg->iterations);
+    j->t_submit = now_s();
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      j->id = s->next_id++;
+      *job_id = j->id;
+      s->queue.push_back(j->id);
+      s->jobs[j->id] = std::move(j);
+      s->stats.submitted++;
+    }
+    s->cv_work.notify_all();
+  });
+}
+
+int fpmhip_server_poll(fpmhip_server* s, uint64_t id, int32_t* state) {
+  return guarded([&] {
+    if (!s || !state) fail(FPMHIP_INVALID_PARAMETER, "null argument", "poll");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) fail(FPMHIP_INVALID_PARAMETER, "unknown job id", "job_id");
+    *state = it->second->state;
+  });
+}
+
+int fpmhip_server_wait(fpmhip_server* s, uint64_t id, double timeout_s) {
+  return guarded([&] {
+    if (!s) fail(FPMHIP_INVALID_PARAMETER, "null argument", "wait");
+    std::unique_lock<std::mutex> lock(s->mu);
+    auto done = [&] {
+      auto it = s->jobs.find(id);
+      return it == s->jobs.end() || it->second->state >= kDone;
+    };
+    if (timeout_s > 0) {
+      if (!s->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_s), done))
+        fail(FPMHIP_UNSUPPORTED, "timed out waiting for the job");
+    } else {
+      s->cv_done.wait(lock, done);
+    }
+    if (s->jobs.find(id) == s->jobs.end()) fail(FPMHIP_INVALID_PARAMETER, "unknown job id", "job_id");
+  });
+}
+
+int fpmhip_server_result(fpmhip_server* s, uint64_t id, double* object, double* spectrum, double* trace,
+                         double* latency_s, double* queue_s) {
+  return guarded([&] {
+    if (!s) fail(FPMHIP_INVALID_PARAMETER, "null argument", "result");
+    std::unique_ptr<Job> j;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      auto it = s->jobs.find(id);
+      if (it == s->jobs.end()) fail(FPMHIP_INVALID_PARAMETER, "unknown job id", "job_id");
+      if (it->second->state < kDone) fail(FPMHIP_UNSUPPORTED, "job is not finished");
+      j = std::move(it->second);
+      s->jobs.erase(it);
+    }
+    if (j->state == kFailed) fail(j->error_code ? j->error_code : FPMHIP_DEVICE_ERROR, j->error);
+    if (object) std::memcpy(object, j->object.data(), j->object.size() * sizeof(double));
+    if (spectrum) std::memcpy(spectrum, j->spectrum.data(), j->spectrum.size() * sizeof(double));
+    if (trace) std::memcpy(trace, j->trace.data(), j->trace.size() * sizeof(double));
+    if (latency_s) *latency_s = j->t_done - j->t_submit;
+    if (queue_s) *queue_s = j->t_admit - j->t_submit;
+  });
+}
+
+int fpmhip_server_stats(fpmhip_server* s, fpmhip_server_statistics* out) {
+  return guarded([&] {
+    if (!s || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "stats");
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = s->stats;
+    out->resident_tiles = s->resident.size();
+    out->resident_bytes = s->resident_bytes;
+  });
+}
+
+}  // extern "C"

@@ -27,7 +27,8 @@ EXPORTS = [
     "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
     "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
-    "fpmhip_batch_timing",
+    "fpmhip_batch_timing", "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
+    "fpmhip_server_poll", "fpmhip_server_wait", "fpmhip_server_result", "fpmhip_server_stats",
 ]
 
 
@@ -340,3 +341,76 @@ class Batch:
         mean, mx, n = ct.c_double(), ct.c_double(), ct.c_uint64()
         check(lib().fpmhip_batch_timing(self.handle, ct.byref(mean), ct.byref(mx), ct.byref(n)))
         return mean.value, mx.value, n.value
+
+
+class ServerConfig(ct.Structure):
+    _fields_ = [("max_tiles", ct.c_uint64), ("hbm_budget_bytes", ct.c_uint64)]
+
+
+class ServerStatistics(ct.Structure):
+    _fields_ = [("submitted", ct.c_uint64), ("admitted", ct.c_uint64), ("completed", ct.c_uint64),
+                ("failed", ct.c_uint64), ("ticks", ct.c_uint64), ("resident_tiles", ct.c_uint64),
+                ("resident_bytes", ct.c_uint64), ("busy_seconds", ct.c_double)]
+
+
+class Server:
+    """Continuous-batching server (fpmhip_server): jobs are admitted by HBM
+    footprint and tile slots and advanced one fused sweep per tick."""
+
+    def __init__(self, max_tiles=256, hbm_budget_bytes=64 << 30, context=None):
+        self.ctx = context or default_context()
+        cfg = ServerConfig(max_tiles, hbm_budget_bytes)
+        self.handle = ct.c_void_p()
+        check(lib().fpmhip_server_create(self.ctx.handle, ct.byref(cfg), ct.byref(self.handle)))
+        self._shapes = {}
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_server_destroy(self.handle)
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, problem: ReconstructionProblem, algorithm) -> int:
+        """Queue one tile job; returns its id."""
+        m = problem.model
+        d = m.desc()
+        meas = problem.measurements
+        alg = algorithm if isinstance(algorithm, abi.Algorithm) else algorithm.to_c()
+        if not isinstance(algorithm, abi.Algorithm):
+            alg.iterations = int(algorithm.iterations)
+        jid = ct.c_uint64(0)
+        check(lib().fpmhip_server_submit(
+            self.handle, ct.byref(d), None if meas.counts is None else abi.ptr(meas.counts, ct.c_uint16),
+            None if meas.frames is None else _cd(meas.frames), None if meas.weights is None else _cd(meas.weights),
+            ct.byref(alg), ct.byref(jid)))
+        self._shapes[jid.value] = (tuple(m.reconstruction_shape), int(alg.iterations))
+        return jid.value
+
+    def poll(self, job_id) -> int:
+        st = ct.c_int32(0)
+        check(lib().fpmhip_server_poll(self.handle, ct.c_uint64(job_id), ct.byref(st)))
+        return st.value
+
+    def wait(self, job_id, timeout_s=0.0):
+        check(lib().fpmhip_server_wait(self.handle, ct.c_uint64(job_id), ct.c_double(timeout_s)))
+
+    def result(self, job_id):
+        """(object, spectrum, trace, latency_s, queue_s) of a finished job."""
+        R, iters = self._shapes.pop(job_id)
+        obj = np.zeros(R, np.complex128)
+        spec = np.zeros(R, np.complex128)
+        trace = np.zeros(iters, np.float64)
+        lat, q = ct.c_double(), ct.c_double()
+        check(lib().fpmhip_server_result(self.handle, ct.c_uint64(job_id), _cd(obj), _cd(spec), _cd(trace),
+                                         ct.byref(lat), ct.byref(q)))
+        return obj, spec, trace, lat.value, q.value
+
+    def stats(self) -> dict:
+        st = ServerStatistics()
+        check(lib().fpmhip_server_stats(self.handle, ct.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStatistics._fields_}

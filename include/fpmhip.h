@@ -300,6 +300,48 @@ int fpmhip_batch_run(fpmhip_batch* batch, uint64_t iterations, double* trace);
 int fpmhip_batch_timing(const fpmhip_batch* batch, double* sweep_ms_mean, double* sweep_ms_max,
                         uint64_t* sweeps);
 
+/* ---- serving form: continuous batching of reconstruction jobs ----
+ * A worker thread owns the device. Submitted jobs wait in a queue and are
+ * admitted while tile slots and the HBM budget allow; every admitted job is
+ * one tile whose state stays resident until it finishes. Each tick is one
+ * fused sweep launch over all resident tiles, each at its own iteration;
+ * finished tiles leave and queued jobs enter between ticks. A job's result
+ * is identical whether it ran alone or among others. Jobs must satisfy the
+ * batched form's constraints (MIXED_F32, square 32/64/128 images, no
+ * multiplexing, not EPRY). */
+typedef struct fpmhip_server_config {
+  uint64_t max_tiles;        /* resident tiles at most (workgroups per tick) */
+  uint64_t hbm_budget_bytes; /* admission limit for resident job state */
+} fpmhip_server_config;
+
+typedef struct fpmhip_server_statistics {
+  uint64_t submitted, admitted, completed, failed;
+  uint64_t ticks;            /* sweep launches */
+  uint64_t resident_tiles;   /* at the time of the call */
+  uint64_t resident_bytes;
+  double busy_seconds;       /* wall time the worker spent with resident tiles */
+} fpmhip_server_statistics;
+
+int fpmhip_server_create(fpmhip_context* context, const fpmhip_server_config* config,
+                         fpmhip_server** out);
+/* Stops the worker after the current tick; unfinished jobs are dropped. */
+int fpmhip_server_destroy(fpmhip_server* server);
+/* Copies the job's inputs (model descriptor arrays, counts or frames,
+ * optional weights) and queues it; returns immediately with its id. */
+int fpmhip_server_submit(fpmhip_server* server, const fpmhip_model_desc* model,
+                         const uint16_t* counts_u16, const double* frames_f64, const double* weights,
+                         const fpmhip_algorithm* algorithm, uint64_t* job_id);
+/* state: 0 queued, 1 resident, 2 done, 3 failed. */
+int fpmhip_server_poll(fpmhip_server* server, uint64_t job_id, int32_t* state);
+/* Blocks until the job is done or failed (timeout_s <= 0: no limit). */
+int fpmhip_server_wait(fpmhip_server* server, uint64_t job_id, double timeout_s);
+/* Results of a finished job: object [R] complex128, spectrum [R] complex128,
+ * trace [iterations] (each may be NULL); latency = completion - submission
+ * (s), queue_s = admission - submission (s). Frees the job's record. */
+int fpmhip_server_result(fpmhip_server* server, uint64_t job_id, double* object, double* spectrum,
+                         double* trace, double* latency_s, double* queue_s);
+int fpmhip_server_stats(fpmhip_server* server, fpmhip_server_statistics* out);
+
 #ifdef __cplusplus
 }
 #endif

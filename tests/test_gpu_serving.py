@@ -1,0 +1,98 @@
+"""Serving form: continuous batching. Jobs of mixed sizes arrive while
+others run; each result equals the same job run alone (bitwise) and the
+FP64 oracle within 1e-5; admission honours the tile-slot limit."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from fpmhip import MeasurementStack, Mpie, ReconstructionProblem, runtime
+from fpmhip import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def job(n, k, seed, wavelength=532e-9, peak=4e3):
+    model = runtime.compile_model(syn.base_optics(wavelength), syn.planar_array(k), (n, n))
+    obj = syn.random_field_object(model.reconstruction_shape, seed)
+    I = runtime.forward_stack(model, syn.object_spectrum(obj))
+    counts, ppp = syn.poisson_counts(I, peak, seed)
+    model.with_frame_gains(np.full(model.frame_count, ppp))
+    return ReconstructionProblem(MeasurementStack(counts=counts), model)
+
+
+ALG = Mpie(iterations=4, momentum_interval=30, momentum_friction=0.5, momentum_feedback=0.5)
+
+
+def test_result_is_identical_alone_and_in_the_mix(ctx):
+    jobs = [job(64, 9, 1, 450e-9), job(128, 7, 2), job(64, 11, 3, 630e-9), job(32, 5, 4)]
+    say("serving mix", [p.model.reconstruction_shape for p in jobs])
+    alone = []
+    for p in jobs:
+        srv = runtime.Server(max_tiles=4)
+        jid = srv.submit(p, ALG)
+        srv.wait(jid, 120)
+        alone.append(srv.result(jid))
+        srv.close()
+    srv = runtime.Server(max_tiles=4)
+    ids = []
+    for i, p in enumerate(jobs):
+        ids.append(srv.submit(p, ALG))
+        time.sleep(0.002 * i)  # staggered arrivals: later jobs join running ticks
+    for jid in ids:
+        srv.wait(jid, 120)
+    mixed = [srv.result(j) for j in ids]
+    st = srv.stats()
+    srv.close()
+    say("stats", st)
+    assert st["completed"] == 4 and st["failed"] == 0
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a[1], b[1])
+        assert np.array_equal(a[2], b[2])
+
+
+def test_served_job_matches_fp64_oracle(ctx):
+    p = job(64, 9, 5)
+    srv = runtime.Server(max_tiles=2)
+    jid = srv.submit(p, ALG)
+    srv.wait(jid, 120)
+    obj, spec, trace, lat, q = srv.result(jid)
+    srv.close()
+    ref = oracle.run(p.model, p.measurements, ALG.to_c(), 4)
+    e = np.linalg.norm(obj - ref["object"]) / np.linalg.norm(ref["object"])
+    say("served vs oracle", e, "latency", lat, "queue", q)
+    assert e <= 1e-5
+    assert np.abs(trace - ref["trace"]).max() / np.abs(ref["trace"]).max() <= 1e-5
+    assert lat >= q >= 0
+
+
+def test_admission_respects_slots_and_reports_failures(ctx):
+    srv = runtime.Server(max_tiles=2)
+    ids = [srv.submit(job(32, 5, 10 + i), ALG) for i in range(5)]
+    seen_max = 0
+    for _ in range(2000):
+        st = srv.stats()
+        seen_max = max(seen_max, st["resident_tiles"])
+        if st["completed"] + st["failed"] == 5:
+            break
+        time.sleep(0.001)
+    for jid in ids:
+        srv.wait(jid, 120)
+        srv.result(jid)
+    assert seen_max <= 2
+    # an unsupported job fails cleanly with a typed error
+    p = job(32, 5, 21)
+    from fpmhip import Epry
+    jid = srv.submit(p, Epry(iterations=2))
+    srv.wait(jid, 60)
+    with pytest.raises(runtime.FpmError) as e:
+        srv.result(jid)
+    assert e.value.kind == "Unsupported"
+    srv.close()
