@@ -60,6 +60,12 @@ __device__ inline double block_sum(double v, double* sh) {
 
 using D = cx<double>;
 
+// MIXED_F32 sessions (float2 buffers) transform the field change psi' - psi.
+template <class V>
+constexpr bool kMixed = false;
+template <>
+constexpr bool kMixed<float2> = true;
+
 // bilinear extract (crop.rs:129-167) of one source, times pupil, ifftshift
 // (forward.rs:538-549) into the field buffer.
 template <class V>
@@ -120,6 +126,9 @@ __global__ void __launch_bounds__(kBlock) k_project(V* __restrict__ fields, uint
     if (K > 1) proj_out[p] = from_d2<V>(proj);
     const double2 f0 = to_d2(fields[p]);
     double2 d = masked ? f0 : (pred > eps ? D::mulr(f0, proj.x) : make_double2(proj.y, 0.0));
+    // MIXED_F32 keeps psi' - psi: FFT(psi' - psi)/N equals FFT(psi')/N - patch P
+    // by linearity and avoids cancelling two nearly equal FP32 spectra.
+    if (kMixed<V>) d = D::sub(d, f0);
     fields[p] = from_d2<V>(d);
   }
   const double s = block_sum<kBlock>(lsum, sh);
@@ -140,6 +149,7 @@ __global__ void k_difference(V* __restrict__ field, const V* __restrict__ proj, 
   const double2 pr = to_d2(proj[p]);
   const bool masked = mask && mask[p] == 0;
   double2 d = masked ? f : (pred[p] > eps ? D::mulr(f, pr.x) : make_double2(pr.y, 0.0));
+  if (kMixed<V>) d = D::sub(d, f);
   field[p] = from_d2<V>(d);
 }
 
@@ -156,7 +166,7 @@ __global__ void k_update(const V* __restrict__ fftdiff, const V* __restrict__ pa
   const double2 ps = to_d2(fftdiff[((r + (h + 1) / 2) % h) * w + (c + (w + 1) / 2) % w]);
   const double2 pa = to_d2(patch[p]);
   const double2 P = to_d2(pupil[p]);
-  const double2 diff = D::sub(ps, D::mul(pa, P));
+  const double2 diff = kMixed<V> ? ps : D::sub(ps, D::mul(pa, P));
   const double local = D::norm2(P);
   const double maxp = fmax(scal[0], eps);
   double den;
