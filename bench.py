@@ -233,8 +233,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (tiles are independent; "
-              "multi-GPU placement is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported: tiles are independent, "
+              "so more GPUs would each run their own batch of tiles, and no figure over several GPUs "
+              "is measured", file=sys.stderr)
         sys.exit(2)
     if args.mode == "serving":
         res = serving(args.requests, args.rate, 30)
