@@ -49,6 +49,7 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.vel = s.b.vel.as<double2>();
   t.anc = s.b.anc.as<double2>();
   t.pupil = s.b.pupil.as<float2>();
+  t.field = m.nh > kFusedLdsMaxN ? s.b.field.as<float2>() : nullptr;  // the session's N-pixel field buffer
   t.frames = me.d_frames.ptr;
   t.masks = me.d_masks.as<uint8_t>();
   t.weights = me.d_weights.as<double>();
@@ -83,10 +84,14 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.feedback = s.cfg.feedback;
 }
 
+void order_tiles(std::vector<TileDesc>& tiles) {
+  std::stable_sort(tiles.begin(), tiles.end(), [](const TileDesc& a, const TileDesc& b) { return a.n > b.n; });
+}
+
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream) {
-  // packed tables W_n^j (forward, inverse) for n = 32, 64, 128
+  // packed tables W_n^j (forward, inverse) for n = 32, 64, 128, 256
   std::vector<float> tw(2 * kTwTotal);
-  for (uint32_t n : {32u, 64u, 128u}) {
+  for (uint32_t n : {32u, 64u, 128u, 256u}) {
     const uint32_t off = tw_offset(n);
     for (uint32_t j = 0; j < n; ++j)
       for (int d = 0; d < 2; ++d) {
@@ -110,8 +115,8 @@ bool sweep_supports(const fpmhip_session& s, const char** why) {
     *why = "batched tiles must use MIXED_F32 sessions";
     return false;
   }
-  if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128)) {
-    *why = "batched tiles must be square 32, 64 or 128 pixel images";
+  if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128 && m.nh != 256)) {
+    *why = "batched tiles must be square 32, 64, 128 or 256 pixel images";
     return false;
   }
   if (m.mux) {
@@ -130,7 +135,8 @@ bool sweep_supports(const fpmhip_session& s, const char** why) {
 struct fpmhip_batch {
   Context* ctx = nullptr;
   std::vector<fpmhip_session*> sessions;
-  std::vector<TileDesc> host_tiles;
+  std::vector<TileDesc> host_tiles;    // in session order
+  std::vector<TileDesc> device_order;  // launch order (largest tiles first)
   DeviceBuffer tiles;      // TileDesc[count]
   DeviceBuffer tw;         // float2[kTwTotal]
   DeviceBuffer per_tile;   // counters, iterations, status
@@ -162,8 +168,10 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
   for (uint32_t i = 0; i < count; ++i)
     fill_tile_desc(b.host_tiles[i], *b.sessions[i], b.traces.as<double>() + (size_t)i * trace_cap,
                    b.losses.as<double>() + (size_t)i * b.fmax, b.per_tile.as<char>() + (size_t)i * 16, trace_cap);
+  b.device_order = b.host_tiles;
+  order_tiles(b.device_order);
   b.tiles.allocate(count * sizeof(TileDesc));
-  HIP_CHECK(hipMemcpyAsync(b.tiles.ptr, b.host_tiles.data(), count * sizeof(TileDesc), hipMemcpyHostToDevice,
+  HIP_CHECK(hipMemcpyAsync(b.tiles.ptr, b.device_order.data(), count * sizeof(TileDesc), hipMemcpyHostToDevice,
                            b.ctx->stream));
   // capture the sweep launch once
   if (b.exec) {

@@ -1,6 +1,8 @@
 // fused.h — descriptor and launcher of the fused per-tile sweep kernel.
 #pragma once
 
+#include <vector>
+
 #include "session.h"
 
 namespace fpmhip {
@@ -11,6 +13,7 @@ struct TileDesc {
   double2* vel;           // [R] mPIE velocity (or null)
   double2* anc;           // [R] mPIE anchor (or null)
   const float2* pupil;    // [N] complex64 pupil
+  float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)
   const void* frames;     // [F*N] uint16 counts or f64 frames
   const uint8_t* masks;   // [F*N] or null
   const double* weights;  // [F]
@@ -32,18 +35,24 @@ struct TileDesc {
 };
 
 // Offset (in float2 units) of the forward/inverse twiddle tables of length n
-// inside the packed table: [32 fwd, 32 inv, 64 fwd, 64 inv, 128 fwd, 128 inv].
+// inside the packed table: [32 fwd, 32 inv, 64 fwd, 64 inv, ..., 256 inv].
 __host__ __device__ inline uint32_t tw_offset(uint32_t n) {
-  return n == 32 ? 0u : n == 64 ? 64u : 192u;
+  return n == 32 ? 0u : n == 64 ? 64u : n == 128 ? 192u : 448u;
 }
-constexpr uint32_t kTwTotal = 2 * (32 + 64 + 128);
-constexpr uint32_t kFusedMaxN = 128;
+constexpr uint32_t kTwTotal = 2 * (32 + 64 + 128 + 256);
+// Largest tile whose whole field is LDS-resident; larger tiles (up to
+// kFusedMaxN) keep it in TileDesc::field.
+constexpr uint32_t kFusedLdsMaxN = 128;
+constexpr uint32_t kFusedMaxN = 256;
 
 size_t sweep_lds_bytes(uint32_t nmax);
 // Descriptor of one MIXED_F32 session's tile; per16 points at 16 device bytes
 // (counter u64, iteration u32, status i32).
 void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame_loss, char* per16,
                     uint32_t trace_len);
+// Launch order of a sweep's tiles: largest first (stable), so the longest
+// workgroups are dispatched first and spread over the XCDs.
+void order_tiles(std::vector<TileDesc>& tiles);
 // Packed FP32 twiddle tables of the sweep kernel (kTwTotal float2).
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream);
 // Whether the fused sweep can run this session (else *why says why not).

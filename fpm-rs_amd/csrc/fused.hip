@@ -134,33 +134,32 @@ __host__ __device__ constexpr int pitch() {
   return N + 1;
 }
 
-// One Stockham stage (radix P, stride S) over all N lines of an N x N array
-// in LDS (element (row, col) at row * (N+1) + col). Consecutive lanes take
-// consecutive lines for row transforms and consecutive columns for column
-// transforms, so every LDS access of a wave is conflict free. Loads complete
-// (barrier) before any store, so the stage runs in place in one buffer.
-template <int N, int P, int S, bool ROWS, bool INV>
+// One Stockham stage (radix P, stride S) over LINES lines of length LEN in
+// LDS; sample e of line l sits at l * LS + e * ES. Consecutive lanes take
+// consecutive lines, so with LS = 1 or LS = an odd number of 8-byte words
+// every LDS access of a wave is conflict free. Loads complete (barrier)
+// before any store, so the stage runs in place in one buffer.
+template <int LEN, int LINES, int P, int S, int LS, int ES, bool INV>
 __device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid) {
   asm volatile("" : "+v"(tid));
-  constexpr int M = N / (P * S);
-  constexpr int BPL = N / P;
-  constexpr int TOTAL = N * BPL;
+  constexpr int M = LEN / (P * S);
+  constexpr int BPL = LEN / P;
+  constexpr int TOTAL = LINES * BPL;
   constexpr int PER = (TOTAL + kThreads - 1) / kThreads;
-  constexpr int LD = pitch<N>();
   float2 a[PER][P];
   int line[PER], pp[PER], q[PER];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int b = tid + u * kThreads;
-    line[u] = b % N;
-    const int t = b / N;
+    line[u] = b % LINES;
+    const int t = b / LINES;
     pp[u] = t / S;
     q[u] = t % S;
     if (b < TOTAL) {
 #pragma unroll
       for (int r = 0; r < P; ++r) {
         const int e = q[u] + S * (pp[u] + r * M);
-        a[u][r] = ROWS ? lds[line[u] * LD + e] : lds[e * LD + line[u]];
+        a[u][r] = lds[line[u] * LS + e * ES];
       }
     }
   }
@@ -174,11 +173,8 @@ __device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid) {
       for (int k = 0; k < P; ++k) {
         const int e = q[u] + S * (P * pp[u] + k);
         float2 v = a[u][k];
-        if (k > 0 && M > 1) v = fmul(v, tw[(pp[u] * k * S) % N]);
-        if (ROWS)
-          lds[line[u] * LD + e] = v;
-        else
-          lds[e * LD + line[u]] = v;
+        if (k > 0 && M > 1) v = fmul(v, tw[(pp[u] * k * S) % LEN]);
+        lds[line[u] * LS + e * ES] = v;
       }
     }
   }
@@ -199,16 +195,21 @@ template <>
 struct Plan<128> {
   static constexpr int P0 = 16, P1 = 8;
 };
+template <>
+struct Plan<256> {
+  static constexpr int P0 = 16, P1 = 16;
+};
 
 // Unscaled 2-D DFT in LDS with the direction given by `tw`.
 template <int N, bool INV>
 __device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw, int tid) {
   constexpr int P0 = Plan<N>::P0, P1 = Plan<N>::P1;
+  constexpr int LD = pitch<N>();
   static_assert(P0 * P1 == N, "plan");
-  stage<N, P0, 1, true, INV>(lds, tw, tid);
-  stage<N, P1, P0, true, INV>(lds, tw, tid);
-  stage<N, P0, 1, false, INV>(lds, tw, tid);
-  stage<N, P1, P0, false, INV>(lds, tw, tid);
+  stage<N, N, P0, 1, LD, 1, INV>(lds, tw, tid);
+  stage<N, N, P1, P0, LD, 1, INV>(lds, tw, tid);
+  stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
+  stage<N, N, P1, P0, 1, LD, INV>(lds, tw, tid);
 }
 
 // The thread index, hidden from the optimiser so that per-thread addresses
@@ -268,13 +269,65 @@ __device__ __forceinline__ float point_loss_f(float predicted, float measured, i
   return r * r;
 }
 
+// Measured value of one pixel (uint16 counts or f64).
+__device__ __forceinline__ float measured(const TileDesc& T, uint64_t i) {
+  return T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[i] : (float)static_cast<const double*>(T.frames)[i];
+}
+
+// Amplitude replacement of pixel p of frame f (fbase = f * NN) kept as the
+// field change psi' - psi (common.rs:172-228); adds the pixel's objective
+// term unless it is masked (a masked pixel keeps psi' = psi).
+__device__ __forceinline__ float2 replace_amplitude(const TileDesc& T, uint64_t fbase, uint64_t NN, int p, float2 fv,
+                                                    float y, float gain_f, float inv_gain, float eps_f, float& lsum,
+                                                    int& lcnt) {
+  if (T.masks && T.masks[fbase + p] == 0) return f2(0.0f, 0.0f);
+  const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
+  const float b = T.bg_len == 0 ? 0.0f : (float)T.bg[T.bg_len == NN ? (uint64_t)p : fbase + p];
+  lsum += point_loss_f(fmaf(gain_f, pred, b), y, T.loss);
+  lcnt += 1;
+  const float target = fmaxf((y - b) * inv_gain, 0.0f);
+  if (pred > eps_f) {
+    const float rm1 = fsqrt(target * frcp(pred)) - 1.0f;
+    return f2(fv.x * rm1, fv.y * rm1);
+  }
+  return f2(fsqrt(target) - fv.x, -fv.y);
+}
+
+// mPIE momentum event over the tile's whole R^2 spectrum (common.rs:308-387).
+__device__ __forceinline__ void momentum_event(const TileDesc& T, int tid) {
+  const uint64_t M = (uint64_t)T.rh * T.rw;
+  double2* __restrict__ spec = T.spec;
+  bool bad = false;
+  for (uint64_t i = tid; i < M; i += kThreads) {
+    const double2 o = spec[i], v = T.vel[i], an = T.anc[i];
+    const double2 nv = make_double2(fma(T.friction, v.x, o.x - an.x), fma(T.friction, v.y, o.y - an.y));
+    const double2 no = T.feedback == 0.0 ? o : make_double2(fma(T.feedback, nv.x, o.x), fma(T.feedback, nv.y, o.y));
+    bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
+    T.vel[i] = nv;
+    spec[i] = no;
+    T.anc[i] = no;
+  }
+  if (bad) T.status[0] = 1;
+  __syncthreads();
+}
+
+// End of a tile's sweep: iteration objective into the trace, counters back.
+__device__ __forceinline__ void finish_sweep(const TileDesc& T, double obj_sum, double w_sum, uint64_t counter) {
+  if (threadIdx.x == 0) {
+    const uint32_t it = *T.iteration;
+    if (it < T.trace_len) T.trace[it] = w_sum > 0.0 ? obj_sum / w_sum : __builtin_nan("");
+    *T.iteration = it + 1;
+    *T.counter = counter;
+  }
+}
+
 // One sweep of one tile. Thread t owns a strip of centred pixels (rows
 // 8*(t/N) .. +7 of column t%N for N = 128) and their swap partners
 // ((r + N/2) mod N, (c + N/2) mod N): for even N, fftshift and ifftshift are
 // both that swap, so the shifted LDS reads and writes of phases 1 and 5 stay
 // inside the owning thread, and vertically adjacent pixels share bilinear taps.
 template <int N>
-__device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi, double* red) {
+__device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi, double* red) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int LD = pitch<N>();
@@ -320,11 +373,12 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
           const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
+          // rows of the (n+1)^2 stencil are read once; no read beyond the window
           uint32_t off = (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc;
           double2 a = spec[off], b = spec[off + dc];
 #pragma unroll
           for (int j = 0; j < STRIP; ++j) {
-            const double2 c2 = spec[off + du], d = spec[off + du + dc];
+            const double2 c2 = du ? spec[off + du] : a, d = du ? spec[off + du + dc] : b;
             const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
                                            fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
             const int r = rb + j;
@@ -334,7 +388,7 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
             if (du) {
               a = c2;
               b = d;
-            } else {
+            } else if (j + 1 < STRIP) {
               a = spec[off];
               b = spec[off + dc];
             }
@@ -361,31 +415,14 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
       for (int j = 0; j < PP; ++j) {
         const int p = tid + j * kThreads;
         y[j] = 0.0f;
-        if (p < NN)
-          y[j] = T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[fbase + p]
-                          : (float)static_cast<const double*>(T.frames)[fbase + p];
+        if (p < NN) y[j] = measured(T, fbase + p);
       }
 #pragma unroll
       for (int j = 0; j < PP; ++j) {
         const int p = tid + j * kThreads;
         if (p < NN) {
           const int lp = (p / N) * LD + p % N;
-          const float2 fv = lds[lp];
-          const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
-          float2 d = f2(0.0f, 0.0f);  // masked: psi' = psi
-          if (!(T.masks && T.masks[fbase + p] == 0)) {
-            const float b = T.bg_len == 0 ? 0.0f : (float)T.bg[T.bg_len == (uint64_t)NN ? p : fbase + p];
-            lsum_f += point_loss_f(fmaf(gain_f, pred, b), y[j], T.loss);
-            lcnt_i += 1;
-            const float target = fmaxf((y[j] - b) * inv_gain, 0.0f);
-            if (pred > eps_f) {
-              const float rm1 = fsqrt(target * frcp(pred)) - 1.0f;
-              d = f2(fv.x * rm1, fv.y * rm1);
-            } else {
-              d = f2(fsqrt(target) - fv.x, -fv.y);
-            }
-          }
-          lds[lp] = d;
+          lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, p, lds[lp], y[j], gain_f, inv_gain, eps_f, lsum_f, lcnt_i);
         }
       }
     }
@@ -485,30 +522,256 @@ __device__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, co
     if (T.momentum) {
       counter += 1;
       if (counter >= T.interval) {
-        const uint64_t M = (uint64_t)T.rh * T.rw;
-        bool bad = false;
-        for (uint64_t i = tid; i < M; i += kThreads) {
-          const double2 o = spec[i], v = T.vel[i], an = T.anc[i];
-          const double2 nv = make_double2(fma(T.friction, v.x, o.x - an.x), fma(T.friction, v.y, o.y - an.y));
-          const double2 no =
-              T.feedback == 0.0 ? o : make_double2(fma(T.feedback, nv.x, o.x), fma(T.feedback, nv.y, o.y));
-          bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
-          T.vel[i] = nv;
-          spec[i] = no;
-          T.anc[i] = no;
-        }
-        if (bad) T.status[0] = 1;
+        momentum_event(T, tid);
         counter = 0;
-        __syncthreads();
       }
     }
   }
-  if (threadIdx.x == 0) {
-    const uint32_t it = *T.iteration;
-    if (it < T.trace_len) T.trace[it] = w_sum > 0.0 ? obj_sum / w_sum : __builtin_nan("");
-    *T.iteration = it + 1;
-    *T.counter = counter;
+  finish_sweep(T, obj_sum, w_sum, counter);
+}
+
+// Tiles whose N x N complex64 field does not fit in LDS (N = 256: 512 KiB).
+// The field lives in the tile's global scratch buffer T.field (it stays in
+// L2 / the last-level cache) and passes through LDS in chunks of CH lines:
+//
+//   A  rows, in shifted order: patch x pupil written ifftshifted, inverse
+//      row transforms, back to T.field
+//   B  columns: inverse column transforms, amplitude replacement kept as the
+//      field change psi' - psi, forward column transforms, back to T.field
+//   C  rows, in centred order: forward row transforms, fftshift by index,
+//      U = conj(P) diff / N / den in LDS, then the adjoint insert of the
+//      destination rows this chunk completes (the row above the chunk comes
+//      from `prev`, the previous chunk's last row of U)
+//
+// The LED loop, objective, momentum and arithmetic are those of sweep_tile.
+template <int N>
+struct BigPlan;
+template <>
+struct BigPlan<256> {
+  static constexpr int CH = 64;  // lines per chunk
+};
+
+template <int N>
+__host__ __device__ constexpr int big_field_elems() {
+  return BigPlan<N>::CH * (N + 1) > N * (BigPlan<N>::CH + 1) ? BigPlan<N>::CH * (N + 1) : N * (BigPlan<N>::CH + 1);
+}
+
+// float2 elements of the LDS field region for an N x N tile.
+__host__ __device__ constexpr uint32_t sweep_field_elems(uint32_t n) {
+  return n <= kFusedLdsMaxN ? n * (n + 1) : (uint32_t)big_field_elems<256>();
+}
+
+template <int N>
+__device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, float2* prev, const float2* twf, const float2* twi,
+                               double* red) {
+  constexpr int NN = N * N;
+  constexpr int HALF = N / 2;
+  constexpr int CH = BigPlan<N>::CH;
+  constexpr int NCH = N / CH;
+  constexpr int LDR = N + 1;   // row chunk: CH rows of N (+1 pad)
+  constexpr int LDC = CH + 1;  // column chunk: N rows of CH (+1 pad)
+  constexpr int P0 = Plan<N>::P0, P1 = Plan<N>::P1;
+  constexpr int SPLIT = kThreads / N;  // row strips per chunk in phase A
+  constexpr int STRIP = CH / SPLIT;    // rows per strip
+  static_assert((N & (N - 1)) == 0 && HALF % CH == 0 && CH % SPLIT == 0, "big tile shape");
+  const float inv_n = 1.0f / (float)NN;
+  double2* __restrict__ spec = T.spec;
+  float2* __restrict__ fld = T.field;
+  const float2* __restrict__ pupil = T.pupil;
+  const float eps_f = (float)T.eps;
+  const Denominator den = make_denominator(T, fmax(T.max_p, T.eps));
+  uint64_t counter = *T.counter;
+  double obj_sum = 0.0, w_sum = 0.0;
+  for (uint32_t pos = 0; pos < T.F; ++pos) {
+    const uint32_t f = T.order ? T.order[pos] : pos;
+    const double fw = T.weights[f];
+    int tid = opaque_tid();
+    if (fw == 0.0) {
+      if (tid == 0) T.frame_loss[pos] = 0.0;
+      continue;
+    }
+    const SrcParam sp = T.src[f];
+    const float gain_f = (float)T.gains[f];
+    const float inv_gain = (float)(1.0 / T.gains[f]);
+    // A. shifted row r' holds centred row r' ^ HALF (ifftshift for even N)
+    for (int k = 0; k < NCH; ++k) {
+      tid = opaque_tid();
+      {
+        const int c = tid % N, s0 = (tid / N) * STRIP;
+        const int rc = ((k * CH) ^ HALF) + s0;  // first centred row of the strip
+        const int cs = c ^ HALF;
+        if (sp.zero) {
+#pragma unroll 4
+          for (int j = 0; j < STRIP; ++j) {
+            const int r = rc + j;
+            const double2 v = spec[((uint32_t)sp.sr + r) * T.rw + (uint32_t)sp.sc + c];
+            lds[(s0 + j) * LDR + cs] = fmul(f2((float)v.x, (float)v.y), pupil[r * N + c]);
+          }
+        } else {
+          const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+                       wuu = sp.ar.wu * sp.ac.wu;
+          const uint32_t dc = sp.ac.upper - sp.ac.lower, du = (sp.ar.upper - sp.ar.lower) * T.rw;
+          uint32_t off = (sp.ar.lower + rc) * T.rw + sp.ac.lower + c;
+          double2 a = spec[off], b = spec[off + dc];
+#pragma unroll 4
+          for (int j = 0; j < STRIP; ++j) {
+            const double2 c2 = du ? spec[off + du] : a, d = du ? spec[off + du + dc] : b;
+            const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
+                                           fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
+            lds[(s0 + j) * LDR + cs] = fmul(f2((float)v.x, (float)v.y), pupil[(rc + j) * N + c]);
+            off += T.rw;
+            if (du) {
+              a = c2;
+              b = d;
+            } else if (j + 1 < STRIP) {
+              a = spec[off];
+              b = spec[off + dc];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      stage<N, CH, P0, 1, LDR, 1, true>(lds, twi, tid);
+      stage<N, CH, P1, P0, LDR, 1, true>(lds, twi, tid);
+      for (int i = tid; i < CH * N; i += kThreads) fld[(k * CH + i / N) * N + i % N] = lds[(i / N) * LDR + i % N];
+      __syncthreads();
+    }
+    // B. columns: natural image order after the inverse column transforms
+    const uint64_t fbase = (uint64_t)f * NN;
+    double lsum = 0.0;
+    int lcnt = 0;
+    for (int jc = 0; jc < NCH; ++jc) {
+      tid = opaque_tid();
+      const int c0 = jc * CH;
+      for (int i = tid; i < N * CH; i += kThreads) lds[(i / CH) * LDC + i % CH] = fld[(i / CH) * N + c0 + i % CH];
+      __syncthreads();
+      stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
+      stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
+      {
+        // measurement loads in groups of G issued before their use
+        constexpr int PP = N * CH / kThreads;
+        constexpr int G = 8;
+        float lsum_f = 0.0f;
+#pragma unroll
+        for (int g = 0; g < PP; g += G) {
+          float y[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            const int i = tid + (g + j) * kThreads;
+            y[j] = measured(T, fbase + (uint64_t)(i / CH) * N + c0 + i % CH);
+          }
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            const int i = tid + (g + j) * kThreads;
+            const int lp = (i / CH) * LDC + i % CH;
+            lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, (i / CH) * N + c0 + i % CH, lds[lp], y[j], gain_f,
+                                        inv_gain, eps_f, lsum_f, lcnt);
+          }
+        }
+        lsum += (double)lsum_f;
+      }
+      __syncthreads();
+      stage<N, CH, P0, 1, 1, LDC, false>(lds, twf, tid);
+      stage<N, CH, P1, P0, 1, LDC, false>(lds, twf, tid);
+      for (int i = tid; i < N * CH; i += kThreads) fld[(i / CH) * N + c0 + i % CH] = lds[(i / CH) * LDC + i % CH];
+      __syncthreads();
+    }
+    {
+      const double ls = wg_sum(lsum, red);
+      const double lc = wg_sum((double)lcnt, red);
+      if (tid == 0) {
+        const double L = lc > 0.0 ? ls / lc : 0.0;
+        T.frame_loss[pos] = L;
+        obj_sum += fw * L;
+        w_sum += fw;
+      }
+      __syncthreads();
+    }
+    // C. rows in centred order: centred row q*CH + lr is natural row (q*CH + lr) ^ HALF
+    const double scale = fw * T.object_step;
+    const bool fr = !sp.zero && sp.ar.upper != sp.ar.lower, fc = !sp.zero && sp.ac.upper != sp.ac.lower;
+    const int ww = N + (fc ? 1 : 0);
+    const float wll = sp.zero ? 1.0f : (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
+                wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
+    const uint32_t dr0 = sp.zero ? (uint32_t)sp.sr : sp.ar.lower, dc0 = sp.zero ? (uint32_t)sp.sc : sp.ac.lower;
+    for (int q = 0; q < NCH; ++q) {
+      tid = opaque_tid();
+      const int nr0 = (q * CH) ^ HALF;
+      for (int i = tid; i < CH * N; i += kThreads) lds[(i / N) * LDR + i % N] = fld[(nr0 + i / N) * N + i % N];
+      __syncthreads();
+      stage<N, CH, P0, 1, LDR, 1, false>(lds, twf, tid);
+      stage<N, CH, P1, P0, LDR, 1, false>(lds, twf, tid);
+      for (int i = tid; i < CH * N; i += kThreads) {
+        const int lr = i / N, kc = i % N;
+        const float2 P = pupil[(q * CH + lr) * N + (kc ^ HALF)];
+        const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
+        const float2 u = fmulc(P, lds[lr * LDR + kc]);
+        lds[lr * LDR + kc] = f2(u.x * kk, u.y * kk);
+      }
+      __syncthreads();
+      // destination rows a of this chunk (and row N after the last chunk)
+      const int a0 = q * CH;
+      const int rows = CH + ((q == NCH - 1 && fr) ? 1 : 0);
+      const int total = rows * ww;
+      constexpr int UNROLL = 4;
+      for (int base = tid; base < total; base += UNROLL * kThreads) {
+        double2 old[UNROLL];
+        float2 add[UNROLL];
+        uint32_t off[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+          const int qd = base + k * kThreads;
+          if (qd < total) {
+            const int a = a0 + qd / ww, b = qd % ww;
+            off[k] = (dr0 + a) * T.rw + dc0 + b;
+            old[k] = spec[off[k]];
+            float ax = 0.0f, ay = 0.0f;
+            auto tap = [&](int sr_, int sc_, float wt) {
+              if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
+                const float2 u = sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)] : prev[sc_ ^ HALF];
+                ax = fmaf(u.x, wt, ax);
+                ay = fmaf(u.y, wt, ay);
+              }
+            };
+            if (fr && fc) {
+              tap(a - 1, b - 1, wuu);
+              tap(a - 1, b, wul);
+              tap(a, b - 1, wlu);
+              tap(a, b, wll);
+            } else if (fr) {
+              tap(a - 1, b, wul);
+              tap(a, b, wll);
+            } else if (fc) {
+              tap(a, b - 1, wlu);
+              tap(a, b, wll);
+            } else {
+              tap(a, b, wll);
+            }
+            add[k] = f2(ax, ay);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+          const int qd = base + k * kThreads;
+          if (qd < total)
+            spec[off[k]] = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
+        }
+      }
+      __syncthreads();
+      if (q + 1 < NCH) {
+        for (int i = tid; i < N; i += kThreads) prev[i] = lds[(CH - 1) * LDR + i];
+        __syncthreads();
+      }
+    }
+    if (T.momentum) {
+      counter += 1;
+      if (counter >= T.interval) {
+        momentum_event(T, tid);
+        counter = 0;
+      }
+    }
   }
+  finish_sweep(T, obj_sum, w_sum, counter);
 }
 
 __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__ tiles, const float2* __restrict__ tw_all) {
@@ -516,8 +779,10 @@ __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__
   float2* lds = reinterpret_cast<float2*>(smem);
   const TileDesc T = tiles[blockIdx.x];
   const int N = (int)T.n;
-  // twiddles W_N^j, forward then inverse, after the N x N field
-  float2* twf = lds + N * (N + 1);
+  // LDS: field (or chunk) region, [big tiles: previous row of U], twiddles
+  // W_N^j forward then inverse, reduction slots
+  float2* prev = lds + sweep_field_elems(N);
+  float2* twf = prev + (N > kFusedLdsMaxN ? N : 0);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
   const float2* src = tw_all + tw_offset(N);
@@ -533,6 +798,9 @@ __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__
     case 128:
       sweep_tile<128>(T, lds, twf, twi, red);
       break;
+    case 256:
+      sweep_tile_big<256>(T, lds, prev, twf, twi, red);
+      break;
     default:
       break;
   }
@@ -541,7 +809,8 @@ __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__
 }  // namespace
 
 size_t sweep_lds_bytes(uint32_t nmax) {
-  return (size_t)nmax * (nmax + 1) * sizeof(float2) + 2 * (size_t)nmax * sizeof(float2) + 16 * sizeof(double);
+  return ((size_t)sweep_field_elems(nmax) + (nmax > kFusedLdsMaxN ? nmax : 0) + 2 * (size_t)nmax) * sizeof(float2) +
+         16 * sizeof(double);
 }
 
 void prepare_sweep(uint32_t nmax) {

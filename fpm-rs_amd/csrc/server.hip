@@ -234,6 +234,7 @@ void fpmhip_server::run() {
                        (uint32_t)j.iterations);
         nmax = std::max(nmax, desc[i].n);
       }
+      order_tiles(desc);
       if (tiles.bytes < desc.size() * sizeof(TileDesc)) tiles.allocate(cfg.max_tiles * sizeof(TileDesc));
       HIP_CHECK(hipMemcpyAsync(tiles.ptr, desc.data(), desc.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));

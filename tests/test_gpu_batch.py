@@ -95,3 +95,54 @@ def test_batch_vs_per_stage_mixed_session(ctx):
     e = np.linalg.norm(s.spectrum() - p.spectrum()) / np.linalg.norm(p.spectrum())
     say("fused vs per-stage", e)
     assert e <= 1e-5
+
+
+def test_batch_with_256_tile_matches_fp64_oracle(ctx):
+    """A 256^2 tile (field in the tile's global scratch, chunked through LDS)
+    next to LDS-resident tiles: mPIE with momentum events on 13x13 LEDs
+    (the uneven-batch shape), each tile against the FP64 oracle."""
+    alg = Mpie(iterations=3, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(64, 9, 51), tile(256, 13, 52), tile(128, 7, 53)]
+    say("batch with 256", [t[0].reconstruction_shape for t in tiles])
+    ss = sessions_for(tiles, alg)
+    trace = runtime.Batch(ss).run(3)
+    for (model, meas), s, tr in zip(tiles, ss, trace):
+        ref = oracle.run(model, meas, alg.to_c(), 3)
+        ea, ep = amp_phase_err(s.object(), ref["object"])
+        te = np.abs(tr - ref["trace"]) / np.abs(ref["trace"])
+        say(model.image_shape, "amp", ea, "phase", ep, "trace", te.max())
+        assert ea <= 1e-5 and ep <= 1e-5 and te.max() <= 1e-5
+
+
+def test_batch_256_masks_background_weights(ctx):
+    """The 256^2 path honours masks, a per-frame background, frame gains and
+    a zero-weight frame like the oracle (FPIE, 2 sweeps)."""
+    model, meas = tile(256, 5, 61)
+    F = model.frame_count
+    rng = np.random.default_rng(61)
+    bg = rng.uniform(0.0, 3.0, (F,) + tuple(model.image_shape))
+    model.with_background(bg)
+    masks = (rng.random((F,) + tuple(model.image_shape)) > 0.05).astype(np.uint8)
+    w = np.ones(F)
+    w[3] = 0.0
+    w[7] = 0.5
+    meas = MeasurementStack(counts=meas.counts, weights=w, masks=masks)
+    alg = Fpie(iterations=2)
+    s = runtime.Session(ReconstructionProblem(meas, model), alg, precision="f32")
+    trace = runtime.Batch([s]).run(2)[0]
+    ref = oracle.run(model, meas, alg.to_c(), 2)
+    ea, ep = amp_phase_err(s.object(), ref["object"])
+    te = np.abs(trace - ref["trace"]) / np.abs(ref["trace"])
+    say("256 masked", "amp", ea, "phase", ep, "trace", te.max())
+    assert ea <= 1e-5 and ep <= 1e-5 and te.max() <= 1e-5
+
+
+def test_batch_256_tile_identical_alone_and_in_mix(ctx):
+    alg = Mpie(iterations=2, momentum_interval=11, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(128, 7, 71), tile(256, 9, 72), tile(64, 5, 73), tile(256, 7, 74)]
+    mix = sessions_for(tiles, alg)
+    runtime.Batch(mix).run(2)
+    for i in (1, 3):
+        alone = sessions_for(tiles[i:i + 1], alg)
+        runtime.Batch(alone).run(2)
+        assert np.array_equal(mix[i].spectrum(), alone[0].spectrum())
