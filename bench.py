@@ -128,8 +128,8 @@ def run_batched(model, meas, iterations, steps, warmup, tiles):
 
 def serving(requests, rate, iterations, seed=5, max_tiles=256):
     """Poisson arrivals (mean `rate` req/s) of jobs with tile size in
-    {64^2, 128^2}, 1-3 channels from {450, 532, 630} nm (each channel one tile
-    with its own compiled pupil), k x k LEDs with k in {9, 11, 13, 15}; FP32
+    {64^2, 128^2, 256^2}, 1-3 channels from {450, 532, 630} nm (each channel
+    one tile with its own compiled pupil), k x k LEDs with k in 9..15; FP32
     mPIE `iterations` sweeps. Inputs are built before the arrival clock
     starts. Returns latency percentiles and throughput."""
     rng = np.random.default_rng(seed)
@@ -151,8 +151,8 @@ def serving(requests, rate, iterations, seed=5, max_tiles=256):
     t = 0.0
     for _ in range(requests):
         t += rng.exponential(1.0 / rate)
-        n = int(rng.choice([64, 128]))
-        k = int(rng.choice([9, 11, 13, 15]))
+        n = int(rng.choice([64, 128, 256]))
+        k = int(rng.integers(9, 16))
         ch = rng.choice(3, size=int(rng.integers(1, 4)), replace=False)
         reqs.append((t, [problem(n, k, wavelengths[c]) for c in sorted(ch)]))
     alg = Mpie(iterations=iterations, momentum_friction=0.5, momentum_feedback=0.5)
@@ -186,7 +186,62 @@ def serving(requests, rate, iterations, seed=5, max_tiles=256):
     return {"requests": requests, "tiles": tiles, "rate_per_s": rate, "iterations": iterations,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "max_ms": float(lat.max()), "tiles_per_s": tiles / wall, "ticks": stats["ticks"],
-            "sizes": "64^2,128^2", "channels": "1-3 of 450/532/630 nm", "leds": "9x9..15x15"}
+            "sizes": "64^2,128^2,256^2", "channels": "1-3 of 450/532/630 nm", "leds": "9x9..15x15"}
+
+
+def uneven_tiles(seed=11):
+    """96 FOV tiles of uneven size: 32 x 64^2, 48 x 128^2, 16 x 256^2 images
+    (-> 168^2, 336^2, 672^2 'smooth'), 13x13 LEDs, each tile with its own
+    object (seed) and its own Zernike Z4-Z15 pupil with RMS ~ U(0, 0.8] rad
+    (simulated with it and reconstructed with it); uint16 Poisson counts at
+    a 4e3 photon peak."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, count in ((256, 16), (128, 48), (64, 32)):
+        for _ in range(count):
+            model = runtime.compile_model(syn.base_optics(), syn.planar_array(13), (n, n))
+            model.pupil = syn.zernike_pupil(model, float(rng.uniform(1e-3, 0.8)), int(rng.integers(1 << 30)))
+            obj = syn.random_field_object(model.reconstruction_shape, int(rng.integers(1 << 30)))
+            I = runtime.forward_stack(model, syn.object_spectrum(obj))
+            counts, ppp = syn.poisson_counts(I, 4e3, int(rng.integers(1 << 30)))
+            model.with_frame_gains(np.full(model.frame_count, ppp))
+            out.append(ReconstructionProblem(MeasurementStack(counts=counts), model))
+    return out
+
+
+def run_uneven(iterations=40, steps=1, warmup=1):
+    """The uneven 96-tile batch, FP32 mPIE `iterations` sweeps, one fused
+    launch per sweep (graph replay). Returns tiles/s, the sweep-kernel
+    figures and the algorithmic bytes."""
+    log("building the uneven 96-tile batch")
+    probs = uneven_tiles()
+    ctx = runtime.default_context()
+    alg = algorithm(iterations)
+    sessions = [runtime.Session(p, alg, precision="f32", context=ctx) for p in probs]
+    batch = runtime.Batch(sessions, context=ctx)
+    for i in range(warmup):
+        batch.run(iterations)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sweep_ms = []
+    for i in range(steps):
+        batch.run(iterations)
+        for s in sessions:
+            s.object()
+        sweep_ms.append(batch.timing()[0])
+        log(f"uneven batch {i + 1}/{steps} done")
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    total = sum(tile_bytes(p.model, iterations, 30, 2, 16, 8) for p in probs)
+    per_sweep = sum((tile_bytes(p.model, iterations, 30, 2, 16, 8) - p.model.frame_count * p.model.image_len * 2
+                     - 4 * p.model.reconstruction_shape[0] * p.model.reconstruction_shape[1] * 16) / iterations
+                    for p in probs)
+    ms = float(np.mean(sweep_ms))
+    return {"tiles": len(probs), "sizes": "32x64^2, 48x128^2, 16x256^2", "leds": "13x13", "iterations": iterations,
+            "tiles_per_s": steps * len(probs) / secs, "ms_per_batch": 1000.0 * secs / steps,
+            "sweep_ms": ms, "sweep_achieved_GBs": per_sweep / (ms * 1e-3) / 1e9,
+            "sweep_frac": per_sweep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "batch_algorithmic_GB": total / 1e9}
 
 
 def cpu_baseline(model, meas, iterations, sample_iterations, threads):
@@ -223,7 +278,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=50)
-    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving", "uneven"])
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--no-extras", action="store_true", help="batched mode without the plain and serving legs")
@@ -242,6 +297,12 @@ def main():
         print(json.dumps({"metric": "serving p99 latency, FP32 mPIE 30 sweeps", "value": res["p99_ms"],
                           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
                           "config": {"workload": "Poisson arrivals", **res}}), flush=True)
+        return
+    if args.mode == "uneven":
+        res = run_uneven(40, args.steps, args.warmup)
+        print(json.dumps({"metric": "uneven batch tiles/sec, FP32 mPIE 40 sweeps", "value": res["tiles_per_s"],
+                          "unit": "tiles/s", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+                          "config": {"workload": "96 uneven tiles", **res}}), flush=True)
         return
     log("building workload")
     model, meas = workload()
@@ -289,6 +350,8 @@ def main():
         psecs = run_plain(model, meas, args.iterations, 1, 1, "c128")
         line["plain_c128_tiles_per_s"] = 1.0 / psecs
         line["speedup_vs_plain_c128"] = tiles_per_s * psecs
+        log("uneven batch leg")
+        line["uneven_batch"] = run_uneven(40, 1, 1)
         log("serving leg")
         line["serving"] = serving(args.requests, args.rate, 30)
     if not args.no_cpu:

@@ -295,17 +295,37 @@ __device__ __forceinline__ float2 replace_amplitude(const TileDesc& T, uint64_t 
 
 // mPIE momentum event over the tile's whole R^2 spectrum (common.rs:308-387).
 __device__ __forceinline__ void momentum_event(const TileDesc& T, int tid) {
-  const uint64_t M = (uint64_t)T.rh * T.rw;
+  const uint32_t M = T.rh * T.rw;
   double2* __restrict__ spec = T.spec;
+  double2* __restrict__ vel = T.vel;
+  double2* __restrict__ anc = T.anc;
   bool bad = false;
-  for (uint64_t i = tid; i < M; i += kThreads) {
-    const double2 o = spec[i], v = T.vel[i], an = T.anc[i];
-    const double2 nv = make_double2(fma(T.friction, v.x, o.x - an.x), fma(T.friction, v.y, o.y - an.y));
-    const double2 no = T.feedback == 0.0 ? o : make_double2(fma(T.feedback, nv.x, o.x), fma(T.feedback, nv.y, o.y));
-    bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
-    T.vel[i] = nv;
-    spec[i] = no;
-    T.anc[i] = no;
+  constexpr int U = 4;  // loads of U elements in flight before their use
+  for (uint32_t base = tid; base < M; base += U * kThreads) {
+    double2 o[U], v[U], an[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = base + u * kThreads;
+      if (i < M) {
+        o[u] = spec[i];
+        v[u] = vel[i];
+        an[u] = anc[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = base + u * kThreads;
+      if (i < M) {
+        const double2 nv =
+            make_double2(fma(T.friction, v[u].x, o[u].x - an[u].x), fma(T.friction, v[u].y, o[u].y - an[u].y));
+        const double2 no =
+            T.feedback == 0.0 ? o[u] : make_double2(fma(T.feedback, nv.x, o[u].x), fma(T.feedback, nv.y, o[u].y));
+        bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
+        vel[i] = nv;
+        spec[i] = no;
+        anc[i] = no;
+      }
+    }
   }
   if (bad) T.status[0] = 1;
   __syncthreads();
@@ -318,6 +338,120 @@ __device__ __forceinline__ void finish_sweep(const TileDesc& T, double obj_sum, 
     if (it < T.trace_len) T.trace[it] = w_sum > 0.0 ? obj_sum / w_sum : __builtin_nan("");
     *T.iteration = it + 1;
     *T.counter = counter;
+  }
+}
+
+// Patch x pupil for STRIP vertically consecutive pixels of one patch column
+// (bilinear extract, crop.rs:129-167), handed to put(j, value). `off` is the
+// window offset of the first pixel's lower-left tap. Where the pupil is zero
+// the product is exactly zero, so the spectrum taps are not read at all;
+// vertically adjacent pixels share one row of taps. No read leaves the
+// (n+1)^2 window.
+template <int STRIP, class Put>
+__device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
+                                             int pstride, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du,
+                                             double wll, double wlu, double wul, double wuu, Put put) {
+  double2 a = make_double2(0.0, 0.0), b = a;
+  bool have = false;
+#pragma unroll
+  for (int j = 0; j < STRIP; ++j) {
+    const float2 P = pcol[j * pstride];
+    float2 out = f2(0.0f, 0.0f);
+    if (P.x != 0.0f || P.y != 0.0f) {
+      if (!have) {
+        a = spec[off];
+        b = spec[off + dc];
+      }
+      const double2 c2 = du ? spec[off + du] : a, d = du ? spec[off + du + dc] : b;
+      const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
+                                     fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
+      out = fmul(f2((float)v.x, (float)v.y), P);
+      a = c2;
+      b = d;
+      have = du != 0;
+    } else {
+      have = false;
+    }
+    put(j, out);
+    off += rw;
+  }
+}
+
+// Integer-offset crop: the patch is a plain copy of the window (crop.rs:79-102).
+template <int STRIP, class Put>
+__device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
+                                           int pstride, uint32_t off, uint32_t rw, Put put) {
+#pragma unroll
+  for (int j = 0; j < STRIP; ++j) {
+    const float2 P = pcol[j * pstride];
+    float2 out = f2(0.0f, 0.0f);
+    if (P.x != 0.0f || P.y != 0.0f) {
+      const double2 v = spec[off];
+      out = fmul(f2((float)v.x, (float)v.y), P);
+    }
+    put(j, out);
+    off += rw;
+  }
+}
+
+// Adjoint insert of one patch update U (crop.rs:188-256) as a gather over
+// destination rows [a0, a1) of the (n+1)^2 window: each destination pixel
+// sums its <= 4 weighted taps of U (fetched by u(row, col) in centred patch
+// coordinates) and is read-modified-written in FP64 only when that sum is
+// non-zero.
+template <int N, int THREADS, class GetU>
+__device__ __forceinline__ void insert_window(double2* __restrict__ spec, const SrcParam& sp, uint32_t rw,
+                                              double scale, int tid, int a0, int a1, GetU u) {
+  const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
+  const int ww = N + (fc ? 1 : 0);
+  const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
+              wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
+  const int total = (a1 - a0) * ww;
+  constexpr int UNROLL = 4;
+  for (int base = tid; base < total; base += UNROLL * THREADS) {
+    float2 add[UNROLL];
+    uint32_t off[UNROLL];
+    bool hit[UNROLL];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+      const int qd = base + k * THREADS;
+      const int a = a0 + qd / ww, b = qd % ww;
+      off[k] = (sp.ar.lower + a) * rw + sp.ac.lower + b;
+      float ax = 0.0f, ay = 0.0f;
+      auto tap = [&](int sr_, int sc_, float wt) {
+        if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
+          const float2 v = u(sr_, sc_);
+          ax = fmaf(v.x, wt, ax);
+          ay = fmaf(v.y, wt, ay);
+        }
+      };
+      if (qd < total) {
+        if (fr && fc) {
+          tap(a - 1, b - 1, wuu);
+          tap(a - 1, b, wul);
+          tap(a, b - 1, wlu);
+          tap(a, b, wll);
+        } else if (fr) {
+          tap(a - 1, b, wul);
+          tap(a, b, wll);
+        } else if (fc) {
+          tap(a, b - 1, wlu);
+          tap(a, b, wll);
+        } else {
+          tap(a, b, wll);
+        }
+      }
+      add[k] = f2(ax, ay);
+      hit[k] = qd < total && (ax != 0.0f || ay != 0.0f);
+    }
+    double2 old[UNROLL];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k)
+      if (hit[k]) old[k] = spec[off[k]];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k)
+      if (hit[k])
+        spec[off[k]] = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
   }
 }
 
@@ -357,42 +491,18 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
     if (tid < COLS_THREADS) {
       const int c = tid % N, r0 = (tid / N) * STRIP;
       const int c1 = (c + HALF) % N;
-      if (sp.zero) {
 #pragma unroll
-        for (int j = 0; j < STRIP; ++j) {
-          const int r = r0 + j;
-          const double2 v0 = spec[((uint32_t)sp.sr + r) * T.rw + (uint32_t)sp.sc + c];
-          const double2 v1 = spec[((uint32_t)sp.sr + r + HALF) * T.rw + (uint32_t)sp.sc + c1];
-          lds[(r + HALF) * LD + c1] = fmul(f2((float)v0.x, (float)v0.y), pupil[r * N + c]);
-          lds[r * LD + c] = fmul(f2((float)v1.x, (float)v1.y), pupil[(r + HALF) * N + c1]);
-        }
-      } else {
-        const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
-                     wuu = sp.ar.wu * sp.ac.wu;
-        const uint32_t dc = sp.ac.upper - sp.ac.lower, du = (sp.ar.upper - sp.ar.lower) * T.rw;
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
-          // rows of the (n+1)^2 stencil are read once; no read beyond the window
-          uint32_t off = (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc;
-          double2 a = spec[off], b = spec[off + dc];
-#pragma unroll
-          for (int j = 0; j < STRIP; ++j) {
-            const double2 c2 = du ? spec[off + du] : a, d = du ? spec[off + du + dc] : b;
-            const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
-                                           fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
-            const int r = rb + j;
-            const int rr = (r + HALF) % N, cs = (cc + HALF) % N;
-            lds[rr * LD + cs] = fmul(f2((float)v.x, (float)v.y), pupil[r * N + cc]);
-            off += T.rw;
-            if (du) {
-              a = c2;
-              b = d;
-            } else if (j + 1 < STRIP) {
-              a = spec[off];
-              b = spec[off + dc];
-            }
-          }
+      for (int half = 0; half < 2; ++half) {
+        const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
+        const int cs = (cc + HALF) % N;
+        auto put = [&](int j, float2 v) { lds[((rb + j + HALF) % N) * LD + cs] = v; };
+        if (sp.zero) {
+          copy_strip<STRIP>(spec, pupil + rb * N + cc, N, ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw,
+                            put);
+        } else {
+          gather_strip<STRIP>(spec, pupil + rb * N + cc, N, (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw,
+                              sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
+                              sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
         }
       }
     }
@@ -457,65 +567,21 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       }
     }
     __syncthreads();
-    // 6. adjoint insert: gather over the destination window, FP64 RMW
+    // 6. adjoint insert: gather over the destination window, FP64 RMW of the
+    //    pixels that receive a non-zero update (U is zero outside the pupil)
     const double scale = fw * T.object_step;
     if (sp.zero) {
       for (int p = tid; p < NN; p += kThreads) {
         const int r = p / N, c = p % N;
-        double2& d = spec[(sp.sr + r) * T.rw + sp.sc + c];
         const float2 u = lds[r * LD + c];
-        d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
+        if (u.x != 0.0f || u.y != 0.0f) {
+          double2& d = spec[(sp.sr + r) * T.rw + sp.sc + c];
+          d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
+        }
       }
     } else {
-      const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
-      const int wh = N + (fr ? 1 : 0), ww = N + (fc ? 1 : 0);
-      const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
-                  wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
-      const int total = wh * ww;
-      constexpr int UNROLL = 4;
-      for (int base = tid; base < total; base += UNROLL * kThreads) {
-        double2 old[UNROLL];
-        float2 add[UNROLL];
-        uint32_t off[UNROLL];
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-          const int qd = base + k * kThreads;
-          if (qd < total) {
-            const int a = qd / ww, b = qd % ww;
-            off[k] = (sp.ar.lower + a) * T.rw + sp.ac.lower + b;
-            old[k] = spec[off[k]];
-            float ax = 0.0f, ay = 0.0f;
-            auto tap = [&](int sr_, int sc_, float wt) {
-              if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
-                const float2 u = lds[sr_ * LD + sc_];
-                ax = fmaf(u.x, wt, ax);
-                ay = fmaf(u.y, wt, ay);
-              }
-            };
-            if (fr && fc) {
-              tap(a - 1, b - 1, wuu);
-              tap(a - 1, b, wul);
-              tap(a, b - 1, wlu);
-              tap(a, b, wll);
-            } else if (fr) {
-              tap(a - 1, b, wul);
-              tap(a, b, wll);
-            } else if (fc) {
-              tap(a, b - 1, wlu);
-              tap(a, b, wll);
-            } else {
-              tap(a, b, wll);
-            }
-            add[k] = f2(ax, ay);
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-          const int qd = base + k * kThreads;
-          if (qd < total)
-            spec[off[k]] = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
-        }
-      }
+      insert_window<N, kThreads>(spec, sp, T.rw, scale, tid, 0, N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0),
+                                 [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
     }
     __syncthreads();
     // 7. mPIE momentum over the whole tile spectrum (common.rs:308-387)
@@ -561,6 +627,43 @@ __host__ __device__ constexpr uint32_t sweep_field_elems(uint32_t n) {
   return n <= kFusedLdsMaxN ? n * (n + 1) : (uint32_t)big_field_elems<256>();
 }
 
+// Copies between a block of a row-major global array (rows of W elements at
+// a stride of `gstride`) and LDS (rows at a pitch of LD): every load of the
+// thread is issued before the first store.
+template <int TOTAL, int W, int LD>
+__device__ __forceinline__ void copy_in(float2* lds, const float2* __restrict__ g, int gstride, int tid) {
+  constexpr int PER = TOTAL / kThreads;
+  static_assert(TOTAL % kThreads == 0, "copy shape");
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = g[(i / W) * gstride + i % W];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    lds[(i / W) * LD + i % W] = v[j];
+  }
+}
+
+template <int TOTAL, int W, int LD>
+__device__ __forceinline__ void copy_out(const float2* lds, float2* __restrict__ g, int gstride, int tid) {
+  constexpr int PER = TOTAL / kThreads;
+  static_assert(TOTAL % kThreads == 0, "copy shape");
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = lds[(i / W) * LD + i % W];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    g[(i / W) * gstride + i % W] = v[j];
+  }
+}
+
 template <int N>
 __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, float2* prev, const float2* twf, const float2* twi,
                                double* red) {
@@ -600,40 +703,20 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         const int c = tid % N, s0 = (tid / N) * STRIP;
         const int rc = ((k * CH) ^ HALF) + s0;  // first centred row of the strip
         const int cs = c ^ HALF;
+        auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
         if (sp.zero) {
-#pragma unroll 4
-          for (int j = 0; j < STRIP; ++j) {
-            const int r = rc + j;
-            const double2 v = spec[((uint32_t)sp.sr + r) * T.rw + (uint32_t)sp.sc + c];
-            lds[(s0 + j) * LDR + cs] = fmul(f2((float)v.x, (float)v.y), pupil[r * N + c]);
-          }
+          copy_strip<STRIP>(spec, pupil + rc * N + c, N, ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw,
+                            put);
         } else {
-          const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
-                       wuu = sp.ar.wu * sp.ac.wu;
-          const uint32_t dc = sp.ac.upper - sp.ac.lower, du = (sp.ar.upper - sp.ar.lower) * T.rw;
-          uint32_t off = (sp.ar.lower + rc) * T.rw + sp.ac.lower + c;
-          double2 a = spec[off], b = spec[off + dc];
-#pragma unroll 4
-          for (int j = 0; j < STRIP; ++j) {
-            const double2 c2 = du ? spec[off + du] : a, d = du ? spec[off + du + dc] : b;
-            const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
-                                           fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
-            lds[(s0 + j) * LDR + cs] = fmul(f2((float)v.x, (float)v.y), pupil[(rc + j) * N + c]);
-            off += T.rw;
-            if (du) {
-              a = c2;
-              b = d;
-            } else if (j + 1 < STRIP) {
-              a = spec[off];
-              b = spec[off + dc];
-            }
-          }
+          gather_strip<STRIP>(spec, pupil + rc * N + c, N, (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
+                              sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
+                              sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
         }
       }
       __syncthreads();
       stage<N, CH, P0, 1, LDR, 1, true>(lds, twi, tid);
       stage<N, CH, P1, P0, LDR, 1, true>(lds, twi, tid);
-      for (int i = tid; i < CH * N; i += kThreads) fld[(k * CH + i / N) * N + i % N] = lds[(i / N) * LDR + i % N];
+      copy_out<CH * N, N, LDR>(lds, fld + k * CH * N, N, opaque_tid());
       __syncthreads();
     }
     // B. columns: natural image order after the inverse column transforms
@@ -643,7 +726,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
     for (int jc = 0; jc < NCH; ++jc) {
       tid = opaque_tid();
       const int c0 = jc * CH;
-      for (int i = tid; i < N * CH; i += kThreads) lds[(i / CH) * LDC + i % CH] = fld[(i / CH) * N + c0 + i % CH];
+      copy_in<N * CH, CH, LDC>(lds, fld + c0, N, opaque_tid());
       __syncthreads();
       stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
       stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
@@ -651,6 +734,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         // measurement loads in groups of G issued before their use
         constexpr int PP = N * CH / kThreads;
         constexpr int G = 8;
+        const int tid = opaque_tid();
         float lsum_f = 0.0f;
 #pragma unroll
         for (int g = 0; g < PP; g += G) {
@@ -673,7 +757,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       __syncthreads();
       stage<N, CH, P0, 1, 1, LDC, false>(lds, twf, tid);
       stage<N, CH, P1, P0, 1, LDC, false>(lds, twf, tid);
-      for (int i = tid; i < N * CH; i += kThreads) fld[(i / CH) * N + c0 + i % CH] = lds[(i / CH) * LDC + i % CH];
+      copy_out<N * CH, CH, LDC>(lds, fld + c0, N, opaque_tid());
       __syncthreads();
     }
     {
@@ -689,15 +773,10 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
     }
     // C. rows in centred order: centred row q*CH + lr is natural row (q*CH + lr) ^ HALF
     const double scale = fw * T.object_step;
-    const bool fr = !sp.zero && sp.ar.upper != sp.ar.lower, fc = !sp.zero && sp.ac.upper != sp.ac.lower;
-    const int ww = N + (fc ? 1 : 0);
-    const float wll = sp.zero ? 1.0f : (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
-                wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
-    const uint32_t dr0 = sp.zero ? (uint32_t)sp.sr : sp.ar.lower, dc0 = sp.zero ? (uint32_t)sp.sc : sp.ac.lower;
     for (int q = 0; q < NCH; ++q) {
       tid = opaque_tid();
       const int nr0 = (q * CH) ^ HALF;
-      for (int i = tid; i < CH * N; i += kThreads) lds[(i / N) * LDR + i % N] = fld[(nr0 + i / N) * N + i % N];
+      copy_in<CH * N, N, LDR>(lds, fld + nr0 * N, N, opaque_tid());
       __syncthreads();
       stage<N, CH, P0, 1, LDR, 1, false>(lds, twf, tid);
       stage<N, CH, P1, P0, LDR, 1, false>(lds, twf, tid);
@@ -709,52 +788,23 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         lds[lr * LDR + kc] = f2(u.x * kk, u.y * kk);
       }
       __syncthreads();
-      // destination rows a of this chunk (and row N after the last chunk)
-      const int a0 = q * CH;
-      const int rows = CH + ((q == NCH - 1 && fr) ? 1 : 0);
-      const int total = rows * ww;
-      constexpr int UNROLL = 4;
-      for (int base = tid; base < total; base += UNROLL * kThreads) {
-        double2 old[UNROLL];
-        float2 add[UNROLL];
-        uint32_t off[UNROLL];
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-          const int qd = base + k * kThreads;
-          if (qd < total) {
-            const int a = a0 + qd / ww, b = qd % ww;
-            off[k] = (dr0 + a) * T.rw + dc0 + b;
-            old[k] = spec[off[k]];
-            float ax = 0.0f, ay = 0.0f;
-            auto tap = [&](int sr_, int sc_, float wt) {
-              if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
-                const float2 u = sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)] : prev[sc_ ^ HALF];
-                ax = fmaf(u.x, wt, ax);
-                ay = fmaf(u.y, wt, ay);
-              }
-            };
-            if (fr && fc) {
-              tap(a - 1, b - 1, wuu);
-              tap(a - 1, b, wul);
-              tap(a, b - 1, wlu);
-              tap(a, b, wll);
-            } else if (fr) {
-              tap(a - 1, b, wul);
-              tap(a, b, wll);
-            } else if (fc) {
-              tap(a, b - 1, wlu);
-              tap(a, b, wll);
-            } else {
-              tap(a, b, wll);
+      // destination rows of this chunk (and row N after the last chunk)
+      {
+        const int a0 = q * CH;
+        if (sp.zero) {
+          for (int i = opaque_tid(); i < CH * N; i += kThreads) {
+            const int lr = i / N, c = i % N;
+            const float2 u = lds[lr * LDR + (c ^ HALF)];
+            if (u.x != 0.0f || u.y != 0.0f) {
+              double2& d = spec[(sp.sr + a0 + lr) * T.rw + sp.sc + c];
+              d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
             }
-            add[k] = f2(ax, ay);
           }
-        }
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-          const int qd = base + k * kThreads;
-          if (qd < total)
-            spec[off[k]] = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
+        } else {
+          const int a1 = a0 + CH + ((q == NCH - 1 && sp.ar.upper != sp.ar.lower) ? 1 : 0);
+          insert_window<N, kThreads>(spec, sp, T.rw, scale, opaque_tid(), a0, a1, [&](int sr_, int sc_) {
+            return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)] : prev[sc_ ^ HALF];
+          });
         }
       }
       __syncthreads();

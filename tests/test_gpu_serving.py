@@ -32,16 +32,16 @@ ALG = Mpie(iterations=4, momentum_interval=30, momentum_friction=0.5, momentum_f
 
 
 def test_result_is_identical_alone_and_in_the_mix(ctx):
-    jobs = [job(64, 9, 1, 450e-9), job(128, 7, 2), job(64, 11, 3, 630e-9), job(32, 5, 4)]
+    jobs = [job(64, 9, 1, 450e-9), job(128, 7, 2), job(256, 9, 6, 630e-9), job(64, 11, 3, 630e-9), job(32, 5, 4)]
     say("serving mix", [p.model.reconstruction_shape for p in jobs])
     alone = []
     for p in jobs:
-        srv = runtime.Server(max_tiles=4)
+        srv = runtime.Server(max_tiles=5)
         jid = srv.submit(p, ALG)
         srv.wait(jid, 120)
         alone.append(srv.result(jid))
         srv.close()
-    srv = runtime.Server(max_tiles=4)
+    srv = runtime.Server(max_tiles=5)
     ids = []
     for i, p in enumerate(jobs):
         ids.append(srv.submit(p, ALG))
@@ -52,14 +52,15 @@ def test_result_is_identical_alone_and_in_the_mix(ctx):
     st = srv.stats()
     srv.close()
     say("stats", st)
-    assert st["completed"] == 4 and st["failed"] == 0
+    assert st["completed"] == 5 and st["failed"] == 0
     for a, b in zip(alone, mixed):
         assert np.array_equal(a[1], b[1])
         assert np.array_equal(a[2], b[2])
 
 
-def test_served_job_matches_fp64_oracle(ctx):
-    p = job(64, 9, 5)
+@pytest.mark.parametrize("n", [64, 256])
+def test_served_job_matches_fp64_oracle(ctx, n):
+    p = job(n, 9, 5)
     srv = runtime.Server(max_tiles=2)
     jid = srv.submit(p, ALG)
     srv.wait(jid, 120)
