@@ -7,6 +7,7 @@
 #include <limits>
 #include <numeric>
 
+#include "fused.h"
 #include "session.h"
 
 using namespace fpmhip;
@@ -647,6 +648,10 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     b.work_m.allocate(std::max(M, N) * sizeof(double2));
     b.avg.allocate(N * sizeof(double));
     b.aw.allocate(N * sizeof(double));
+    if (precision == FPMHIP_MIXED_F32 && m->nh == m->nw && m->nh > kFusedLdsMaxN && m->nh <= kFusedMaxN) {
+      b.group.allocate(group_state_bytes(m->nh));
+      HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
+    }
     launch_convert_pupil(m->d_pupil64.as<double2>(), b.pupil.ptr, precision == FPMHIP_MIXED_F32, N, m->ctx->stream);
     launch_flag_reset(*s);
     HIP_CHECK(hipStreamSynchronize(m->ctx->stream));

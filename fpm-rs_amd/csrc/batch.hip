@@ -26,9 +26,14 @@ int guarded(F&& body) {
 
 __global__ void k_reset_tiles(TileDesc* tiles, uint32_t count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) {
+  if (i < count && tiles[i].member == 0) {
     *tiles[i].iteration = 0;
     tiles[i].status[0] = 0;
+    if (tiles[i].group) {
+      tiles[i].group[0] = 0;
+      tiles[i].group[1] = 0;
+      tiles[i].group[2] = 0;
+    }
   }
 }
 
@@ -50,6 +55,11 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.anc = s.b.anc.as<double2>();
   t.pupil = s.b.pupil.as<float2>();
   t.field = m.nh > kFusedLdsMaxN ? s.b.field.as<float2>() : nullptr;  // the session's N-pixel field buffer
+  t.group = s.b.group.as<uint64_t>();
+  t.partial = s.b.group.ptr ? reinterpret_cast<double2*>(s.b.group.as<char>() + 64) : nullptr;
+  t.edge = s.b.group.ptr ? reinterpret_cast<float2*>(s.b.group.as<char>() + 64 + 16 * kGroupMembers) : nullptr;
+  t.member = 0;
+  t.members = 1;
   t.frames = me.d_frames.ptr;
   t.masks = me.d_masks.as<uint8_t>();
   t.weights = me.d_weights.as<double>();
@@ -64,6 +74,7 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.iteration = reinterpret_cast<uint32_t*>(per16 + 8);
   t.status = reinterpret_cast<int32_t*>(per16 + 12);
   t.trace_len = trace_len;
+  t.sweeps = 1;
   t.n = m.nh;
   t.rh = m.rh;
   t.rw = m.rw;
@@ -84,8 +95,34 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.feedback = s.cfg.feedback;
 }
 
-void order_tiles(std::vector<TileDesc>& tiles) {
-  std::stable_sort(tiles.begin(), tiles.end(), [](const TileDesc& a, const TileDesc& b) { return a.n > b.n; });
+std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,
+                                  bool largest_first) {
+  std::vector<TileDesc> sorted(tiles);
+  if (largest_first)
+    std::stable_sort(sorted.begin(), sorted.end(), [](const TileDesc& a, const TileDesc& b) { return a.n > b.n; });
+  std::vector<TileDesc> out;
+  out.reserve(sorted.size() * members);
+  for (TileDesc t : sorted) {
+    if (radix) t.plan = radix[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3];
+    const uint32_t g = (t.n > kFusedLdsMaxN && t.group) ? members : 1;
+    for (uint32_t m = 0; m < g; ++m) {
+      out.push_back(t);
+      out.back().member = m;
+      out.back().members = g;
+    }
+  }
+  return out;
+}
+
+uint32_t group_members_for(const std::vector<TileDesc>& tiles, uint32_t nmax, uint32_t requested) {
+  if (requested <= 1) return 1;
+  uint64_t big = 0;
+  for (const TileDesc& t : tiles)
+    if (t.n > kFusedLdsMaxN) ++big;
+  if (big == 0) return 1;
+  // cooperative groups need the whole grid resident at once
+  const uint64_t grid = tiles.size() + big * (requested - 1);
+  return grid <= sweep_coresident_limit(nmax) ? requested : 1;
 }
 
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream) {
@@ -136,8 +173,11 @@ struct fpmhip_batch {
   Context* ctx = nullptr;
   std::vector<fpmhip_session*> sessions;
   std::vector<TileDesc> host_tiles;    // in session order
-  std::vector<TileDesc> device_order;  // launch order (largest tiles first)
-  DeviceBuffer tiles;      // TileDesc[count]
+  std::vector<TileDesc> device_order;  // launch order (largest tiles first, groups expanded)
+  fpmhip_batch_schedule schedule{kGroupMembers, 1, {0, 0, 0, 0}};
+  uint32_t launched = 0;               // workgroups per sweep launch
+  bool cooperative = false;
+  DeviceBuffer tiles;      // TileDesc[launched]
   DeviceBuffer tw;         // float2[kTwTotal]
   DeviceBuffer per_tile;   // counters, iterations, status
   DeviceBuffer traces;     // double[count * trace_cap]
@@ -168,10 +208,14 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
   for (uint32_t i = 0; i < count; ++i)
     fill_tile_desc(b.host_tiles[i], *b.sessions[i], b.traces.as<double>() + (size_t)i * trace_cap,
                    b.losses.as<double>() + (size_t)i * b.fmax, b.per_tile.as<char>() + (size_t)i * 16, trace_cap);
-  b.device_order = b.host_tiles;
-  order_tiles(b.device_order);
-  b.tiles.allocate(count * sizeof(TileDesc));
-  HIP_CHECK(hipMemcpyAsync(b.tiles.ptr, b.device_order.data(), count * sizeof(TileDesc), hipMemcpyHostToDevice,
+  prepare_sweep(b.nmax);  // the LDS attribute first: the residency query depends on it
+  b.device_order = order_tiles(b.host_tiles, group_members_for(b.host_tiles, b.nmax, b.schedule.group_members),
+                               b.schedule.radix, b.schedule.largest_first != 0);
+  b.launched = (uint32_t)b.device_order.size();
+  b.cooperative = false;
+  for (const TileDesc& t : b.device_order) b.cooperative |= t.members > 1;
+  b.tiles.allocate(b.launched * sizeof(TileDesc));
+  HIP_CHECK(hipMemcpyAsync(b.tiles.ptr, b.device_order.data(), b.launched * sizeof(TileDesc), hipMemcpyHostToDevice,
                            b.ctx->stream));
   // capture the sweep launch once
   if (b.exec) {
@@ -183,12 +227,68 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
     b.graph = nullptr;
   }
   HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
-  // set the LDS attribute outside capture, then capture the launch
-  prepare_sweep(b.nmax);
+  // capture the launch (the LDS attribute was set above, outside capture)
   HIP_CHECK(hipStreamBeginCapture(b.ctx->stream, hipStreamCaptureModeThreadLocal));
-  launch_sweep(b.tiles.as<TileDesc>(), count, b.tw.as<float2>(), b.nmax, b.ctx->stream);
+  launch_sweep(b.tiles.as<TileDesc>(), b.launched, b.tw.as<float2>(), b.nmax, b.cooperative, b.ctx->stream);
   HIP_CHECK(hipStreamEndCapture(b.ctx->stream, &b.graph));
   HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+}
+
+// ReconstructionState::initialize (+ mPIE auxiliary state) of every tile and
+// a reset of the per-tile sweep state.
+void initialize_tiles(fpmhip_batch& b) {
+  hipStream_t st = b.ctx->stream;
+  for (fpmhip_session* s : b.sessions) {
+    if (int rc = fpmhip_session_initialize(s)) throw Failure{rc};
+  }
+  for (size_t i = 0; i < b.sessions.size(); ++i)
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, b.host_tiles[i].counter, b.sessions[i]->counter);
+  hipLaunchKernelGGL(k_reset_tiles, dim3((b.launched + 255) / 256), dim3(256), 0, st, b.tiles.as<TileDesc>(),
+                     b.launched);
+  HIP_CHECK(hipGetLastError());
+}
+
+// `sweeps` graph replays, each bracketed by HIP events on the batch stream.
+void replay(fpmhip_batch& b, uint64_t sweeps, double* mean, double* mx) {
+  hipStream_t st = b.ctx->stream;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  double total = 0.0, worst = 0.0;
+  for (uint64_t it = 0; it < sweeps; ++it) {
+    HIP_CHECK(hipEventRecord(e0, st));
+    HIP_CHECK(hipGraphLaunch(b.exec, st));
+    HIP_CHECK(hipEventRecord(e1, st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    total += ms;
+    worst = std::max(worst, (double)ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *mean = sweeps ? total / (double)sweeps : 0.0;
+  *mx = worst;
+}
+
+void check_tile_status(int32_t status) {
+  if (status == 2) fail(FPMHIP_DEVICE_ERROR, "a cooperative tile group stalled at a group barrier");
+  if (status) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
+}
+
+void validate_schedule(const fpmhip_batch_schedule& s) {
+  if (s.group_members != 1 && s.group_members != kGroupMembers)
+    fail(FPMHIP_INVALID_PARAMETER, "group_members must be 1 or 4", "group_members");
+  if (s.largest_first > 1) fail(FPMHIP_INVALID_PARAMETER, "largest_first must be 0 or 1", "largest_first");
+  const uint32_t sizes[4] = {32, 64, 128, 256};
+  for (int k = 0; k < 4; ++k) {
+    if (s.radix[k] == 0) continue;
+    uint32_t choices[3];
+    const uint32_t nc = sweep_radix_choices(sizes[k], choices);
+    bool ok = false;
+    for (uint32_t i = 0; i < nc; ++i) ok |= choices[i] == s.radix[k];
+    if (!ok) fail(FPMHIP_INVALID_PARAMETER, "radix is not a plan of the fused sweep for that tile size", "radix");
+  }
 }
 
 }  // namespace
@@ -240,33 +340,9 @@ int fpmhip_batch_run(fpmhip_batch* b, uint64_t iterations, double* trace) {
     hipStream_t st = b->ctx->stream;
     const uint32_t count = (uint32_t)b->sessions.size();
     if (iterations > b->trace_cap) build_tiles(*b, (uint32_t)iterations);
-    // initialise every tile (ReconstructionState::initialize + mPIE auxiliary)
-    for (fpmhip_session* s : b->sessions) {
-      if (int rc = fpmhip_session_initialize(s)) throw Failure{rc};
-    }
-    for (uint32_t i = 0; i < count; ++i)
-      hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, b->host_tiles[i].counter, b->sessions[i]->counter);
-    hipLaunchKernelGGL(k_reset_tiles, dim3((count + 255) / 256), dim3(256), 0, st, b->tiles.as<TileDesc>(), count);
-    HIP_CHECK(hipGetLastError());
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    double total = 0.0, mx = 0.0;
-    for (uint64_t it = 0; it < iterations; ++it) {
-      HIP_CHECK(hipEventRecord(e0, st));
-      HIP_CHECK(hipGraphLaunch(b->exec, st));
-      HIP_CHECK(hipEventRecord(e1, st));
-      HIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      total += ms;
-      mx = std::max(mx, (double)ms);
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    initialize_tiles(*b);
+    replay(*b, iterations, &b->ms_mean, &b->ms_max);
     b->sweeps = iterations;
-    b->ms_mean = iterations ? total / (double)iterations : 0.0;
-    b->ms_max = mx;
     std::vector<char> per(count * 16);
     HIP_CHECK(hipMemcpyAsync(per.data(), b->per_tile.ptr, per.size(), hipMemcpyDeviceToHost, st));
     std::vector<double> tr((size_t)count * b->trace_cap);
@@ -274,11 +350,99 @@ int fpmhip_batch_run(fpmhip_batch* b, uint64_t iterations, double* trace) {
     HIP_CHECK(hipStreamSynchronize(st));
     for (uint32_t i = 0; i < count; ++i) {
       const char* base = per.data() + (size_t)i * 16;
-      const int32_t status = *reinterpret_cast<const int32_t*>(base + 12);
-      if (status) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
+      check_tile_status(*reinterpret_cast<const int32_t*>(base + 12));
       b->sessions[i]->counter = *reinterpret_cast<const uint64_t*>(base);
       if (trace)
         for (uint64_t it = 0; it < iterations; ++it) trace[i * iterations + it] = tr[(size_t)i * b->trace_cap + it];
+    }
+  });
+}
+
+int fpmhip_batch_get_schedule(const fpmhip_batch* b, fpmhip_batch_schedule* out) {
+  return guarded([&] {
+    if (!b || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
+    *out = b->schedule;
+  });
+}
+
+int fpmhip_batch_set_schedule(fpmhip_batch* b, const fpmhip_batch_schedule* schedule) {
+  return guarded([&] {
+    if (!b || !schedule) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
+    validate_schedule(*schedule);
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    b->schedule = *schedule;
+    build_tiles(*b, b->trace_cap);
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int fpmhip_batch_tune(fpmhip_batch* b, uint32_t trial_sweeps, fpmhip_batch_tuning* report) {
+  return guarded([&] {
+    if (!b) fail(FPMHIP_INVALID_PARAMETER, "null batch", "batch");
+    if (trial_sweeps == 0 || trial_sweeps > 1000)
+      fail(FPMHIP_INVALID_PARAMETER, "trial_sweeps must be in 1..1000", "trial_sweeps");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    uint32_t timed = 0;
+    // one candidate: rebuild the launch, start every tile from its initial
+    // state, one untimed sweep, then the mean of `trial_sweeps` sweeps
+    auto measure = [&](const fpmhip_batch_schedule& cand) {
+      b->schedule = cand;
+      build_tiles(*b, b->trace_cap);
+      initialize_tiles(*b);
+      double mean = 0.0, mx = 0.0;
+      replay(*b, 1, &mean, &mx);
+      replay(*b, trial_sweeps, &mean, &mx);
+      ++timed;
+      return mean;
+    };
+    fpmhip_batch_schedule best = b->schedule;
+    const double first = measure(best);
+    double best_ms = first;
+    auto consider = [&](fpmhip_batch_schedule cand) {
+      const double ms = measure(cand);
+      if (ms < 0.98 * best_ms) {  // keep a change only for a clear gain
+        best = cand;
+        best_ms = ms;
+      }
+    };
+    bool big = false, present[4] = {false, false, false, false};
+    for (const TileDesc& t : b->host_tiles) {
+      big |= t.n > kFusedLdsMaxN;
+      present[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3] = true;
+    }
+    // coordinate descent: group size, radix per tile size, launch order
+    if (big)
+      for (uint32_t g : {1u, kGroupMembers})
+        if (g != best.group_members) {
+          fpmhip_batch_schedule c = best;
+          c.group_members = g;
+          consider(c);
+        }
+    const uint32_t sizes[4] = {32, 64, 128, 256};
+    for (int k = 0; k < 4; ++k) {
+      if (!present[k]) continue;
+      uint32_t choices[3];
+      const uint32_t nc = sweep_radix_choices(sizes[k], choices);
+      const uint32_t cur = best.radix[k] ? best.radix[k] : choices[0];
+      for (uint32_t i = 0; i < nc; ++i)
+        if (choices[i] != cur) {
+          fpmhip_batch_schedule c = best;
+          c.radix[k] = choices[i];
+          consider(c);
+        }
+    }
+    {
+      fpmhip_batch_schedule c = best;
+      c.largest_first = best.largest_first ? 0 : 1;
+      consider(c);
+    }
+    b->schedule = best;
+    build_tiles(*b, b->trace_cap);
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    if (report) {
+      report->candidates = timed;
+      report->first_ms = first;
+      report->best_ms = best_ms;
     }
   });
 }

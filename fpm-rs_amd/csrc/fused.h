@@ -26,8 +26,16 @@ struct TileDesc {
   double* frame_loss;     // [F] objective of each order position, last sweep
   uint64_t* counter;      // mPIE effective-frame counter (persists across sweeps)
   uint32_t* iteration;    // sweeps completed
-  int32_t* status;        // [0] non-finite momentum
+  int32_t* status;        // [0] 1: non-finite momentum, 2: cooperative group stalled
+  // cooperative tile group (tiles above kFusedLdsMaxN): `members` workgroups
+  // share one tile; this descriptor is workgroup `member` of the group
+  uint64_t* group;        // [0] barrier arrivals, [1] barriers completed before this launch, [2] stall flag
+  double2* partial;       // [members] per-member objective sums (loss, count)
+  float2* edge;           // [N / 64][N] last row of U of each 64-row chunk
+  uint32_t member, members;
+  uint32_t plan;          // first-stage radix of the LDS line transforms (0: default)
   uint32_t trace_len;
+  uint32_t sweeps;        // sweeps this launch runs (>= 1)
   uint32_t n, rh, rw, F;
   uint32_t interval;
   int32_t is_u16, denom, momentum, loss;
@@ -51,14 +59,35 @@ size_t sweep_lds_bytes(uint32_t nmax);
 void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame_loss, char* per16,
                     uint32_t trace_len);
 // Launch order of a sweep's tiles: largest first (stable), so the longest
-// workgroups are dispatched first and spread over the XCDs.
-void order_tiles(std::vector<TileDesc>& tiles);
+// workgroups are dispatched first and spread over the XCDs, or session
+// order; every tile above kFusedLdsMaxN becomes `members` descriptors (a
+// cooperative group, members 1 or kGroupMembers). Returns the expanded list.
+// radix[4]: first-stage radix per tile size 32, 64, 128, 256 (0: default;
+// null: keep the descriptors' plans); largest_first: else session order.
+std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members,
+                                  const uint32_t* radix = nullptr, bool largest_first = true);
+// First-stage radix choices of the fused sweep for an n x n tile; the first
+// entry is the default. Returns the number of choices (0: size unsupported).
+uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]);
+// Workgroups of a cooperative group for a 256^2 tile (one 64-line chunk each).
+constexpr uint32_t kGroupMembers = 4;
+// Group size to use for the large tiles of a launch: `requested` when the
+// expanded grid can be resident at once, else 1.
+uint32_t group_members_for(const std::vector<TileDesc>& tiles, uint32_t nmax, uint32_t requested);
+// Bytes of a session's cooperative-group state for an n x n tile.
+size_t group_state_bytes(uint32_t n);
+// Largest grid whose workgroups are all resident at once (one per CU at the
+// sweep kernel's LDS size).
+uint32_t sweep_coresident_limit(uint32_t nmax);
 // Packed FP32 twiddle tables of the sweep kernel (kTwTotal float2).
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream);
 // Whether the fused sweep can run this session (else *why says why not).
 bool sweep_supports(const fpmhip_session& s, const char** why);
 // Raises the kernel's dynamic LDS limit (outside any graph capture).
 void prepare_sweep(uint32_t nmax);
-void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, hipStream_t stream);
+// Launches the sweep over `count` descriptors; a grid holding cooperative
+// groups is launched cooperatively (all workgroups co-resident).
+void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, bool cooperative,
+                  hipStream_t stream);
 
 }  // namespace fpmhip

@@ -181,31 +181,34 @@ __device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid) {
   __syncthreads();
 }
 
+// Two-stage radix plans of an N-point line transform: first stage radix P0,
+// second N / P0 (each at most 16). kDefaultP0 is the plan used unless a
+// batch schedule picks another.
 template <int N>
-struct Plan;
+struct DefaultPlan;
 template <>
-struct Plan<32> {
-  static constexpr int P0 = 8, P1 = 4;
+struct DefaultPlan<32> {
+  static constexpr int P0 = 8;
 };
 template <>
-struct Plan<64> {
-  static constexpr int P0 = 8, P1 = 8;
+struct DefaultPlan<64> {
+  static constexpr int P0 = 8;
 };
 template <>
-struct Plan<128> {
-  static constexpr int P0 = 16, P1 = 8;
+struct DefaultPlan<128> {
+  static constexpr int P0 = 16;
 };
 template <>
-struct Plan<256> {
-  static constexpr int P0 = 16, P1 = 16;
+struct DefaultPlan<256> {
+  static constexpr int P0 = 16;
 };
 
 // Unscaled 2-D DFT in LDS with the direction given by `tw`.
-template <int N, bool INV>
+template <int N, bool INV, int P0>
 __device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw, int tid) {
-  constexpr int P0 = Plan<N>::P0, P1 = Plan<N>::P1;
+  constexpr int P1 = N / P0;
   constexpr int LD = pitch<N>();
-  static_assert(P0 * P1 == N, "plan");
+  static_assert(P0 * P1 == N && P0 <= 16 && P1 <= 16, "plan");
   stage<N, N, P0, 1, LD, 1, INV>(lds, tw, tid);
   stage<N, N, P1, P0, LD, 1, INV>(lds, tw, tid);
   stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
@@ -269,6 +272,46 @@ __device__ __forceinline__ float point_loss_f(float predicted, float measured, i
   return r * r;
 }
 
+// Global-memory access policies. Plain: ordinary cached loads and stores
+// (a tile owned by one workgroup). Coherent: relaxed device-scope atomic
+// loads and stores (64-bit each), which the hardware serves from the
+// device-coherent level, so that workgroups of one cooperative tile group on
+// different CUs / XCDs see each other's writes after a group barrier without
+// invalidating or writing back whole caches.
+struct PlainMem {
+  static constexpr bool kCoherent = false;
+  __device__ static __forceinline__ double2 ld(const double2* p) { return *p; }
+  __device__ static __forceinline__ void st(double2* p, double2 v) { *p = v; }
+  __device__ static __forceinline__ float2 ld(const float2* p) { return *p; }
+  __device__ static __forceinline__ void st(float2* p, float2 v) { *p = v; }
+};
+
+struct CoherentMem {
+  static constexpr bool kCoherent = true;
+  __device__ static __forceinline__ uint64_t ld64(const void* p) {
+    return __hip_atomic_load(reinterpret_cast<uint64_t*>(const_cast<void*>(p)), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static __forceinline__ void st64(void* p, uint64_t v) {
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static __forceinline__ double2 ld(const double2* p) {
+    return make_double2(__longlong_as_double((long long)ld64(p)),
+                        __longlong_as_double((long long)ld64(reinterpret_cast<const double*>(p) + 1)));
+  }
+  __device__ static __forceinline__ void st(double2* p, double2 v) {
+    st64(p, (uint64_t)__double_as_longlong(v.x));
+    st64(reinterpret_cast<double*>(p) + 1, (uint64_t)__double_as_longlong(v.y));
+  }
+  __device__ static __forceinline__ float2 ld(const float2* p) {
+    const uint64_t u = ld64(p);
+    return make_float2(__uint_as_float((uint32_t)u), __uint_as_float((uint32_t)(u >> 32)));
+  }
+  __device__ static __forceinline__ void st(float2* p, float2 v) {
+    st64(p, (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32));
+  }
+};
+
 // Measured value of one pixel (uint16 counts or f64).
 __device__ __forceinline__ float measured(const TileDesc& T, uint64_t i) {
   return T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[i] : (float)static_cast<const double*>(T.frames)[i];
@@ -293,23 +336,26 @@ __device__ __forceinline__ float2 replace_amplitude(const TileDesc& T, uint64_t 
   return f2(fsqrt(target) - fv.x, -fv.y);
 }
 
-// mPIE momentum event over the tile's whole R^2 spectrum (common.rs:308-387).
-__device__ __forceinline__ void momentum_event(const TileDesc& T, int tid) {
+// mPIE momentum event over the tile's whole R^2 spectrum (common.rs:308-387);
+// workgroup `member` of `members` takes every members-th block of elements.
+template <class MEM = PlainMem>
+__device__ __forceinline__ void momentum_event(const TileDesc& T, int tid, uint32_t member = 0,
+                                               uint32_t members = 1) {
   const uint32_t M = T.rh * T.rw;
   double2* __restrict__ spec = T.spec;
   double2* __restrict__ vel = T.vel;
   double2* __restrict__ anc = T.anc;
   bool bad = false;
   constexpr int U = 4;  // loads of U elements in flight before their use
-  for (uint32_t base = tid; base < M; base += U * kThreads) {
+  for (uint32_t base = member * U * kThreads + tid; base < M; base += members * U * kThreads) {
     double2 o[U], v[U], an[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t i = base + u * kThreads;
       if (i < M) {
-        o[u] = spec[i];
-        v[u] = vel[i];
-        an[u] = anc[i];
+        o[u] = MEM::ld(spec + i);
+        v[u] = MEM::ld(vel + i);
+        an[u] = MEM::ld(anc + i);
       }
     }
 #pragma unroll
@@ -321,9 +367,9 @@ __device__ __forceinline__ void momentum_event(const TileDesc& T, int tid) {
         const double2 no =
             T.feedback == 0.0 ? o[u] : make_double2(fma(T.feedback, nv.x, o[u].x), fma(T.feedback, nv.y, o[u].y));
         bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
-        vel[i] = nv;
-        spec[i] = no;
-        anc[i] = no;
+        MEM::st(vel + i, nv);
+        MEM::st(spec + i, no);
+        MEM::st(anc + i, no);
       }
     }
   }
@@ -347,7 +393,7 @@ __device__ __forceinline__ void finish_sweep(const TileDesc& T, double obj_sum, 
 // the product is exactly zero, so the spectrum taps are not read at all;
 // vertically adjacent pixels share one row of taps. No read leaves the
 // (n+1)^2 window.
-template <int STRIP, class Put>
+template <int STRIP, class MEM = PlainMem, class Put>
 __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
                                              int pstride, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du,
                                              double wll, double wlu, double wul, double wuu, Put put) {
@@ -359,10 +405,10 @@ __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, c
     float2 out = f2(0.0f, 0.0f);
     if (P.x != 0.0f || P.y != 0.0f) {
       if (!have) {
-        a = spec[off];
-        b = spec[off + dc];
+        a = MEM::ld(spec + off);
+        b = MEM::ld(spec + off + dc);
       }
-      const double2 c2 = du ? spec[off + du] : a, d = du ? spec[off + du + dc] : b;
+      const double2 c2 = du ? MEM::ld(spec + off + du) : a, d = du ? MEM::ld(spec + off + du + dc) : b;
       const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
                                      fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
       out = fmul(f2((float)v.x, (float)v.y), P);
@@ -378,7 +424,7 @@ __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, c
 }
 
 // Integer-offset crop: the patch is a plain copy of the window (crop.rs:79-102).
-template <int STRIP, class Put>
+template <int STRIP, class MEM = PlainMem, class Put>
 __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
                                            int pstride, uint32_t off, uint32_t rw, Put put) {
 #pragma unroll
@@ -386,7 +432,7 @@ __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, con
     const float2 P = pcol[j * pstride];
     float2 out = f2(0.0f, 0.0f);
     if (P.x != 0.0f || P.y != 0.0f) {
-      const double2 v = spec[off];
+      const double2 v = MEM::ld(spec + off);
       out = fmul(f2((float)v.x, (float)v.y), P);
     }
     put(j, out);
@@ -399,7 +445,7 @@ __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, con
 // sums its <= 4 weighted taps of U (fetched by u(row, col) in centred patch
 // coordinates) and is read-modified-written in FP64 only when that sum is
 // non-zero.
-template <int N, int THREADS, class GetU>
+template <int N, int THREADS, class MEM = PlainMem, class GetU>
 __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const SrcParam& sp, uint32_t rw,
                                               double scale, int tid, int a0, int a1, GetU u) {
   const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
@@ -447,11 +493,12 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const 
     double2 old[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k)
-      if (hit[k]) old[k] = spec[off[k]];
+      if (hit[k]) old[k] = MEM::ld(spec + off[k]);
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k)
       if (hit[k])
-        spec[off[k]] = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
+        MEM::st(spec + off[k],
+                make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
   }
 }
 
@@ -460,8 +507,9 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const 
 // ((r + N/2) mod N, (c + N/2) mod N): for even N, fftshift and ifftshift are
 // both that swap, so the shifted LDS reads and writes of phases 1 and 5 stay
 // inside the owning thread, and vertically adjacent pixels share bilinear taps.
-template <int N>
-__device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi, double* red) {
+template <int N, int RADIX>
+__device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi,
+                                           double* red) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int LD = pitch<N>();
@@ -475,125 +523,201 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   const float eps_f = (float)eps;
   const Denominator den = make_denominator(T, fmax(T.max_p, eps));
   uint64_t counter = *T.counter;
-  double obj_sum = 0.0, w_sum = 0.0;
-  for (uint32_t pos = 0; pos < T.F; ++pos) {
-    const uint32_t f = T.order ? T.order[pos] : pos;
-    const double fw = T.weights[f];
-    const int tid = opaque_tid();
-    if (fw == 0.0) {
-      if (tid == 0) T.frame_loss[pos] = 0.0;
-      continue;
-    }
-    const SrcParam sp = T.src[f];
-    const float gain_f = (float)T.gains[f];
-    const float inv_gain = (float)(1.0 / T.gains[f]);
-    // 1. patch x pupil on a strip and its partner strip, written swapped (ifftshift)
-    if (tid < COLS_THREADS) {
-      const int c = tid % N, r0 = (tid / N) * STRIP;
-      const int c1 = (c + HALF) % N;
+  for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
+    double obj_sum = 0.0, w_sum = 0.0;
+    for (uint32_t pos = 0; pos < T.F; ++pos) {
+      const uint32_t f = T.order ? T.order[pos] : pos;
+      const double fw = T.weights[f];
+      const int tid = opaque_tid();
+      if (fw == 0.0) {
+        if (tid == 0) T.frame_loss[pos] = 0.0;
+        continue;
+      }
+      const SrcParam sp = T.src[f];
+      const float gain_f = (float)T.gains[f];
+      const float inv_gain = (float)(1.0 / T.gains[f]);
+      // 1. patch x pupil on a strip and its partner strip, written swapped (ifftshift)
+      if (tid < COLS_THREADS) {
+        const int c = tid % N, r0 = (tid / N) * STRIP;
+        const int c1 = (c + HALF) % N;
 #pragma unroll
-      for (int half = 0; half < 2; ++half) {
-        const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
-        const int cs = (cc + HALF) % N;
-        auto put = [&](int j, float2 v) { lds[((rb + j + HALF) % N) * LD + cs] = v; };
-        if (sp.zero) {
-          copy_strip<STRIP>(spec, pupil + rb * N + cc, N, ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw,
-                            put);
-        } else {
-          gather_strip<STRIP>(spec, pupil + rb * N + cc, N, (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw,
-                              sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
-                              sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
+        for (int half = 0; half < 2; ++half) {
+          const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
+          const int cs = (cc + HALF) % N;
+          auto put = [&](int j, float2 v) { lds[((rb + j + HALF) % N) * LD + cs] = v; };
+          if (sp.zero) {
+            copy_strip<STRIP>(spec, pupil + rb * N + cc, N, ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw,
+                              put);
+          } else {
+            gather_strip<STRIP>(spec, pupil + rb * N + cc, N, (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw,
+                                sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
+                                sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
+          }
+        }
+      }
+      __syncthreads();
+      // 2. inverse FFT (unscaled)
+      fft2_lds<N, true, RADIX>(lds, twi, tid);
+      // 3. amplitude replacement and objective terms (pointwise, natural order).
+      //    The LDS field becomes psi' - psi: the forward transform is linear,
+      //    so FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
+      //    without cancelling two nearly equal spectra in FP32. Per-pixel terms
+      //    are FP32; the frame objective is accumulated in FP64.
+      const uint64_t fbase = (uint64_t)f * NN;
+      float lsum_f = 0.0f;
+      int lcnt_i = 0;
+      {
+        constexpr int PP = (NN + kThreads - 1) / kThreads;  // pixels per thread
+        float y[PP];
+        // issue every measurement load of the thread before using any of them
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+          const int p = tid + j * kThreads;
+          y[j] = 0.0f;
+          if (p < NN) y[j] = measured(T, fbase + p);
+        }
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+          const int p = tid + j * kThreads;
+          if (p < NN) {
+            const int lp = (p / N) * LD + p % N;
+            lds[lp] =
+                replace_amplitude(T, fbase, (uint64_t)NN, p, lds[lp], y[j], gain_f, inv_gain, eps_f, lsum_f, lcnt_i);
+          }
+        }
+      }
+      // per-thread partials (at most 16 terms) in FP32, workgroup reduction in FP64
+      const double lsum = (double)lsum_f, lcnt = (double)lcnt_i;
+      const double ls = wg_sum(lsum, red);
+      const double lc = wg_sum(lcnt, red);
+      if (tid == 0) {
+        const double L = lc > 0.0 ? ls / lc : 0.0;
+        T.frame_loss[pos] = L;
+        obj_sum += fw * L;
+        w_sum += fw;
+      }
+      __syncthreads();
+      // 4. forward FFT (x 1/N applied in phase 5)
+      fft2_lds<N, false, RADIX>(lds, twf, tid);
+      // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
+      if (tid < COLS_THREADS) {
+        const int c = tid % N, r0 = (tid / N) * STRIP;
+        const int c1 = (c + HALF) % N;
+#pragma unroll
+        for (int j = 0; j < STRIP; ++j) {
+          const int r = r0 + j, r1 = r + HALF;
+          const int l0 = r * LD + c, l1 = r1 * LD + c1;
+          const float2 s0 = lds[l1], s1 = lds[l0];
+          const float2 P0 = pupil[r * N + c], P1 = pupil[r1 * N + c1];
+          const float k0 = inv_n / fmaf(den.b, fmaf(P0.x, P0.x, P0.y * P0.y), den.a);
+          const float k1 = inv_n / fmaf(den.b, fmaf(P1.x, P1.x, P1.y * P1.y), den.a);
+          const float2 u0 = fmulc(P0, s0), u1 = fmulc(P1, s1);
+          lds[l0] = f2(u0.x * k0, u0.y * k0);
+          lds[l1] = f2(u1.x * k1, u1.y * k1);
+        }
+      }
+      __syncthreads();
+      // 6. adjoint insert: gather over the destination window, FP64 RMW of the
+      //    pixels that receive a non-zero update (U is zero outside the pupil)
+      const double scale = fw * T.object_step;
+      if (sp.zero) {
+        for (int p = tid; p < NN; p += kThreads) {
+          const int r = p / N, c = p % N;
+          const float2 u = lds[r * LD + c];
+          if (u.x != 0.0f || u.y != 0.0f) {
+            double2& d = spec[(sp.sr + r) * T.rw + sp.sc + c];
+            d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
+          }
+        }
+      } else {
+        insert_window<N, kThreads>(spec, sp, T.rw, scale, tid, 0, N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0),
+                                   [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
+      }
+      __syncthreads();
+      // 7. mPIE momentum over the whole tile spectrum (common.rs:308-387)
+      if (T.momentum) {
+        counter += 1;
+        if (counter >= T.interval) {
+          momentum_event(T, tid);
+          counter = 0;
         }
       }
     }
-    __syncthreads();
-    // 2. inverse FFT (unscaled)
-    fft2_lds<N, true>(lds, twi, tid);
-    // 3. amplitude replacement and objective terms (pointwise, natural order).
-    //    The LDS field becomes psi' - psi: the forward transform is linear,
-    //    so FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
-    //    without cancelling two nearly equal spectra in FP32. Per-pixel terms
-    //    are FP32; the frame objective is accumulated in FP64.
-    const uint64_t fbase = (uint64_t)f * NN;
-    float lsum_f = 0.0f;
-    int lcnt_i = 0;
-    {
-      constexpr int PP = (NN + kThreads - 1) / kThreads;  // pixels per thread
-      float y[PP];
-      // issue every measurement load of the thread before using any of them
-#pragma unroll
-      for (int j = 0; j < PP; ++j) {
-        const int p = tid + j * kThreads;
-        y[j] = 0.0f;
-        if (p < NN) y[j] = measured(T, fbase + p);
-      }
-#pragma unroll
-      for (int j = 0; j < PP; ++j) {
-        const int p = tid + j * kThreads;
-        if (p < NN) {
-          const int lp = (p / N) * LD + p % N;
-          lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, p, lds[lp], y[j], gain_f, inv_gain, eps_f, lsum_f, lcnt_i);
-        }
-      }
-    }
-    // per-thread partials (at most 16 terms) in FP32, workgroup reduction in FP64
-    const double lsum = (double)lsum_f, lcnt = (double)lcnt_i;
-    const double ls = wg_sum(lsum, red);
-    const double lc = wg_sum(lcnt, red);
-    if (tid == 0) {
-      const double L = lc > 0.0 ? ls / lc : 0.0;
-      T.frame_loss[pos] = L;
-      obj_sum += fw * L;
-      w_sum += fw;
-    }
-    __syncthreads();
-    // 4. forward FFT (x 1/N applied in phase 5)
-    fft2_lds<N, false>(lds, twf, tid);
-    // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
-    if (tid < COLS_THREADS) {
-      const int c = tid % N, r0 = (tid / N) * STRIP;
-      const int c1 = (c + HALF) % N;
-#pragma unroll
-      for (int j = 0; j < STRIP; ++j) {
-        const int r = r0 + j, r1 = r + HALF;
-        const int l0 = r * LD + c, l1 = r1 * LD + c1;
-        const float2 s0 = lds[l1], s1 = lds[l0];
-        const float2 P0 = pupil[r * N + c], P1 = pupil[r1 * N + c1];
-        const float k0 = inv_n / fmaf(den.b, fmaf(P0.x, P0.x, P0.y * P0.y), den.a);
-        const float k1 = inv_n / fmaf(den.b, fmaf(P1.x, P1.x, P1.y * P1.y), den.a);
-        const float2 u0 = fmulc(P0, s0), u1 = fmulc(P1, s1);
-        lds[l0] = f2(u0.x * k0, u0.y * k0);
-        lds[l1] = f2(u1.x * k1, u1.y * k1);
-      }
-    }
-    __syncthreads();
-    // 6. adjoint insert: gather over the destination window, FP64 RMW of the
-    //    pixels that receive a non-zero update (U is zero outside the pupil)
-    const double scale = fw * T.object_step;
-    if (sp.zero) {
-      for (int p = tid; p < NN; p += kThreads) {
-        const int r = p / N, c = p % N;
-        const float2 u = lds[r * LD + c];
-        if (u.x != 0.0f || u.y != 0.0f) {
-          double2& d = spec[(sp.sr + r) * T.rw + sp.sc + c];
-          d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
-        }
-      }
-    } else {
-      insert_window<N, kThreads>(spec, sp, T.rw, scale, tid, 0, N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0),
-                                 [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
-    }
-    __syncthreads();
-    // 7. mPIE momentum over the whole tile spectrum (common.rs:308-387)
-    if (T.momentum) {
-      counter += 1;
-      if (counter >= T.interval) {
-        momentum_event(T, tid);
-        counter = 0;
-      }
-    }
+    finish_sweep(T, obj_sum, w_sum, counter);
   }
-  finish_sweep(T, obj_sum, w_sum, counter);
+}
+
+// Copies between a block of a row-major global array (rows of W elements at
+// a stride of `gstride`) and LDS (rows at a pitch of LD): every load of the
+// thread is issued before the first store.
+template <int TOTAL, int W, int LD, class MEM = PlainMem>
+__device__ __forceinline__ void copy_in(float2* lds, const float2* __restrict__ g, int gstride, int tid) {
+  constexpr int PER = TOTAL / kThreads;
+  static_assert(TOTAL % kThreads == 0, "copy shape");
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = MEM::ld(g + (i / W) * gstride + i % W);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    lds[(i / W) * LD + i % W] = v[j];
+  }
+}
+
+template <int TOTAL, int W, int LD, class MEM = PlainMem>
+__device__ __forceinline__ void copy_out(const float2* lds, float2* __restrict__ g, int gstride, int tid) {
+  constexpr int PER = TOTAL / kThreads;
+  static_assert(TOTAL % kThreads == 0, "copy shape");
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = lds[(i / W) * LD + i % W];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    MEM::st(g + (i / W) * gstride + i % W, v[j]);
+  }
+}
+
+// Group barrier of a cooperative tile (members > 1). Each wave first waits
+// for its own device-coherent stores to complete (s_waitcnt vmcnt(0); a
+// completed device-scope store is visible to device-scope loads on every
+// CU), then one thread per workgroup arrives on the group's counter and
+// spins, bounded, until every member has arrived. `passed` counts the
+// group's barriers since its state was last reset. Returns false when the
+// group stalled (the spin limit expired here or in another member): the
+// caller abandons the sweep, so every workgroup still exits.
+constexpr int kWaitVmcnt0 = 0x0F70;  // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15)
+constexpr uint32_t kGroupSpinLimit = 1u << 20;
+
+__device__ __forceinline__ bool group_barrier(const TileDesc& T, uint64_t& passed, int* bflag) {
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+  passed += 1;
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(&T.group[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t target = passed * T.members;
+    uint32_t spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(&T.group[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&T.group[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+          ++spins > kGroupSpinLimit) {
+        __hip_atomic_store(&T.group[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        T.status[0] = 2;
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    *bflag = ok;
+  }
+  __syncthreads();
+  return *bflag != 0;
 }
 
 // Tiles whose N x N complex64 field does not fit in LDS (N = 256: 512 KiB).
@@ -606,10 +730,16 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
 //      field change psi' - psi, forward column transforms, back to T.field
 //   C  rows, in centred order: forward row transforms, fftshift by index,
 //      U = conj(P) diff / N / den in LDS, then the adjoint insert of the
-//      destination rows this chunk completes (the row above the chunk comes
-//      from `prev`, the previous chunk's last row of U)
+//      destination rows this chunk completes (the row above the chunk is the
+//      previous chunk's last row of U)
 //
-// The LED loop, objective, momentum and arithmetic are those of sweep_tile.
+// One workgroup runs the chunks in turn (MEM = PlainMem), or a cooperative
+// group of N/CH workgroups takes one chunk each (MEM = CoherentMem): group
+// barriers separate A, B and C, the insert from the publication of the
+// chunk edges, and one LED from the next; per-member objective sums are
+// combined by member 0 in member order. The LED loop, objective, momentum
+// and arithmetic are those of sweep_tile, and the spectra do not depend on
+// the group size.
 template <int N>
 struct BigPlan;
 template <>
@@ -627,56 +757,24 @@ __host__ __device__ constexpr uint32_t sweep_field_elems(uint32_t n) {
   return n <= kFusedLdsMaxN ? n * (n + 1) : (uint32_t)big_field_elems<256>();
 }
 
-// Copies between a block of a row-major global array (rows of W elements at
-// a stride of `gstride`) and LDS (rows at a pitch of LD): every load of the
-// thread is issued before the first store.
-template <int TOTAL, int W, int LD>
-__device__ __forceinline__ void copy_in(float2* lds, const float2* __restrict__ g, int gstride, int tid) {
-  constexpr int PER = TOTAL / kThreads;
-  static_assert(TOTAL % kThreads == 0, "copy shape");
-  float2 v[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    v[j] = g[(i / W) * gstride + i % W];
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    lds[(i / W) * LD + i % W] = v[j];
-  }
-}
-
-template <int TOTAL, int W, int LD>
-__device__ __forceinline__ void copy_out(const float2* lds, float2* __restrict__ g, int gstride, int tid) {
-  constexpr int PER = TOTAL / kThreads;
-  static_assert(TOTAL % kThreads == 0, "copy shape");
-  float2 v[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    v[j] = lds[(i / W) * LD + i % W];
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    g[(i / W) * gstride + i % W] = v[j];
-  }
-}
-
-template <int N>
-__device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, float2* prev, const float2* twf, const float2* twi,
-                               double* red) {
+template <int N, class MEM>
+__device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, float2* prev, const float2* twf,
+                                               const float2* twi, double* red, int* bflag) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int CH = BigPlan<N>::CH;
   constexpr int NCH = N / CH;
   constexpr int LDR = N + 1;   // row chunk: CH rows of N (+1 pad)
   constexpr int LDC = CH + 1;  // column chunk: N rows of CH (+1 pad)
-  constexpr int P0 = Plan<N>::P0, P1 = Plan<N>::P1;
+  constexpr int P0 = DefaultPlan<N>::P0, P1 = N / P0;
   constexpr int SPLIT = kThreads / N;  // row strips per chunk in phase A
   constexpr int STRIP = CH / SPLIT;    // rows per strip
+  constexpr bool COOP = MEM::kCoherent;
   static_assert((N & (N - 1)) == 0 && HALF % CH == 0 && CH % SPLIT == 0, "big tile shape");
+  // a cooperative group has one member per chunk; a lone workgroup runs them all
+  const int first = COOP ? (int)T.member : 0, step = COOP ? NCH : 1;
+  const bool lead = !COOP || T.member == 0;
+  uint64_t passed = COOP ? T.group[1] : 0;
   const float inv_n = 1.0f / (float)NN;
   double2* __restrict__ spec = T.spec;
   float2* __restrict__ fld = T.field;
@@ -684,172 +782,215 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   const float eps_f = (float)T.eps;
   const Denominator den = make_denominator(T, fmax(T.max_p, T.eps));
   uint64_t counter = *T.counter;
-  double obj_sum = 0.0, w_sum = 0.0;
-  for (uint32_t pos = 0; pos < T.F; ++pos) {
-    const uint32_t f = T.order ? T.order[pos] : pos;
-    const double fw = T.weights[f];
-    int tid = opaque_tid();
-    if (fw == 0.0) {
-      if (tid == 0) T.frame_loss[pos] = 0.0;
-      continue;
-    }
-    const SrcParam sp = T.src[f];
-    const float gain_f = (float)T.gains[f];
-    const float inv_gain = (float)(1.0 / T.gains[f]);
-    // A. shifted row r' holds centred row r' ^ HALF (ifftshift for even N)
-    for (int k = 0; k < NCH; ++k) {
-      tid = opaque_tid();
-      {
-        const int c = tid % N, s0 = (tid / N) * STRIP;
-        const int rc = ((k * CH) ^ HALF) + s0;  // first centred row of the strip
-        const int cs = c ^ HALF;
-        auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
-        if (sp.zero) {
-          copy_strip<STRIP>(spec, pupil + rc * N + c, N, ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw,
-                            put);
-        } else {
-          gather_strip<STRIP>(spec, pupil + rc * N + c, N, (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
-                              sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
-                              sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
-        }
+  for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
+    double obj_sum = 0.0, w_sum = 0.0;
+    for (uint32_t pos = 0; pos < T.F; ++pos) {
+      const uint32_t f = T.order ? T.order[pos] : pos;
+      const double fw = T.weights[f];
+      int tid = opaque_tid();
+      if (fw == 0.0) {
+        if (lead && tid == 0) T.frame_loss[pos] = 0.0;
+        continue;
       }
-      __syncthreads();
-      stage<N, CH, P0, 1, LDR, 1, true>(lds, twi, tid);
-      stage<N, CH, P1, P0, LDR, 1, true>(lds, twi, tid);
-      copy_out<CH * N, N, LDR>(lds, fld + k * CH * N, N, opaque_tid());
-      __syncthreads();
-    }
-    // B. columns: natural image order after the inverse column transforms
-    const uint64_t fbase = (uint64_t)f * NN;
-    double lsum = 0.0;
-    int lcnt = 0;
-    for (int jc = 0; jc < NCH; ++jc) {
-      tid = opaque_tid();
-      const int c0 = jc * CH;
-      copy_in<N * CH, CH, LDC>(lds, fld + c0, N, opaque_tid());
-      __syncthreads();
-      stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
-      stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
-      {
-        // measurement loads in groups of G issued before their use
-        constexpr int PP = N * CH / kThreads;
-        constexpr int G = 8;
-        const int tid = opaque_tid();
-        float lsum_f = 0.0f;
-#pragma unroll
-        for (int g = 0; g < PP; g += G) {
-          float y[G];
-#pragma unroll
-          for (int j = 0; j < G; ++j) {
-            const int i = tid + (g + j) * kThreads;
-            y[j] = measured(T, fbase + (uint64_t)(i / CH) * N + c0 + i % CH);
-          }
-#pragma unroll
-          for (int j = 0; j < G; ++j) {
-            const int i = tid + (g + j) * kThreads;
-            const int lp = (i / CH) * LDC + i % CH;
-            lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, (i / CH) * N + c0 + i % CH, lds[lp], y[j], gain_f,
-                                        inv_gain, eps_f, lsum_f, lcnt);
+      const SrcParam sp = T.src[f];
+      const float gain_f = (float)T.gains[f];
+      const float inv_gain = (float)(1.0 / T.gains[f]);
+      // A. shifted row r' holds centred row r' ^ HALF (ifftshift for even N)
+      for (int k = first; k < NCH; k += step) {
+        tid = opaque_tid();
+        {
+          const int c = tid % N, s0 = (tid / N) * STRIP;
+          const int rc = ((k * CH) ^ HALF) + s0;  // first centred row of the strip
+          const int cs = c ^ HALF;
+          auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
+          if (sp.zero) {
+            copy_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c,
+                                   T.rw, put);
+          } else {
+            gather_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
+                                     sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw,
+                                     sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl,
+                                     sp.ar.wu * sp.ac.wu, put);
           }
         }
-        lsum += (double)lsum_f;
+        __syncthreads();
+        stage<N, CH, P0, 1, LDR, 1, true>(lds, twi, tid);
+        stage<N, CH, P1, P0, LDR, 1, true>(lds, twi, tid);
+        copy_out<CH * N, N, LDR, MEM>(lds, fld + k * CH * N, N, opaque_tid());
+        __syncthreads();
       }
-      __syncthreads();
-      stage<N, CH, P0, 1, 1, LDC, false>(lds, twf, tid);
-      stage<N, CH, P1, P0, 1, LDC, false>(lds, twf, tid);
-      copy_out<N * CH, CH, LDC>(lds, fld + c0, N, opaque_tid());
-      __syncthreads();
-    }
-    {
-      const double ls = wg_sum(lsum, red);
-      const double lc = wg_sum((double)lcnt, red);
-      if (tid == 0) {
-        const double L = lc > 0.0 ? ls / lc : 0.0;
-        T.frame_loss[pos] = L;
-        obj_sum += fw * L;
-        w_sum += fw;
+      if (COOP && !group_barrier(T, passed, bflag)) return;
+      // B. columns: natural image order after the inverse column transforms
+      const uint64_t fbase = (uint64_t)f * NN;
+      double lsum = 0.0;
+      int lcnt = 0;
+      for (int jc = first; jc < NCH; jc += step) {
+        tid = opaque_tid();
+        const int c0 = jc * CH;
+        copy_in<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid());
+        __syncthreads();
+        stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
+        stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
+        {
+          // measurement loads in groups of G issued before their use
+          constexpr int PP = N * CH / kThreads;
+          constexpr int G = 8;
+          const int tid = opaque_tid();
+          float lsum_f = 0.0f;
+#pragma unroll
+          for (int g = 0; g < PP; g += G) {
+            float y[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              const int i = tid + (g + j) * kThreads;
+              y[j] = measured(T, fbase + (uint64_t)(i / CH) * N + c0 + i % CH);
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              const int i = tid + (g + j) * kThreads;
+              const int lp = (i / CH) * LDC + i % CH;
+              lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, (i / CH) * N + c0 + i % CH, lds[lp], y[j], gain_f,
+                                          inv_gain, eps_f, lsum_f, lcnt);
+            }
+          }
+          lsum += (double)lsum_f;
+        }
+        __syncthreads();
+        stage<N, CH, P0, 1, 1, LDC, false>(lds, twf, tid);
+        stage<N, CH, P1, P0, 1, LDC, false>(lds, twf, tid);
+        copy_out<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid());
+        __syncthreads();
       }
-      __syncthreads();
-    }
-    // C. rows in centred order: centred row q*CH + lr is natural row (q*CH + lr) ^ HALF
-    const double scale = fw * T.object_step;
-    for (int q = 0; q < NCH; ++q) {
-      tid = opaque_tid();
-      const int nr0 = (q * CH) ^ HALF;
-      copy_in<CH * N, N, LDR>(lds, fld + nr0 * N, N, opaque_tid());
-      __syncthreads();
-      stage<N, CH, P0, 1, LDR, 1, false>(lds, twf, tid);
-      stage<N, CH, P1, P0, LDR, 1, false>(lds, twf, tid);
-      for (int i = tid; i < CH * N; i += kThreads) {
-        const int lr = i / N, kc = i % N;
-        const float2 P = pupil[(q * CH + lr) * N + (kc ^ HALF)];
-        const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
-        const float2 u = fmulc(P, lds[lr * LDR + kc]);
-        lds[lr * LDR + kc] = f2(u.x * kk, u.y * kk);
-      }
-      __syncthreads();
-      // destination rows of this chunk (and row N after the last chunk)
       {
+        double ls = wg_sum(lsum, red);
+        double lc = wg_sum((double)lcnt, red);
+        if (COOP) {
+          if (tid == 0) MEM::st(T.partial + T.member, make_double2(ls, lc));
+          if (!group_barrier(T, passed, bflag)) return;
+          if (lead && tid == 0) {
+            ls = 0.0;
+            lc = 0.0;
+            for (uint32_t m = 0; m < T.members; ++m) {
+              const double2 pm = MEM::ld(T.partial + m);
+              ls += pm.x;
+              lc += pm.y;
+            }
+          }
+        }
+        if (lead && tid == 0) {
+          const double L = lc > 0.0 ? ls / lc : 0.0;
+          T.frame_loss[pos] = L;
+          obj_sum += fw * L;
+          w_sum += fw;
+        }
+        __syncthreads();
+      }
+      // C. rows in centred order: centred row q*CH + lr is natural row (q*CH + lr) ^ HALF
+      const double scale = fw * T.object_step;
+      for (int q = first; q < NCH; q += step) {
+        tid = opaque_tid();
+        const int nr0 = (q * CH) ^ HALF;
+        copy_in<CH * N, N, LDR, MEM>(lds, fld + nr0 * N, N, opaque_tid());
+        __syncthreads();
+        stage<N, CH, P0, 1, LDR, 1, false>(lds, twf, tid);
+        stage<N, CH, P1, P0, LDR, 1, false>(lds, twf, tid);
+        for (int i = tid; i < CH * N; i += kThreads) {
+          const int lr = i / N, kc = i % N;
+          const float2 P = pupil[(q * CH + lr) * N + (kc ^ HALF)];
+          const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
+          const float2 u = fmulc(P, lds[lr * LDR + kc]);
+          lds[lr * LDR + kc] = f2(u.x * kk, u.y * kk);
+        }
+        __syncthreads();
         const int a0 = q * CH;
+        if (COOP) {
+          // publish this chunk's last row of U for the next chunk's insert
+          if (q + 1 < NCH)
+            for (int i = opaque_tid(); i < N; i += kThreads) MEM::st(T.edge + q * N + i, lds[(CH - 1) * LDR + i]);
+          if (!group_barrier(T, passed, bflag)) return;
+        }
+        // destination rows of this chunk (and row N after the last chunk)
         if (sp.zero) {
           for (int i = opaque_tid(); i < CH * N; i += kThreads) {
             const int lr = i / N, c = i % N;
             const float2 u = lds[lr * LDR + (c ^ HALF)];
             if (u.x != 0.0f || u.y != 0.0f) {
-              double2& d = spec[(sp.sr + a0 + lr) * T.rw + sp.sc + c];
-              d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
+              double2* d = spec + (sp.sr + a0 + lr) * T.rw + sp.sc + c;
+              const double2 o = MEM::ld(d);
+              MEM::st(d, make_double2(fma(scale, (double)u.x, o.x), fma(scale, (double)u.y, o.y)));
             }
           }
         } else {
           const int a1 = a0 + CH + ((q == NCH - 1 && sp.ar.upper != sp.ar.lower) ? 1 : 0);
-          insert_window<N, kThreads>(spec, sp, T.rw, scale, opaque_tid(), a0, a1, [&](int sr_, int sc_) {
-            return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)] : prev[sc_ ^ HALF];
+          const float2* above = COOP ? T.edge + (q - 1) * N : prev;
+          insert_window<N, kThreads, MEM>(spec, sp, T.rw, scale, opaque_tid(), a0, a1, [&](int sr_, int sc_) {
+            return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)]
+                             : (COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF]);
           });
         }
-      }
-      __syncthreads();
-      if (q + 1 < NCH) {
-        for (int i = tid; i < N; i += kThreads) prev[i] = lds[(CH - 1) * LDR + i];
         __syncthreads();
+        if (!COOP && q + 1 < NCH) {
+          for (int i = tid; i < N; i += kThreads) prev[i] = lds[(CH - 1) * LDR + i];
+          __syncthreads();
+        }
+      }
+      if (COOP && !group_barrier(T, passed, bflag)) return;
+      if (T.momentum) {
+        counter += 1;
+        if (counter >= T.interval) {
+          momentum_event<MEM>(T, tid, COOP ? T.member : 0, COOP ? T.members : 1);
+          counter = 0;
+          if (COOP && !group_barrier(T, passed, bflag)) return;
+        }
       }
     }
-    if (T.momentum) {
-      counter += 1;
-      if (counter >= T.interval) {
-        momentum_event(T, tid);
-        counter = 0;
-      }
-    }
+    if (lead) finish_sweep(T, obj_sum, w_sum, counter);
   }
-  finish_sweep(T, obj_sum, w_sum, counter);
+  if (COOP && T.member == 0 && threadIdx.x == 0) T.group[1] = passed;
 }
 
-__global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__ tiles, const float2* __restrict__ tw_all) {
+__global__ void __launch_bounds__(kThreads)
+    k_sweep(const TileDesc* __restrict__ tiles, const float2* __restrict__ tw_all) {
   extern __shared__ unsigned char smem[];
   float2* lds = reinterpret_cast<float2*>(smem);
-  const TileDesc T = tiles[blockIdx.x];
+  const TileDesc& T = tiles[blockIdx.x];  // fields are read (scalar loads) where used
   const int N = (int)T.n;
   // LDS: field (or chunk) region, [big tiles: previous row of U], twiddles
-  // W_N^j forward then inverse, reduction slots
+  // W_N^j forward then inverse, reduction slots, barrier flag
   float2* prev = lds + sweep_field_elems(N);
-  float2* twf = prev + (N > kFusedLdsMaxN ? N : 0);
+  float2* twf = prev + (N > (int)kFusedLdsMaxN ? N : 0);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
+  int* bflag = reinterpret_cast<int*>(red + 16);
   const float2* src = tw_all + tw_offset(N);
   for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) twf[j] = src[j];
   __syncthreads();
   switch (N) {
     case 32:
-      sweep_tile<32>(T, lds, twf, twi, red);
+      if (T.plan == 4)
+        sweep_tile<32, 4>(T, lds, twf, twi, red);
+      else
+        sweep_tile<32, 8>(T, lds, twf, twi, red);
       break;
     case 64:
-      sweep_tile<64>(T, lds, twf, twi, red);
+      if (T.plan == 4)
+        sweep_tile<64, 4>(T, lds, twf, twi, red);
+      else if (T.plan == 16)
+        sweep_tile<64, 16>(T, lds, twf, twi, red);
+      else
+        sweep_tile<64, 8>(T, lds, twf, twi, red);
       break;
     case 128:
-      sweep_tile<128>(T, lds, twf, twi, red);
+      if (T.plan == 8)
+        sweep_tile<128, 8>(T, lds, twf, twi, red);
+      else
+        sweep_tile<128, 16>(T, lds, twf, twi, red);
       break;
     case 256:
-      sweep_tile_big<256>(T, lds, prev, twf, twi, red);
+      if (T.members > 1)
+        sweep_tile_big<256, CoherentMem>(T, lds, prev, twf, twi, red, bflag);
+      else
+        sweep_tile_big<256, PlainMem>(T, lds, prev, twf, twi, red, bflag);
       break;
     default:
       break;
@@ -860,7 +1001,31 @@ __global__ void __launch_bounds__(kThreads) k_sweep(const TileDesc* __restrict__
 
 size_t sweep_lds_bytes(uint32_t nmax) {
   return ((size_t)sweep_field_elems(nmax) + (nmax > kFusedLdsMaxN ? nmax : 0) + 2 * (size_t)nmax) * sizeof(float2) +
-         16 * sizeof(double);
+         16 * sizeof(double) + 16;
+}
+
+uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
+  switch (n) {
+    case 32:
+      out[0] = 8, out[1] = 4;
+      return 2;
+    case 64:
+      out[0] = 8, out[1] = 4, out[2] = 16;
+      return 3;
+    case 128:
+      out[0] = 16, out[1] = 8;
+      return 2;
+    case 256:
+      out[0] = 16;
+      return 1;
+    default:
+      return 0;
+  }
+}
+
+size_t group_state_bytes(uint32_t n) {
+  // [0, 24) counters, [64, 64 + 16 kGroupMembers) partial sums, then n/64 edge rows
+  return 64 + 16 * (size_t)kGroupMembers + (size_t)(n / 64) * n * sizeof(float2);
 }
 
 void prepare_sweep(uint32_t nmax) {
@@ -868,10 +1033,27 @@ void prepare_sweep(uint32_t nmax) {
                                 (int)sweep_lds_bytes(nmax)));
 }
 
-void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, hipStream_t stream) {
+uint32_t sweep_coresident_limit(uint32_t nmax) {
+  int dev = 0, per_cu = 0, cus = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_sweep), kThreads,
+                                                         sweep_lds_bytes(nmax)));
+  return (uint32_t)(std::max(per_cu, 0) * std::max(cus, 0));
+}
+
+void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, bool cooperative,
+                  hipStream_t stream) {
   const size_t smem = sweep_lds_bytes(nmax);
-  hipLaunchKernelGGL(k_sweep, dim3(count), dim3(kThreads), smem, stream, tiles_dev, tw_all);
-  HIP_CHECK(hipGetLastError());
+  if (cooperative) {
+    // every workgroup of every cooperative group must be resident at once
+    void* args[] = {(void*)&tiles_dev, (void*)&tw_all};
+    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_sweep), dim3(count), dim3(kThreads), args,
+                                         (unsigned)smem, stream));
+  } else {
+    hipLaunchKernelGGL(k_sweep, dim3(count), dim3(kThreads), smem, stream, tiles_dev, tw_all);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 }  // namespace fpmhip

@@ -2,12 +2,16 @@
 //
 // A worker thread owns the device. Jobs are queued by fpmhip_server_submit,
 // admitted while tile slots and the HBM budget allow, initialised on the
-// device, and then advanced one sweep per tick together with every other
-// resident tile by one launch of the fused sweep kernel. A job that reaches
-// its iteration count is finished (result transform, download) and leaves;
-// the queue is re-examined between ticks. Tiles are independent workgroups,
+// device, and then advanced together with every other resident tile by one
+// launch of the fused sweep kernel per tick. Within a tick every tile runs a
+// quota of sweeps inversely proportional to its per-sweep work (LEDs x
+// pixels), so that a small tile is not held to the pace of the largest one
+// resident. A job that reaches its iteration count is finished (result
+// transform, download) and leaves; the queue is re-examined between ticks.
+// Tiles are independent workgroups and sweeps are sequential within a tile,
 // so a job's result does not depend on which other jobs share its ticks.
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -111,6 +115,12 @@ uint64_t job_bytes(const fpmhip_model_desc& d, uint64_t iterations) {
   return 4 * M * 16 + 12 * N * 16 + F * N * 2 + N * 25 + F * 64 + 16 + (iterations + F) * 8;
 }
 
+// Work of one sweep of a job's tile: LED updates x pixels per update.
+double sweep_work(const Job& j) {
+  const double F = (double)(j.desc.mux_row_ptr ? j.desc.frame_count : j.desc.source_count);
+  return F * (double)(j.desc.image_h * j.desc.image_w);
+}
+
 }  // namespace
 
 bool fpmhip_server::admit(Job& j) {
@@ -152,7 +162,10 @@ void fpmhip_server::finish(Job& j) {
     int32_t status = 0;
     HIP_CHECK(hipMemcpyAsync(&status, j.scratch.as<char>() + 12, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (status) {
+    if (status == 2) {
+      rc = FPMHIP_DEVICE_ERROR;
+      j.error = "a cooperative tile group stalled at a group barrier";
+    } else if (status) {
       rc = FPMHIP_NUMERICAL;
       j.error = "mPIE momentum produced a non-finite object or velocity";
     }
@@ -222,27 +235,36 @@ void fpmhip_server::run() {
       continue;
     }
     if (busy_since < 0.0) busy_since = now_s();
-    // one tick: one sweep of every resident tile
+    // one tick: every resident tile advances by its quota of sweeps
+    std::vector<uint32_t> quota(active.size());
     try {
       std::vector<TileDesc> desc(active.size());
       uint32_t nmax = 0;
+      double wmax = 0.0;
+      for (Job* j : active) wmax = std::max(wmax, sweep_work(*j));
       for (size_t i = 0; i < active.size(); ++i) {
         Job& j = *active[i];
         char* base = j.scratch.as<char>();
         fill_tile_desc(desc[i], *j.sess, reinterpret_cast<double*>(base + 16),
                        reinterpret_cast<double*>(base + 16 + j.iterations * sizeof(double)), base,
                        (uint32_t)j.iterations);
+        const double share = std::floor(wmax / std::max(sweep_work(j), 1.0));
+        quota[i] = (uint32_t)std::min<double>(std::max(share, 1.0), (double)(j.iterations - j.done_iterations));
+        desc[i].sweeps = quota[i];
         nmax = std::max(nmax, desc[i].n);
       }
-      order_tiles(desc);
-      if (tiles.bytes < desc.size() * sizeof(TileDesc)) tiles.allocate(cfg.max_tiles * sizeof(TileDesc));
-      HIP_CHECK(hipMemcpyAsync(tiles.ptr, desc.data(), desc.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
-                               ctx->stream));
       if (nmax != prepared_nmax) {
         prepare_sweep(nmax);
         prepared_nmax = nmax;
       }
-      launch_sweep(tiles.as<TileDesc>(), (uint32_t)desc.size(), tw.as<float2>(), nmax, ctx->stream);
+      const std::vector<TileDesc> launch = order_tiles(desc, group_members_for(desc, nmax, kGroupMembers));
+      bool cooperative = false;
+      for (const TileDesc& t : launch) cooperative |= t.members > 1;
+      if (tiles.bytes < launch.size() * sizeof(TileDesc))
+        tiles.allocate(std::max<size_t>(launch.size(), cfg.max_tiles * kGroupMembers) * sizeof(TileDesc));
+      HIP_CHECK(hipMemcpyAsync(tiles.ptr, launch.data(), launch.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
+                               ctx->stream));
+      launch_sweep(tiles.as<TileDesc>(), (uint32_t)launch.size(), tw.as<float2>(), nmax, cooperative, ctx->stream);
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (const Failure& f) {
       std::lock_guard<std::mutex> lock(mu);
@@ -263,8 +285,11 @@ void fpmhip_server::run() {
     {
       std::lock_guard<std::mutex> lock(mu);
       stats.ticks++;
-      for (Job* j : active)
-        if (++j->done_iterations >= j->iterations) finished.push_back(j);
+      for (size_t i = 0; i < active.size(); ++i) {
+        Job* j = active[i];
+        j->done_iterations += quota[i];
+        if (j->done_iterations >= j->iterations) finished.push_back(j);
+      }
     }
     for (Job* j : finished) {
       try {

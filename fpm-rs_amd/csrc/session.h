@@ -48,6 +48,7 @@ struct SessionBuffers {
   DeviceBuffer flag;      // int[4]
   DeviceBuffer work_m;    // double2[M] (init / result scratch)
   DeviceBuffer avg, aw;   // double[N] (init)
+  DeviceBuffer group;     // cooperative-group state of the fused sweep (large MIXED_F32 tiles)
 };
 
 }  // namespace fpmhip

@@ -27,7 +27,8 @@ EXPORTS = [
     "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
     "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
-    "fpmhip_batch_timing", "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
+    "fpmhip_batch_timing", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
+    "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
     "fpmhip_server_poll", "fpmhip_server_wait", "fpmhip_server_result", "fpmhip_server_stats",
 ]
 
@@ -341,6 +342,38 @@ class Batch:
         mean, mx, n = ct.c_double(), ct.c_double(), ct.c_uint64()
         check(lib().fpmhip_batch_timing(self.handle, ct.byref(mean), ct.byref(mx), ct.byref(n)))
         return mean.value, mx.value, n.value
+
+    def schedule(self) -> dict:
+        """The sweep launch's schedule: group_members (workgroups per 256^2
+        tile), largest_first, radix (first-stage FFT radix for 32, 64, 128,
+        256 pixel tiles; 0 = default)."""
+        sch = BatchSchedule()
+        check(lib().fpmhip_batch_get_schedule(self.handle, ct.byref(sch)))
+        return {"group_members": sch.group_members, "largest_first": sch.largest_first, "radix": list(sch.radix)}
+
+    def set_schedule(self, group_members=None, largest_first=None, radix=None):
+        cur = self.schedule()
+        sch = BatchSchedule()
+        sch.group_members = cur["group_members"] if group_members is None else group_members
+        sch.largest_first = cur["largest_first"] if largest_first is None else int(largest_first)
+        for i, r in enumerate(cur["radix"] if radix is None else radix):
+            sch.radix[i] = r
+        check(lib().fpmhip_batch_set_schedule(self.handle, ct.byref(sch)))
+
+    def tune(self, trial_sweeps=2) -> dict:
+        """Autotune the schedule on this batch's tiles (fpmhip_batch_tune)."""
+        rep = BatchTuning()
+        check(lib().fpmhip_batch_tune(self.handle, ct.c_uint32(trial_sweeps), ct.byref(rep)))
+        return {"candidates": rep.candidates, "first_ms": rep.first_ms, "best_ms": rep.best_ms,
+                "schedule": self.schedule()}
+
+
+class BatchSchedule(ct.Structure):
+    _fields_ = [("group_members", ct.c_uint32), ("largest_first", ct.c_uint32), ("radix", ct.c_uint32 * 4)]
+
+
+class BatchTuning(ct.Structure):
+    _fields_ = [("candidates", ct.c_uint32), ("first_ms", ct.c_double), ("best_ms", ct.c_double)]
 
 
 class ServerConfig(ct.Structure):

@@ -288,7 +288,8 @@ int fpmhip_session_shape(const fpmhip_session* session, uint64_t* image_h, uint6
 /* ---- batched form: many independent tiles (each its own model, pupil and
  * measurements, uneven sizes) advanced together, one launch per sweep for
  * all tiles, captured in a HIP graph. Sessions must be MIXED_F32, sequential
- * schedule, and all use the same algorithm kind. ---- */
+ * schedule, square 32/64/128/256 images without multiplexing, and all use
+ * the same algorithm kind (AP, FPIE or mPIE). ---- */
 int fpmhip_batch_create(fpmhip_context* context, fpmhip_session* const* sessions, uint64_t count,
                         fpmhip_batch** out);
 int fpmhip_batch_destroy(fpmhip_batch* batch);
@@ -300,15 +301,43 @@ int fpmhip_batch_run(fpmhip_batch* batch, uint64_t iterations, double* trace);
 int fpmhip_batch_timing(const fpmhip_batch* batch, double* sweep_ms_mean, double* sweep_ms_max,
                         uint64_t* sweeps);
 
+/* Schedule of the batch's sweep launch (no counterpart in the reference; the
+ * reference runs one tile at a time on the host). */
+typedef struct fpmhip_batch_schedule {
+  uint32_t group_members; /* workgroups per 256^2 tile: 1, or 4 (a cooperative group; used only
+                             when the whole grid can be resident at once) */
+  uint32_t largest_first; /* 1: launch the largest tiles first; 0: session order */
+  uint32_t radix[4];      /* first-stage FFT radix for 32, 64, 128, 256 pixel tiles (0: default;
+                             32: 8|4, 64: 8|4|16, 128: 16|8, 256: 16) */
+} fpmhip_batch_schedule;
+int fpmhip_batch_get_schedule(const fpmhip_batch* batch, fpmhip_batch_schedule* out);
+/* Rebuilds the captured launch with `schedule`. */
+int fpmhip_batch_set_schedule(fpmhip_batch* batch, const fpmhip_batch_schedule* schedule);
+
+typedef struct fpmhip_batch_tuning {
+  uint32_t candidates; /* schedules timed */
+  double first_ms;     /* mean sweep time of the schedule in force before tuning */
+  double best_ms;      /* mean sweep time of the schedule chosen */
+} fpmhip_batch_tuning;
+/* Autotuner: times candidate schedules on the batch's own tiles (each from
+ * the tiles' initial state, one untimed sweep then `trial_sweeps` timed
+ * sweeps) by coordinate descent over the group size, the radix of each tile
+ * size present and the launch order, and keeps the fastest (a change must
+ * gain 2%). Tile state afterwards is that of the trials; fpmhip_batch_run
+ * re-initialises every tile. */
+int fpmhip_batch_tune(fpmhip_batch* batch, uint32_t trial_sweeps, fpmhip_batch_tuning* report);
+
 /* ---- serving form: continuous batching of reconstruction jobs ----
  * A worker thread owns the device. Submitted jobs wait in a queue and are
  * admitted while tile slots and the HBM budget allow; every admitted job is
  * one tile whose state stays resident until it finishes. Each tick is one
  * fused sweep launch over all resident tiles, each at its own iteration;
- * finished tiles leave and queued jobs enter between ticks. A job's result
- * is identical whether it ran alone or among others. Jobs must satisfy the
- * batched form's constraints (MIXED_F32, square 32/64/128 images, no
- * multiplexing, not EPRY). */
+ * a tile runs a quota of sweeps per tick inversely proportional to its
+ * per-sweep work (LEDs x pixels; at least one), so small tiles are not held
+ * to the pace of the largest resident one. Finished tiles leave and queued
+ * jobs enter between ticks. A job's result is identical whether it ran
+ * alone or among others. Jobs must satisfy the batched form's constraints
+ * (MIXED_F32, square 32/64/128/256 images, no multiplexing, not EPRY). */
 typedef struct fpmhip_server_config {
   uint64_t max_tiles;        /* resident tiles at most (workgroups per tick) */
   uint64_t hbm_budget_bytes; /* admission limit for resident job state */
@@ -316,7 +345,7 @@ typedef struct fpmhip_server_config {
 
 typedef struct fpmhip_server_statistics {
   uint64_t submitted, admitted, completed, failed;
-  uint64_t ticks;            /* sweep launches */
+  uint64_t ticks;            /* fused sweep launches */
   uint64_t resident_tiles;   /* at the time of the call */
   uint64_t resident_bytes;
   double busy_seconds;       /* wall time the worker spent with resident tiles */

@@ -146,3 +146,85 @@ def test_batch_256_tile_identical_alone_and_in_mix(ctx):
         alone = sessions_for(tiles[i:i + 1], alg)
         runtime.Batch(alone).run(2)
         assert np.array_equal(mix[i].spectrum(), alone[0].spectrum())
+
+
+def test_cooperative_group_matches_single_workgroup(ctx):
+    """A 256^2 tile split over a cooperative group of 4 workgroups gives the
+    same spectrum, bit for bit, as one workgroup running all chunks (the
+    per-pixel arithmetic and insert order are the same); the objective sums
+    differ only by summation order."""
+    alg = Mpie(iterations=3, momentum_interval=20, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(256, 9, 91), tile(128, 7, 92), tile(256, 7, 93)]
+    results = {}
+    for g in (4, 1):
+        ss = sessions_for(tiles, alg)
+        b = runtime.Batch(ss)
+        b.set_schedule(group_members=g)
+        assert b.schedule()["group_members"] == g
+        tr = b.run(3)
+        results[g] = ([s.spectrum() for s in ss], tr)
+        say("group", g, "sweep ms", b.timing()[0])
+    for a, c in zip(results[4][0], results[1][0]):
+        assert np.array_equal(a, c)
+    assert np.allclose(results[4][1], results[1][1], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("radix", [[4, 4, 8, 0], [8, 16, 16, 0]])
+def test_radix_plans_match_oracle(ctx, radix):
+    """Every first-stage radix plan of the LDS transforms stays within the
+    precision-form bound of the FP64 oracle."""
+    alg = Fpie(iterations=2)
+    tiles = [tile(32, 5, 101), tile(64, 7, 102), tile(128, 7, 103)]
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    b.set_schedule(radix=radix)
+    b.run(2)
+    for (model, meas), s in zip(tiles, ss):
+        ref = oracle.run(model, meas, alg.to_c(), 2)
+        ea, ep = amp_phase_err(s.object(), ref["object"])
+        say(radix, model.image_shape, "amp", ea, "phase", ep)
+        assert ea <= 1e-5 and ep <= 1e-5
+
+
+def test_launch_order_does_not_change_results(ctx):
+    alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(64, 5, 111), tile(256, 5, 112), tile(32, 5, 113)]
+    out = []
+    for first in (1, 0):
+        ss = sessions_for(tiles, alg)
+        b = runtime.Batch(ss)
+        b.set_schedule(largest_first=first)
+        b.run(2)
+        out.append([s.spectrum() for s in ss])
+    for a, c in zip(*out):
+        assert np.array_equal(a, c)
+
+
+def test_autotuner_keeps_a_valid_schedule(ctx):
+    """fpmhip_batch_tune times candidate schedules on the batch's own tiles,
+    keeps one, and the batch still reconstructs within the FP64 bound."""
+    alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(256, 7, 121), tile(128, 7, 122), tile(64, 7, 123), tile(32, 5, 124)]
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    rep = b.tune(1)
+    say("tuning", rep)
+    # group size, radix of 32/64/128 and launch order: 1 + 1 + 1 + 2 + 1 + 1 candidates
+    assert rep["candidates"] == 7
+    assert rep["best_ms"] <= rep["first_ms"]
+    sch = rep["schedule"]
+    assert sch["group_members"] in (1, 4) and sch["largest_first"] in (0, 1)
+    b.run(2)
+    for (model, meas), s in zip(tiles, ss):
+        ref = oracle.run(model, meas, alg.to_c(), 2)
+        ea, ep = amp_phase_err(s.object(), ref["object"])
+        assert ea <= 1e-5 and ep <= 1e-5
+
+
+def test_invalid_schedule_is_rejected(ctx):
+    ss = sessions_for([tile(64, 5, 131)], Fpie(iterations=1))
+    b = runtime.Batch(ss)
+    for kw in ({"group_members": 3}, {"radix": [0, 32, 0, 0]}, {"largest_first": 2}):
+        with pytest.raises(runtime.FpmError) as e:
+            b.set_schedule(**kw)
+        assert e.value.kind == "InvalidParameter"
