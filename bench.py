@@ -281,6 +281,8 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving", "uneven"])
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--sustained-rate", type=float, default=40.0,
+                    help="second serving leg below the GPU's capacity (the first leg's rate overloads it)")
     ap.add_argument("--no-extras", action="store_true", help="batched mode without the plain and serving legs")
     ap.add_argument("--tiles", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -294,9 +296,10 @@ def main():
         sys.exit(2)
     if args.mode == "serving":
         res = serving(args.requests, args.rate, 30)
+        sus = serving(args.requests, args.sustained_rate, 30)
         print(json.dumps({"metric": "serving p99 latency, FP32 mPIE 30 sweeps", "value": res["p99_ms"],
                           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
-                          "config": {"workload": "Poisson arrivals", **res}}), flush=True)
+                          "config": {"workload": "Poisson arrivals", **res}, "sustained": sus}), flush=True)
         return
     if args.mode == "uneven":
         res = run_uneven(40, args.steps, args.warmup)
@@ -354,6 +357,8 @@ def main():
         line["uneven_batch"] = run_uneven(40, 1, 1)
         log("serving leg")
         line["serving"] = serving(args.requests, args.rate, 30)
+        log("serving leg at a sustainable rate")
+        line["serving_sustained"] = serving(args.requests, args.sustained_rate, 30)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

@@ -96,15 +96,27 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
 }
 
 std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,
-                                  bool largest_first) {
+                                  bool largest_first, uint32_t grid_limit) {
   std::vector<TileDesc> sorted(tiles);
   if (largest_first)
     std::stable_sort(sorted.begin(), sorted.end(), [](const TileDesc& a, const TileDesc& b) { return a.n > b.n; });
+  // large tiles that become cooperative groups: as many as the resident-grid
+  // limit leaves room for, those with the most LEDs first
+  std::vector<uint32_t> big;
+  for (uint32_t i = 0; i < sorted.size(); ++i)
+    if (sorted[i].n > kFusedLdsMaxN && sorted[i].group) big.push_back(i);
+  std::vector<char> grouped(sorted.size(), 0);
+  if (members > 1 && !big.empty()) {
+    const uint64_t room = grid_limit > sorted.size() ? (grid_limit - sorted.size()) / (members - 1) : 0;
+    std::stable_sort(big.begin(), big.end(), [&](uint32_t x, uint32_t y) { return sorted[x].F > sorted[y].F; });
+    for (uint64_t k = 0; k < big.size() && k < room; ++k) grouped[big[k]] = 1;
+  }
   std::vector<TileDesc> out;
-  out.reserve(sorted.size() * members);
-  for (TileDesc t : sorted) {
+  out.reserve(sorted.size() + big.size() * (members - 1));
+  for (uint32_t i = 0; i < sorted.size(); ++i) {
+    TileDesc t = sorted[i];
     if (radix) t.plan = radix[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3];
-    const uint32_t g = (t.n > kFusedLdsMaxN && t.group) ? members : 1;
+    const uint32_t g = grouped[i] ? members : 1;
     for (uint32_t m = 0; m < g; ++m) {
       out.push_back(t);
       out.back().member = m;
@@ -112,17 +124,6 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
     }
   }
   return out;
-}
-
-uint32_t group_members_for(const std::vector<TileDesc>& tiles, uint32_t nmax, uint32_t requested) {
-  if (requested <= 1) return 1;
-  uint64_t big = 0;
-  for (const TileDesc& t : tiles)
-    if (t.n > kFusedLdsMaxN) ++big;
-  if (big == 0) return 1;
-  // cooperative groups need the whole grid resident at once
-  const uint64_t grid = tiles.size() + big * (requested - 1);
-  return grid <= sweep_coresident_limit(nmax) ? requested : 1;
 }
 
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream) {
@@ -209,8 +210,8 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
     fill_tile_desc(b.host_tiles[i], *b.sessions[i], b.traces.as<double>() + (size_t)i * trace_cap,
                    b.losses.as<double>() + (size_t)i * b.fmax, b.per_tile.as<char>() + (size_t)i * 16, trace_cap);
   prepare_sweep(b.nmax);  // the LDS attribute first: the residency query depends on it
-  b.device_order = order_tiles(b.host_tiles, group_members_for(b.host_tiles, b.nmax, b.schedule.group_members),
-                               b.schedule.radix, b.schedule.largest_first != 0);
+  b.device_order = order_tiles(b.host_tiles, b.schedule.group_members, b.schedule.radix,
+                               b.schedule.largest_first != 0, sweep_coresident_limit(b.nmax));
   b.launched = (uint32_t)b.device_order.size();
   b.cooperative = false;
   for (const TileDesc& t : b.device_order) b.cooperative |= t.members > 1;

@@ -60,20 +60,18 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
                     uint32_t trace_len);
 // Launch order of a sweep's tiles: largest first (stable), so the longest
 // workgroups are dispatched first and spread over the XCDs, or session
-// order; every tile above kFusedLdsMaxN becomes `members` descriptors (a
-// cooperative group, members 1 or kGroupMembers). Returns the expanded list.
-// radix[4]: first-stage radix per tile size 32, 64, 128, 256 (0: default;
-// null: keep the descriptors' plans); largest_first: else session order.
-std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members,
-                                  const uint32_t* radix = nullptr, bool largest_first = true);
+// order. Tiles above kFusedLdsMaxN become cooperative groups of `members`
+// descriptors (1 or kGroupMembers) as far as the whole grid stays within
+// `grid_limit` resident workgroups, tiles with more LEDs first; the others
+// run on one workgroup. radix[4]: first-stage radix per tile size 32, 64,
+// 128, 256 (0: default; null: keep the descriptors' plans).
+std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,
+                                  bool largest_first, uint32_t grid_limit);
 // First-stage radix choices of the fused sweep for an n x n tile; the first
 // entry is the default. Returns the number of choices (0: size unsupported).
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]);
 // Workgroups of a cooperative group for a 256^2 tile (one 64-line chunk each).
 constexpr uint32_t kGroupMembers = 4;
-// Group size to use for the large tiles of a launch: `requested` when the
-// expanded grid can be resident at once, else 1.
-uint32_t group_members_for(const std::vector<TileDesc>& tiles, uint32_t nmax, uint32_t requested);
 // Bytes of a session's cooperative-group state for an n x n tile.
 size_t group_state_bytes(uint32_t n);
 // Largest grid whose workgroups are all resident at once (one per CU at the
