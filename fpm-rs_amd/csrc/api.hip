@@ -653,6 +653,10 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
     }
     launch_convert_pupil(m->d_pupil64.as<double2>(), b.pupil.ptr, precision == FPMHIP_MIXED_F32, N, m->ctx->stream);
+    if (precision == FPMHIP_MIXED_F32) {
+      b.extent.allocate(m->nh * sizeof(uint32_t));
+      launch_pupil_extent(*s);
+    }
     launch_flag_reset(*s);
     HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
     *out = s.release();
@@ -679,6 +683,7 @@ int fpmhip_session_initialize(fpmhip_session* s) {
     launch_init(*s);
     launch_convert_pupil(s->model->d_pupil64.as<double2>(), s->b.pupil.ptr, s->precision == FPMHIP_MIXED_F32,
                          s->model->N(), s->ctx->stream);
+    launch_pupil_extent(*s);
     s->has_gain = s->model->has_gain;
     s->gain = s->model->gain;
     check_flags(*s);
@@ -700,6 +705,7 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
     const uint64_t M = s->model->M();
     HIP_CHECK(hipMemcpyAsync(s->b.spec.ptr, spectrum, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
     upload_pupil64(*s, pupil);
+    launch_pupil_extent(*s);
     if (frame_gains) {
       s->has_gain = true;
       s->gain.assign(frame_gains, frame_gains + s->model->F);

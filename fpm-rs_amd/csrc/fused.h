@@ -13,6 +13,7 @@ struct TileDesc {
   double2* vel;           // [R] mPIE velocity (or null)
   double2* anc;           // [R] mPIE anchor (or null)
   const float2* pupil;    // [N] complex64 pupil
+  const uint32_t* extent; // [n] per-row support of the pupil: columns [lo, hi) as lo | hi << 16
   float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)
   const void* frames;     // [F*N] uint16 counts or f64 frames
   const uint8_t* masks;   // [F*N] or null

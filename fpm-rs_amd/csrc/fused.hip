@@ -387,21 +387,28 @@ __device__ __forceinline__ void finish_sweep(const TileDesc& T, double obj_sum, 
   }
 }
 
+// Whether column c lies in a row's support extent (lo | hi << 16).
+__device__ __forceinline__ bool in_extent(uint32_t e, int c) {
+  return c >= (int)(e & 0xFFFFu) && c < (int)(e >> 16);
+}
+
 // Patch x pupil for STRIP vertically consecutive pixels of one patch column
 // (bilinear extract, crop.rs:129-167), handed to put(j, value). `off` is the
 // window offset of the first pixel's lower-left tap. Where the pupil is zero
 // the product is exactly zero, so the spectrum taps are not read at all;
-// vertically adjacent pixels share one row of taps. No read leaves the
-// (n+1)^2 window.
+// vertically adjacent pixels share one row of taps. Columns outside a row's
+// support extent `ext[j]` (row of pixel j) read nothing, not even the
+// pupil. No read leaves the (n+1)^2 window.
 template <int STRIP, class MEM = PlainMem, class Put>
 __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
-                                             int pstride, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du,
-                                             double wll, double wlu, double wul, double wuu, Put put) {
+                                             int pstride, const uint32_t* ext, int c, uint32_t off, uint32_t rw,
+                                             uint32_t dc, uint32_t du, double wll, double wlu, double wul, double wuu,
+                                             Put put) {
   double2 a = make_double2(0.0, 0.0), b = a;
   bool have = false;
 #pragma unroll
   for (int j = 0; j < STRIP; ++j) {
-    const float2 P = pcol[j * pstride];
+    const float2 P = in_extent(ext[j], c) ? pcol[j * pstride] : f2(0.0f, 0.0f);
     float2 out = f2(0.0f, 0.0f);
     if (P.x != 0.0f || P.y != 0.0f) {
       if (!have) {
@@ -426,10 +433,11 @@ __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, c
 // Integer-offset crop: the patch is a plain copy of the window (crop.rs:79-102).
 template <int STRIP, class MEM = PlainMem, class Put>
 __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
-                                           int pstride, uint32_t off, uint32_t rw, Put put) {
+                                           int pstride, const uint32_t* ext, int c, uint32_t off, uint32_t rw,
+                                           Put put) {
 #pragma unroll
   for (int j = 0; j < STRIP; ++j) {
-    const float2 P = pcol[j * pstride];
+    const float2 P = in_extent(ext[j], c) ? pcol[j * pstride] : f2(0.0f, 0.0f);
     float2 out = f2(0.0f, 0.0f);
     if (P.x != 0.0f || P.y != 0.0f) {
       const double2 v = MEM::ld(spec + off);
@@ -444,11 +452,27 @@ __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, con
 // destination rows [a0, a1) of the (n+1)^2 window: each destination pixel
 // sums its <= 4 weighted taps of U (fetched by u(row, col) in centred patch
 // coordinates) and is read-modified-written in FP64 only when that sum is
-// non-zero.
+// non-zero. U is zero outside the pupil's support extents `ext`, so pixels
+// beyond them are skipped without reading U.
 template <int N, int THREADS, class MEM = PlainMem, class GetU>
 __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const SrcParam& sp, uint32_t rw,
-                                              double scale, int tid, int a0, int a1, GetU u) {
+                                              double scale, int tid, int a0, int a1, const uint32_t* ext, GetU u) {
   const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
+  // destination columns that can receive a non-zero update: the union of the
+  // support extents of tap rows a - 1 and a, widened by one for a column tap
+  auto span = [&](int a, int& lo, int& hi) {
+    lo = N;
+    hi = 0;
+    if (a < N) {
+      lo = (int)(ext[a] & 0xFFFFu);
+      hi = (int)(ext[a] >> 16);
+    }
+    if (fr && a >= 1) {
+      lo = min(lo, (int)(ext[a - 1] & 0xFFFFu));
+      hi = max(hi, (int)(ext[a - 1] >> 16));
+    }
+    if (fc) hi += 1;
+  };
   const int ww = N + (fc ? 1 : 0);
   const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
               wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
@@ -471,7 +495,9 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const 
           ay = fmaf(v.y, wt, ay);
         }
       };
-      if (qd < total) {
+      int lo, hi;
+      span(a, lo, hi);
+      if (qd < total && b >= lo && b < hi) {
         if (fr && fc) {
           tap(a - 1, b - 1, wuu);
           tap(a - 1, b, wul);
@@ -509,7 +535,7 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const 
 // inside the owning thread, and vertically adjacent pixels share bilinear taps.
 template <int N, int RADIX>
 __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi,
-                                           double* red) {
+                                           double* red, const uint32_t* ext) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int LD = pitch<N>();
@@ -546,10 +572,11 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           const int cs = (cc + HALF) % N;
           auto put = [&](int j, float2 v) { lds[((rb + j + HALF) % N) * LD + cs] = v; };
           if (sp.zero) {
-            copy_strip<STRIP>(spec, pupil + rb * N + cc, N, ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw,
-                              put);
+            copy_strip<STRIP>(spec, pupil + rb * N + cc, N, ext + rb, cc,
+                              ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw, put);
           } else {
-            gather_strip<STRIP>(spec, pupil + rb * N + cc, N, (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw,
+            gather_strip<STRIP>(spec, pupil + rb * N + cc, N, ext + rb, cc,
+                                (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw,
                                 sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
                                 sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
           }
@@ -608,12 +635,14 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           const int r = r0 + j, r1 = r + HALF;
           const int l0 = r * LD + c, l1 = r1 * LD + c1;
           const float2 s0 = lds[l1], s1 = lds[l0];
-          const float2 P0 = pupil[r * N + c], P1 = pupil[r1 * N + c1];
+          // U is zero outside the support extents: no pupil read there
+          const bool in0 = in_extent(ext[r], c), in1 = in_extent(ext[r1], c1);
+          const float2 P0 = in0 ? pupil[r * N + c] : f2(0.0f, 0.0f), P1 = in1 ? pupil[r1 * N + c1] : f2(0.0f, 0.0f);
           const float k0 = inv_n / fmaf(den.b, fmaf(P0.x, P0.x, P0.y * P0.y), den.a);
           const float k1 = inv_n / fmaf(den.b, fmaf(P1.x, P1.x, P1.y * P1.y), den.a);
           const float2 u0 = fmulc(P0, s0), u1 = fmulc(P1, s1);
-          lds[l0] = f2(u0.x * k0, u0.y * k0);
-          lds[l1] = f2(u1.x * k1, u1.y * k1);
+          lds[l0] = in0 ? f2(u0.x * k0, u0.y * k0) : f2(0.0f, 0.0f);
+          lds[l1] = in1 ? f2(u1.x * k1, u1.y * k1) : f2(0.0f, 0.0f);
         }
       }
       __syncthreads();
@@ -630,7 +659,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           }
         }
       } else {
-        insert_window<N, kThreads>(spec, sp, T.rw, scale, tid, 0, N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0),
+        insert_window<N, kThreads>(spec, sp, T.rw, scale, tid, 0, N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0), ext,
                                    [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
       }
       __syncthreads();
@@ -759,7 +788,7 @@ __host__ __device__ constexpr uint32_t sweep_field_elems(uint32_t n) {
 
 template <int N, class MEM>
 __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, float2* prev, const float2* twf,
-                                               const float2* twi, double* red, int* bflag) {
+                                               const float2* twi, double* red, int* bflag, const uint32_t* ext) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int CH = BigPlan<N>::CH;
@@ -804,10 +833,11 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           const int cs = c ^ HALF;
           auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
           if (sp.zero) {
-            copy_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c,
-                                   T.rw, put);
+            copy_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, ext + rc, c,
+                                   ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw, put);
           } else {
-            gather_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
+            gather_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, ext + rc, c,
+                                     (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
                                      sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw,
                                      sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl,
                                      sp.ar.wu * sp.ac.wu, put);
@@ -896,10 +926,11 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         stage<N, CH, P1, P0, LDR, 1, false>(lds, twf, tid);
         for (int i = tid; i < CH * N; i += kThreads) {
           const int lr = i / N, kc = i % N;
-          const float2 P = pupil[(q * CH + lr) * N + (kc ^ HALF)];
+          const bool in = in_extent(ext[q * CH + lr], kc ^ HALF);  // U is zero outside the support
+          const float2 P = in ? pupil[(q * CH + lr) * N + (kc ^ HALF)] : f2(0.0f, 0.0f);
           const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
           const float2 u = fmulc(P, lds[lr * LDR + kc]);
-          lds[lr * LDR + kc] = f2(u.x * kk, u.y * kk);
+          lds[lr * LDR + kc] = in ? f2(u.x * kk, u.y * kk) : f2(0.0f, 0.0f);
         }
         __syncthreads();
         const int a0 = q * CH;
@@ -923,7 +954,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         } else {
           const int a1 = a0 + CH + ((q == NCH - 1 && sp.ar.upper != sp.ar.lower) ? 1 : 0);
           const float2* above = COOP ? T.edge + (q - 1) * N : prev;
-          insert_window<N, kThreads, MEM>(spec, sp, T.rw, scale, opaque_tid(), a0, a1, [&](int sr_, int sc_) {
+          insert_window<N, kThreads, MEM>(spec, sp, T.rw, scale, opaque_tid(), a0, a1, ext, [&](int sr_, int sc_) {
             return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)]
                              : (COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF]);
           });
@@ -956,41 +987,43 @@ __global__ void __launch_bounds__(kThreads)
   const TileDesc& T = tiles[blockIdx.x];  // fields are read (scalar loads) where used
   const int N = (int)T.n;
   // LDS: field (or chunk) region, [big tiles: previous row of U], twiddles
-  // W_N^j forward then inverse, reduction slots, barrier flag
+  // W_N^j forward then inverse, reduction slots, barrier flag, pupil extents
   float2* prev = lds + sweep_field_elems(N);
   float2* twf = prev + (N > (int)kFusedLdsMaxN ? N : 0);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
   int* bflag = reinterpret_cast<int*>(red + 16);
+  uint32_t* ext = reinterpret_cast<uint32_t*>(bflag + 4);
   const float2* src = tw_all + tw_offset(N);
   for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) twf[j] = src[j];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) ext[j] = T.extent[j];
   __syncthreads();
   switch (N) {
     case 32:
       if (T.plan == 4)
-        sweep_tile<32, 4>(T, lds, twf, twi, red);
+        sweep_tile<32, 4>(T, lds, twf, twi, red, ext);
       else
-        sweep_tile<32, 8>(T, lds, twf, twi, red);
+        sweep_tile<32, 8>(T, lds, twf, twi, red, ext);
       break;
     case 64:
       if (T.plan == 4)
-        sweep_tile<64, 4>(T, lds, twf, twi, red);
+        sweep_tile<64, 4>(T, lds, twf, twi, red, ext);
       else if (T.plan == 16)
-        sweep_tile<64, 16>(T, lds, twf, twi, red);
+        sweep_tile<64, 16>(T, lds, twf, twi, red, ext);
       else
-        sweep_tile<64, 8>(T, lds, twf, twi, red);
+        sweep_tile<64, 8>(T, lds, twf, twi, red, ext);
       break;
     case 128:
       if (T.plan == 8)
-        sweep_tile<128, 8>(T, lds, twf, twi, red);
+        sweep_tile<128, 8>(T, lds, twf, twi, red, ext);
       else
-        sweep_tile<128, 16>(T, lds, twf, twi, red);
+        sweep_tile<128, 16>(T, lds, twf, twi, red, ext);
       break;
     case 256:
       if (T.members > 1)
-        sweep_tile_big<256, CoherentMem>(T, lds, prev, twf, twi, red, bflag);
+        sweep_tile_big<256, CoherentMem>(T, lds, prev, twf, twi, red, bflag, ext);
       else
-        sweep_tile_big<256, PlainMem>(T, lds, prev, twf, twi, red, bflag);
+        sweep_tile_big<256, PlainMem>(T, lds, prev, twf, twi, red, bflag, ext);
       break;
     default:
       break;
@@ -1001,7 +1034,7 @@ __global__ void __launch_bounds__(kThreads)
 
 size_t sweep_lds_bytes(uint32_t nmax) {
   return ((size_t)sweep_field_elems(nmax) + (nmax > kFusedLdsMaxN ? nmax : 0) + 2 * (size_t)nmax) * sizeof(float2) +
-         16 * sizeof(double) + 16;
+         16 * sizeof(double) + 16 + (size_t)nmax * sizeof(uint32_t);
 }
 
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {

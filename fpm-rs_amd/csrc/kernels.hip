@@ -529,6 +529,31 @@ void launch_convert_pupil(const double2* src, void* dst, bool to_f32, uint64_t n
   HIP_CHECK(hipGetLastError());
 }
 
+// Per-row support of a complex64 pupil: extent[r] = lo | hi << 16, where
+// columns [lo, hi) hold every non-zero value of row r (lo = w, hi = 0 for a
+// row of zeros).
+__global__ void k_pupil_extent(const float2* __restrict__ pupil, uint32_t h, uint32_t w, uint32_t* __restrict__ extent) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= h) return;
+  uint32_t lo = w, hi = 0;
+  for (uint32_t c = 0; c < w; ++c) {
+    const float2 p = pupil[(uint64_t)r * w + c];
+    if (p.x != 0.0f || p.y != 0.0f) {
+      lo = c < lo ? c : lo;
+      hi = c + 1;
+    }
+  }
+  extent[r] = lo | (hi << 16);
+}
+
+void launch_pupil_extent(const fpmhip_session& s) {
+  const Model& m = *s.model;
+  if (s.precision != FPMHIP_MIXED_F32 || !s.b.extent.ptr) return;
+  hipLaunchKernelGGL(k_pupil_extent, dim3((m.nh + 63) / 64), dim3(64), 0, s.ctx->stream, s.b.pupil.as<float2>(),
+                     (uint32_t)m.nh, (uint32_t)m.nw, s.b.extent.as<uint32_t>());
+  HIP_CHECK(hipGetLastError());
+}
+
 void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream) {
   if (from_f32)
     hipLaunchKernelGGL(k_convert_to_d2<float2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream,

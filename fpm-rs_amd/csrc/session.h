@@ -49,6 +49,7 @@ struct SessionBuffers {
   DeviceBuffer work_m;    // double2[M] (init / result scratch)
   DeviceBuffer avg, aw;   // double[N] (init)
   DeviceBuffer group;     // cooperative-group state of the fused sweep (large MIXED_F32 tiles)
+  DeviceBuffer extent;    // uint32[n_h] per-row support of the complex64 pupil (MIXED_F32)
 };
 
 }  // namespace fpmhip
@@ -89,6 +90,8 @@ void launch_object(const fpmhip_session& s, double2* out_dev);
 void launch_shift(const double2* src, double2* dst, uint32_t h, uint32_t w, uint32_t rs, uint32_t cs,
                   hipStream_t stream);
 void launch_convert_pupil(const double2* src, void* dst, bool to_f32, uint64_t n, hipStream_t stream);
+// Per-row support extents of a MIXED_F32 session's pupil (for the fused sweep).
+void launch_pupil_extent(const fpmhip_session& s);
 void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream);
 void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream);
 void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields,
