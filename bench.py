@@ -286,7 +286,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="batched mode without the plain and serving legs")
     ap.add_argument("--tiles", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample-iterations", type=int, default=4)
+    ap.add_argument("--cpu-sample-iterations", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -277,6 +277,76 @@ std::vector<uint64_t> schedule_order(const fpmhip_session& s, int schedule) {
   return order;
 }
 
+// Frame gains and background of the reconstruction state, as initialised
+// from the model (state.rs:280-355): host gains, device gains, device
+// background (length 0, N or F * N).
+void load_calibration(fpmhip_session& s) {
+  const Model& m = *s.model;
+  s.has_gain = m.has_gain;
+  s.gain = m.gain;
+  std::vector<double> g(m.F, 1.0);
+  if (m.has_gain) g = m.gain;
+  if (m.F)
+    HIP_CHECK(hipMemcpyAsync(s.b.gains.ptr, g.data(), m.F * sizeof(double), hipMemcpyHostToDevice, s.ctx->stream));
+  s.bg_len = m.background.size();
+  if (s.bg_len)
+    HIP_CHECK(hipMemcpyAsync(s.b.bg.ptr, m.d_background.ptr, s.bg_len * sizeof(double), hipMemcpyDeviceToDevice,
+                             s.ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+}
+
+// Sum of the calibration partials of the last launch_calibration_sums, in
+// block order.
+std::pair<double, double> calibration_totals(fpmhip_session& s) {
+  std::vector<double> part((size_t)s.nblk * 2);
+  HIP_CHECK(hipMemcpyAsync(part.data(), s.b.calib.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           s.ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+  double a = 0.0, b = 0.0;
+  for (uint32_t k = 0; k < s.nblk; ++k) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  return {a, b};
+}
+
+double clamp_nan(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// update_gain then update_background for one frame (common.rs:144-168,
+// 390-489), from the frame's intrinsic prediction; `gain` is the frame's
+// gain in and out.
+void calibrate(fpmhip_session& s, uint64_t f, uint32_t K, const double* mode_w_dev, double& gain) {
+  const Model& m = *s.model;
+  if (s.cfg.gain_update) {
+    launch_calibration_sums(s, f, K, mode_w_dev, gain, 0);
+    const auto [num, den] = calibration_totals(s);
+    if (den > s.cfg.epsilon) {
+      const double est = clamp_nan(num / den, s.cfg.gain_min, s.cfg.gain_max);
+      gain = clamp_nan((1.0 - s.cfg.gain_step) * gain + s.cfg.gain_step * est, s.cfg.gain_min, s.cfg.gain_max);
+      if (!s.has_gain) {
+        s.has_gain = true;
+        s.gain.assign(m.F, 1.0);
+      }
+      s.gain[f] = gain;
+      HIP_CHECK(hipMemcpyAsync(s.b.gains.as<double>() + f, &s.gain[f], sizeof(double), hipMemcpyHostToDevice,
+                               s.ctx->stream));
+    }
+  }
+  if (s.cfg.bg_update) {
+    launch_calibration_sums(s, f, K, mode_w_dev, gain, 1);
+    const auto [rsum, rcount] = calibration_totals(s);
+    if (rcount > 0.0) {
+      const double correction = s.cfg.bg_step * rsum / rcount;
+      const uint64_t total = m.N() * m.F;
+      if (s.bg_len != total) {
+        launch_background_expand(s);
+        s.bg_len = total;
+      }
+      launch_background_shift(s, f, correction, s.cfg.bg_min, s.cfg.bg_max);
+    }
+  }
+}
+
 // One iteration of projection_update over `order` (common.rs:57-306), frame
 // by frame on the session stream; objectives reduced on the host afterwards.
 void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_frame, double* objective) {
@@ -312,7 +382,10 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
       fft2_f64(*m.fft_low, s.b.field.as<double2>(), K, true, s.ctx->stream);
     else
       fft2_f32(*m.fft_low, s.b.field.as<float2>(), K, true, s.ctx->stream);
-    launch_project(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, sw_sum, gain, i);
+    double frame_gain = gain;
+    if (s.cfg.gain_update || s.cfg.bg_update)
+      calibrate(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, frame_gain);
+    launch_project(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, sw_sum, frame_gain, i);
     ran[i] = true;
     for (uint32_t k = 0; k < K; ++k) {
       const uint64_t src = m.mux ? m.msrc[e0 + k] : f;
@@ -610,8 +683,6 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     if (precision != FPMHIP_PLAIN_C128 && precision != FPMHIP_MIXED_F32)
       fail(FPMHIP_INVALID_PARAMETER, "unknown precision", "precision");
     if (int rc = fpmhip_algorithm_validate(alg)) throw Failure{rc};
-    if (alg->kind == FPMHIP_EPRY && (alg->recover_gain || alg->recover_background))
-      fail(FPMHIP_UNSUPPORTED, "EPRY gain/background recovery is not implemented on the device path");
     set_device(m->ctx);
     auto s = std::make_unique<fpmhip_session>();
     s->ctx = m->ctx;
@@ -648,6 +719,11 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     b.work_m.allocate(std::max(M, N) * sizeof(double2));
     b.avg.allocate(N * sizeof(double));
     b.aw.allocate(N * sizeof(double));
+    // background of the state: per frame once recovery may write it
+    b.bg.allocate(std::max<uint64_t>(s->cfg.bg_update ? N * m->F : m->background.size(), 1) * sizeof(double));
+    b.calib.allocate((size_t)s->nblk * 2 * sizeof(double));
+    b.gains.allocate(std::max<uint64_t>(m->F, 1) * sizeof(double));
+    load_calibration(*s);
     if (precision == FPMHIP_MIXED_F32 && m->nh == m->nw && m->nh > kFusedLdsMaxN && m->nh <= kFusedMaxN) {
       b.group.allocate(group_state_bytes(m->nh));
       HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
@@ -684,8 +760,7 @@ int fpmhip_session_initialize(fpmhip_session* s) {
     launch_convert_pupil(s->model->d_pupil64.as<double2>(), s->b.pupil.ptr, s->precision == FPMHIP_MIXED_F32,
                          s->model->N(), s->ctx->stream);
     launch_pupil_extent(*s);
-    s->has_gain = s->model->has_gain;
-    s->gain = s->model->gain;
+    load_calibration(*s);
     check_flags(*s);
     init_aux(*s);
     refresh_max_pupil(*s);
@@ -700,15 +775,33 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
                              const double* background, uint64_t background_len) {
   return guarded([&] {
     require(s && spectrum && pupil, "state", "null argument");
-    if (background) fail(FPMHIP_UNSUPPORTED, "a restored background must equal the model background on the device path");
+    if (background) {
+      const uint64_t N = s->model->N(), F = s->model->F;
+      if (background_len != N && background_len != N * F)
+        fail(FPMHIP_INVALID_SHAPE, "background must hold one image or one image per frame");
+    }
     set_device(s->ctx);
     const uint64_t M = s->model->M();
     HIP_CHECK(hipMemcpyAsync(s->b.spec.ptr, spectrum, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
     upload_pupil64(*s, pupil);
     launch_pupil_extent(*s);
     if (frame_gains) {
+      for (uint64_t f = 0; f < s->model->F; ++f)
+        if (!std::isfinite(frame_gains[f]) || frame_gains[f] <= 0.0)
+          fail(FPMHIP_INVALID_MODEL, "frame gains must be finite and positive");
       s->has_gain = true;
       s->gain.assign(frame_gains, frame_gains + s->model->F);
+      HIP_CHECK(hipMemcpyAsync(s->b.gains.ptr, s->gain.data(), s->gain.size() * sizeof(double), hipMemcpyHostToDevice,
+                               s->ctx->stream));
+    }
+    if (background) {
+      if (s->b.bg.bytes < background_len * sizeof(double)) {
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        s->b.bg.allocate(background_len * sizeof(double));
+      }
+      HIP_CHECK(hipMemcpyAsync(s->b.bg.ptr, background, background_len * sizeof(double), hipMemcpyHostToDevice,
+                               s->ctx->stream));
+      s->bg_len = background_len;
     }
     if (s->cfg.momentum) {
       if (velocity && anchor) {
@@ -721,7 +814,6 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
         init_aux(*s);
       }
     }
-    (void)background_len;
     refresh_max_pupil(*s);
     if (s->cfg.gauge) canonicalize_session(*s);
     HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
@@ -821,8 +913,13 @@ int fpmhip_session_get_calibration(fpmhip_session* s, double* frame_gains, doubl
     const Model& m = *s->model;
     if (frame_gains)
       for (uint64_t f = 0; f < m.F; ++f) frame_gains[f] = s->has_gain ? s->gain[f] : 1.0;
-    if (background) std::copy(m.background.begin(), m.background.end(), background);
-    if (background_len) *background_len = m.background.size();
+    set_device(s->ctx);
+    if (background && s->bg_len) {
+      HIP_CHECK(hipMemcpyAsync(background, s->b.bg.ptr, s->bg_len * sizeof(double), hipMemcpyDeviceToHost,
+                               s->ctx->stream));
+      HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    }
+    if (background_len) *background_len = s->bg_len;
   });
 }
 

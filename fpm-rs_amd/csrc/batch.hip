@@ -64,9 +64,9 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.frames = me.d_frames.ptr;
   t.masks = me.d_masks.as<uint8_t>();
   t.weights = me.d_weights.as<double>();
-  t.gains = m.d_gains.as<double>();
-  t.bg = m.d_background.as<double>();
-  t.bg_len = m.background.size();
+  t.gains = s.b.gains.as<double>();
+  t.bg = s.b.bg.as<double>();
+  t.bg_len = s.bg_len;
   t.src = m.d_src.as<SrcParam>();
   t.order = nullptr;
   t.trace = trace;

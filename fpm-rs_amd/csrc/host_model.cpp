@@ -83,6 +83,14 @@ Config make_config(const fpmhip_algorithm& a) {
       c.pupil_step = a.pupil_step;
       c.constrain = a.constrain_pupil != 0;
       c.gauge = a.recover_pupil != 0;
+      c.gain_update = a.recover_gain != 0;  // GainUpdateConfiguration (common.rs:390-422)
+      c.gain_step = a.gain_step;
+      c.gain_min = a.minimum_gain;
+      c.gain_max = a.maximum_gain;
+      c.bg_update = a.recover_background != 0;  // BackgroundUpdateConfiguration (common.rs:425-489)
+      c.bg_step = a.background_step;
+      c.bg_min = a.minimum_background;
+      c.bg_max = a.maximum_background;
       break;
     default:
       fail(FPMHIP_INVALID_PARAMETER, "unknown algorithm kind", "kind");

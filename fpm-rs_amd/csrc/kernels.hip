@@ -139,6 +139,55 @@ __global__ void __launch_bounds__(kBlock) k_project(V* __restrict__ fields, uint
   }
 }
 
+// Pixel sums of the calibration updates over the unmasked pixels of one
+// frame, from the intrinsic prediction sum_k w_k |psi_k|^2 of the current
+// fields (common.rs:390-460): mode 0 (update_gain) sums pred * max(y - bg, 0)
+// and pred^2; mode 1 (update_background) sums y - gain * pred - bg and the
+// pixel count. One partial pair per block, summed on the host in block order.
+template <class V>
+__global__ void __launch_bounds__(kBlock)
+    k_calibration_sums(const V* __restrict__ fields, uint32_t K, const double* __restrict__ mode_w, uint32_t N,
+                       const void* __restrict__ frames, int is_u16, uint64_t frame, const uint8_t* __restrict__ masks,
+                       const double* __restrict__ bg, uint64_t bg_len, double gain, int mode,
+                       double* __restrict__ partials) {
+  __shared__ double sh[kBlock];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (p < N && !(masks && masks[frame * N + p] == 0)) {
+    double pred = 0.0;
+    for (uint32_t k = 0; k < K; ++k) pred += (mode_w ? mode_w[k] : 1.0) * D::norm2(to_d2(fields[(uint64_t)k * N + p]));
+    const double bgv = bg_value(bg, bg_len, N, frame, p);
+    const double y = meas_value(frames, is_u16, frame * N + p);
+    if (mode == 0) {
+      a = pred * fmax(y - bgv, 0.0);
+      b = pred * pred;
+    } else {
+      a = y - gain * pred - bgv;
+      b = 1.0;
+    }
+  }
+  const double sa = block_sum<kBlock>(a, sh);
+  const double sb = block_sum<kBlock>(b, sh);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sa;
+    partials[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+// clamp as f64::clamp: NaN stays NaN
+__device__ inline double clamp_nan(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__global__ void k_background_shift(double* __restrict__ bg, uint64_t N, double correction, double lo, double hi) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < N) bg[p] = clamp_nan(bg[p] + correction, lo, hi);
+}
+
+// in place: entries [0, N) hold the shared background (or nothing)
+__global__ void k_background_expand(double* __restrict__ bg, uint64_t src_len, uint64_t N, uint64_t total) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total && i >= src_len) bg[i] = src_len == 0 ? 0.0 : bg[i % N];
+}
+
 // difference field of a further multiplexed mode (common.rs:219-228)
 template <class V>
 __global__ void k_difference(V* __restrict__ field, const V* __restrict__ proj, const double* __restrict__ pred,
@@ -431,9 +480,34 @@ void launch_project(const fpmhip_session& s, uint64_t frame, uint32_t K, const d
   const uint64_t N = m.N();
   DISPATCH_V(s, hipLaunchKernelGGL(k_project<V>, dim3(s.nblk), dim3(kBlock), 0, s.ctx->stream, s.b.field.as<V>(),
                                    K, mode_w_dev, sw_sum, (uint32_t)N, me.d_frames.ptr, me.is_u16 ? 1 : 0, frame,
-                                   me.d_masks.as<uint8_t>(), gain, m.d_background.as<double>(),
-                                   (uint64_t)m.background.size(), s.cfg.epsilon, s.cfg.loss, s.b.pred.as<double>(),
+                                   me.d_masks.as<uint8_t>(), gain, s.b.bg.as<double>(), s.bg_len, s.cfg.epsilon,
+                                   s.cfg.loss, s.b.pred.as<double>(),
                                    s.b.proj.as<V>(), s.b.partials.as<double>() + slot * s.nblk * 2));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_calibration_sums(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
+                             double gain, int mode) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  DISPATCH_V(s, hipLaunchKernelGGL(k_calibration_sums<V>, dim3(s.nblk), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.field.as<V>(), K, mode_w_dev, (uint32_t)m.N(), me.d_frames.ptr,
+                                   me.is_u16 ? 1 : 0, frame, me.d_masks.as<uint8_t>(), s.b.bg.as<double>(),
+                                   s.bg_len, gain, mode, s.b.calib.as<double>()));
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_background_shift(const fpmhip_session& s, uint64_t frame, double correction, double lo, double hi) {
+  const uint64_t N = s.model->N();
+  hipLaunchKernelGGL(k_background_shift, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
+                     s.b.bg.as<double>() + frame * N, N, correction, lo, hi);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_background_expand(const fpmhip_session& s) {
+  const uint64_t N = s.model->N(), total = N * s.model->F;
+  hipLaunchKernelGGL(k_background_expand, dim3(blocks_for(total)), dim3(kBlock), 0, s.ctx->stream,
+                     s.b.bg.as<double>(), s.bg_len, N, total);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -30,6 +30,9 @@ struct Config {
   uint64_t interval = 0;
   double friction = 0.0, feedback = 0.0;
   bool gauge = false;  // canonicalize_state after init and each iteration (EPRY)
+  bool gain_update = false, bg_update = false;  // EPRY frame-gain / background recovery
+  double gain_step = 0.0, gain_min = 0.0, gain_max = 0.0;
+  double bg_step = 0.0, bg_min = 0.0, bg_max = 0.0;
 };
 Config make_config(const fpmhip_algorithm& a);
 
@@ -50,6 +53,9 @@ struct SessionBuffers {
   DeviceBuffer avg, aw;   // double[N] (init)
   DeviceBuffer group;     // cooperative-group state of the fused sweep (large MIXED_F32 tiles)
   DeviceBuffer extent;    // uint32[n_h] per-row support of the complex64 pupil (MIXED_F32)
+  DeviceBuffer bg;        // double[F * N] background of the reconstruction state (bg_len used)
+  DeviceBuffer gains;     // double[F] frame gains of the state (1 where the model has none)
+  DeviceBuffer calib;     // double[nblk * 2] block partial sums of the calibration updates
 };
 
 }  // namespace fpmhip
@@ -67,6 +73,7 @@ struct fpmhip_session {
   uint32_t nblk = 0;   // blocks of the projection kernel
   bool has_gain = false;
   std::vector<double> gain;
+  uint64_t bg_len = 0;  // background of the state: 0 (none), N (shared) or F * N (per frame)
   fpmhip::SessionBuffers b;
   size_t vbytes() const { return precision == FPMHIP_PLAIN_C128 ? sizeof(double2) : sizeof(float2); }
 };
@@ -78,6 +85,15 @@ namespace fpmhip {
 void launch_extract(const fpmhip_session& s, uint64_t source, uint32_t mode);
 void launch_project(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
                     double sw_sum, double gain, uint64_t slot);
+// Block partial sums over the unmasked pixels of `frame` from the current
+// fields: mode 0 (update_gain): sum pred * max(y - bg, 0), sum pred^2;
+// mode 1 (update_background): sum (y - gain * pred - bg), pixel count.
+void launch_calibration_sums(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
+                             double gain, int mode);
+// Background state of the session: bg[frame * N + p] = clamp(bg + correction, lo, hi).
+void launch_background_shift(const fpmhip_session& s, uint64_t frame, double correction, double lo, double hi);
+// Expands a shared (length N) or absent background state to per-frame F * N.
+void launch_background_expand(const fpmhip_session& s);
 void launch_difference_frame(const fpmhip_session& s, uint32_t mode, uint64_t frame);
 void launch_update(const fpmhip_session& s, uint32_t mode, double nsw, bool first_mode);
 void launch_insert(const fpmhip_session& s, uint64_t source, uint32_t mode, double scale);

@@ -226,7 +226,9 @@ class Session:
     def initialize(self):
         check(lib().fpmhip_session_initialize(self.handle))
 
-    def set_state(self, spectrum, pupil, velocity=None, anchor=None, counter=0, frame_gains=None):
+    def set_state(self, spectrum, pupil, velocity=None, anchor=None, counter=0, frame_gains=None, background=None):
+        """Checkpoint restore (ReconstructionState::from_checkpoint, state.rs:398-419);
+        background: one image or one per frame."""
         keep = []
 
         def c(a, dt=np.complex128):
@@ -236,9 +238,10 @@ class Session:
             keep.append(a)
             return _cd(a)
 
+        bg = None if background is None else np.ascontiguousarray(background, dtype=np.float64).ravel()
         check(lib().fpmhip_session_set_state(self.handle, c(spectrum), c(pupil), c(velocity), c(anchor),
-                                             ct.c_uint64(counter), c(frame_gains, np.float64), None,
-                                             ct.c_uint64(0)))
+                                             ct.c_uint64(counter), c(frame_gains, np.float64), c(bg, np.float64),
+                                             ct.c_uint64(0 if bg is None else bg.size)))
 
     def sweep(self, order=None):
         per = np.zeros(self.F if order is None else len(order), np.float64)
@@ -275,6 +278,18 @@ class Session:
         check(lib().fpmhip_session_get_object(self.handle, _cd(out)))
         return out
 
+    def calibration(self):
+        """Frame gains (F,) and background of the state: None, one image
+        (N,) or one per frame (F, N) (EPRY recovers both)."""
+        gains = np.zeros(self.F, np.float64)
+        n_img = self.N[0] * self.N[1]
+        bg = np.zeros(self.F * n_img, np.float64)
+        n = ct.c_uint64(0)
+        check(lib().fpmhip_session_get_calibration(self.handle, _cd(gains), _cd(bg), ct.byref(n)))
+        if n.value == 0:
+            return gains, None
+        return gains, (bg[:n_img].reshape(self.N) if n.value == n_img else bg.reshape((self.F,) + self.N))
+
     def momentum(self):
         v = np.zeros(self.R, np.complex128)
         a = np.zeros(self.R, np.complex128)
@@ -309,6 +324,9 @@ def run_algorithm(algorithm, problem: ReconstructionProblem, precision="c128", s
     if alg.kind == abi.MPIE:
         v, a, c = s.momentum()
         res.extra.update(velocity=v, anchor=a, counter=c)
+    if alg.kind == abi.EPRY and (alg.recover_gain or alg.recover_background):
+        gains, bg = s.calibration()
+        res.extra.update(recovered_frame_gains=gains, recovered_background=bg)
     return res
 
 

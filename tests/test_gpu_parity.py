@@ -226,3 +226,78 @@ def test_mixed_f32_per_stage_matches_fp64_oracle(ctx, n):
     ep = rel(np.angle(got["object"] * np.conj(ref["object"])) + np.angle(ref["object"]), np.angle(ref["object"]))
     say("mixed amplitude relL2", ea, "phase relL2", ep)
     assert ea <= 1e-5 and ep <= 1e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-10), ("f32", 1e-5)])
+def test_epry_gain_and_background_recovery_matches_oracle(ctx, precision, tol):
+    """EPRY frame-gain and background recovery (common.rs:144-168, 390-489) on
+    the device against the oracle: 81 LEDs, 64^2, per-frame gains and a
+    per-frame background in the data, both recovered from a model without
+    them; spectrum, pupil, gains and background."""
+    model = runtime.compile_model(syn.base_optics(), syn.planar_array(9), (64, 64))
+    obj = syn.random_field_object(model.reconstruction_shape, 3)
+    F, N = model.frame_count, model.image_len
+    rng = np.random.default_rng(3)
+    true = runtime.compile_model(syn.base_optics(), syn.planar_array(9), (64, 64))
+    true.with_frame_gains(rng.uniform(0.7, 1.3, F) * 50.0)
+    true.with_background(np.repeat(rng.uniform(0.0, 4.0, F), N))
+    I = oracle.forward_stack(true, syn.object_spectrum(obj))
+    model.with_frame_gains(np.full(F, 50.0))
+    meas = MeasurementStack(frames=I)
+    alg = Epry(iterations=3, recover_frame_gains=True, gain_step=0.5, recover_background=True,
+               background_step=0.5, minimum_background=-10.0, maximum_background=10.0)
+    ref = oracle.run(model, meas, alg.to_c(), 3)
+    s = runtime.Session(ReconstructionProblem(meas, model), alg, precision=precision)
+    s.initialize()
+    s.run(3)
+    gains, bg = s.calibration()
+    for got, key in ((s.spectrum(), "object_spectrum"), (s.pupil(), "recovered_pupil"), (gains, "frame_gains"),
+                     (bg, "background")):
+        e = rel(np.asarray(got).ravel(), ref[key].ravel())
+        say(precision, key, e)
+        assert e <= tol, (key, e)
+    assert bg.shape == (F,) + tuple(model.image_shape)
+
+
+def test_epry_recovery_reference_cases_on_device(ctx):
+    """The reference's gain (:3037) and background (:3081) recovery cases,
+    run through run_algorithm on the device."""
+    true_gains = np.array([0.5, 1.0, 1.5, 2.0, 0.75])
+    meas = MeasurementStack(frames=oracle.forward_stack(ro.direct_model().with_frame_gains(true_gains),
+                                                        ro.spectrum_of(np.full((16, 16), 2.0 + 0j))))
+    r = Epry(iterations=2, object_step=1e-6, recover_pupil=False, recover_frame_gains=True,
+             gain_step=1.0).run(ReconstructionProblem(meas, ro.direct_model()))
+    g = r.extra["recovered_frame_gains"]
+    a = np.dot(g, true_gains) / np.dot(g, g)
+    err = math.sqrt(np.sum((a * g - true_gains) ** 2) / np.sum(true_gains ** 2))
+    init = math.sqrt(np.sum((true_gains.mean() - true_gains) ** 2) / np.sum(true_gains ** 2))
+    assert err < 0.1 * init
+    true_bg = np.array([0.0, 1.0, 2.0, 3.0, 0.5])
+    meas = MeasurementStack(frames=oracle.forward_stack(ro.direct_model().with_background(np.repeat(true_bg, 64)),
+                                                        ro.spectrum_of(np.full((16, 16), 2.0 + 0j))))
+    r = Epry(iterations=1, object_step=1e-6, recover_pupil=False, recover_background=True, background_step=1.0,
+             minimum_background=-10.0, maximum_background=10.0).run(ReconstructionProblem(meas, ro.direct_model()))
+    means = r.extra["recovered_background"].reshape(5, 64).mean(axis=1)
+    assert np.allclose(means - means.mean(), true_bg - true_bg.mean(), atol=1e-8, rtol=0)
+
+
+def test_state_background_and_gains_restore(ctx):
+    """set_state with per-frame gains and background: the device continues
+    from them like the oracle's resume (FPIE, 2 sweeps, 1e-10)."""
+    model, meas = _direct_problem(ro.mixed_test_pattern((16, 16)))
+    F = model.frame_count
+    rng = np.random.default_rng(5)
+    gains = rng.uniform(0.8, 1.2, F)
+    bg = rng.uniform(0.0, 0.1, (F, 8, 8))
+    spec = ro.spectrum_of(ro.mixed_test_pattern((16, 16)))
+    alg = Fpie(iterations=2)
+    ref = oracle.run(model, meas, alg.to_c(), 2, resume={"spectrum": spec, "pupil": model.pupil,
+                                                          "frame_gains": gains, "background": bg.ravel()})
+    s = runtime.Session(ReconstructionProblem(meas, model), alg, precision="c128")
+    s.set_state(spec, model.pupil, frame_gains=gains, background=bg)
+    s.run(2)
+    e = rel(s.spectrum(), ref["object_spectrum"])
+    say("restored calibration", e)
+    assert e <= 1e-10
+    g2, b2 = s.calibration()
+    assert np.array_equal(g2, gains) and np.array_equal(b2, bg)

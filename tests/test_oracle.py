@@ -385,3 +385,41 @@ def test_epry_amplifies_initial_perturbations():
     ea = np.linalg.norm(np.abs(moved) - np.abs(base)) / np.linalg.norm(np.abs(base))
     ep = np.linalg.norm(np.angle(moved * np.conj(base))) / np.linalg.norm(np.angle(base))
     assert ea > 1e-6 and ep > 2e-6
+
+
+def _aligned_gain_error(reference, candidate):
+    """compare_frame_gains (src/metrics/model.rs:130-161): relative error after
+    the least-squares scale of the candidate."""
+    reference, candidate = np.asarray(reference), np.asarray(candidate)
+    alignment = np.dot(candidate, reference) / max(np.dot(candidate, candidate), np.finfo(float).eps)
+    err = np.sum((alignment * candidate - reference) ** 2)
+    return math.sqrt(err / max(np.dot(reference, reference), np.finfo(float).eps))
+
+
+def test_epry_recovers_relative_frame_gain_mismatch():
+    """tests/reconstruction_tests.rs:3037: true gains 0.5/1/1.5/2/0.75 on the
+    direct model, constant object 2, EPRY 2 iterations with object step 1e-6,
+    no pupil recovery, gain recovery at step 1: the aligned relative gain
+    error falls below a tenth of that of the mean gain."""
+    true_gains = np.array([0.5, 1.0, 1.5, 2.0, 0.75])
+    true_model = ro.direct_model().with_frame_gains(true_gains)
+    meas = MeasurementStack(frames=oracle.forward_stack(true_model, ro.spectrum_of(np.full((16, 16), 2.0 + 0j))))
+    alg = Epry(iterations=2, object_step=1e-6, recover_pupil=False, recover_frame_gains=True, gain_step=1.0)
+    res = oracle.run(ro.direct_model(), meas, alg.to_c(), 2)
+    recovered = _aligned_gain_error(true_gains, res["frame_gains"])
+    initial = math.sqrt(np.sum((true_gains.mean() - true_gains) ** 2) / np.sum(true_gains ** 2))
+    assert recovered < 0.1 * initial
+
+
+def test_epry_recovers_relative_per_frame_background():
+    """tests/reconstruction_tests.rs:3081: per-frame backgrounds 0/1/2/3/0.5,
+    EPRY 1 iteration, background recovery at step 1 within (-10, 10): the
+    recovered frame means relative to their mean equal the truth's (1e-8)."""
+    true_bg = np.array([0.0, 1.0, 2.0, 3.0, 0.5])
+    true_model = ro.direct_model().with_background(np.repeat(true_bg, 64))
+    meas = MeasurementStack(frames=oracle.forward_stack(true_model, ro.spectrum_of(np.full((16, 16), 2.0 + 0j))))
+    alg = Epry(iterations=1, object_step=1e-6, recover_pupil=False, recover_background=True, background_step=1.0,
+               minimum_background=-10.0, maximum_background=10.0)
+    res = oracle.run(ro.direct_model(), meas, alg.to_c(), 1)
+    means = res["background"].reshape(5, 64).mean(axis=1)
+    assert np.allclose(means - means.mean(), true_bg - true_bg.mean(), atol=1e-8, rtol=0)
