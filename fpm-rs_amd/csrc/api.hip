@@ -214,8 +214,13 @@ void check_flags(fpmhip_session& s) {
 void init_aux(fpmhip_session& s) {
   if (!s.cfg.momentum) return;
   const uint64_t M = s.model->M();
-  launch_fill_zero(s.b.vel.as<double2>(), M, s.ctx->stream);
-  HIP_CHECK(hipMemcpyAsync(s.b.anc.ptr, s.b.spec.ptr, M * sizeof(double2), hipMemcpyDeviceToDevice, s.ctx->stream));
+  if (s.precision == FPMHIP_PLAIN_C128) {
+    launch_fill_zero(s.b.vel.as<double2>(), M, s.ctx->stream);
+    HIP_CHECK(hipMemcpyAsync(s.b.anc.ptr, s.b.spec.ptr, M * sizeof(double2), hipMemcpyDeviceToDevice, s.ctx->stream));
+  } else {
+    launch_fill_zero(s.b.vel.as<float2>(), M, s.ctx->stream);
+    launch_convert_pupil(s.b.spec.as<double2>(), s.b.anc.ptr, true, M, s.ctx->stream);
+  }
   s.counter = 0;
 }
 
@@ -703,8 +708,8 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     SessionBuffers& b = s->b;
     b.spec.allocate(M * sizeof(double2));
     if (s->cfg.momentum) {
-      b.vel.allocate(M * sizeof(double2));
-      b.anc.allocate(M * sizeof(double2));
+      b.vel.allocate(M * vb);
+      b.anc.allocate(M * vb);
     }
     b.pupil.allocate(N * vb);
     b.patch.allocate(kmax * N * vb);
@@ -807,8 +812,17 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
       if (velocity && anchor) {
         if (counter >= s->cfg.interval)
           fail(FPMHIP_INVALID_MODEL, "mPIE frame counter exceeds the configured momentum interval");
-        HIP_CHECK(hipMemcpyAsync(s->b.vel.ptr, velocity, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(s->b.anc.ptr, anchor, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+        if (s->precision == FPMHIP_PLAIN_C128) {
+          HIP_CHECK(hipMemcpyAsync(s->b.vel.ptr, velocity, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+          HIP_CHECK(hipMemcpyAsync(s->b.anc.ptr, anchor, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+        } else {
+          // complex64 storage: through the FP64 scratch, rounded on the device
+          for (int k = 0; k < 2; ++k) {
+            HIP_CHECK(hipMemcpyAsync(s->b.work_m.ptr, k ? anchor : velocity, M * sizeof(double2),
+                                     hipMemcpyHostToDevice, s->ctx->stream));
+            launch_convert_pupil(s->b.work_m.as<double2>(), k ? s->b.anc.ptr : s->b.vel.ptr, true, M, s->ctx->stream);
+          }
+        }
         s->counter = counter;
       } else {
         init_aux(*s);
@@ -898,9 +912,19 @@ int fpmhip_session_get_momentum(fpmhip_session* s, double* velocity, double* anc
     require(s != nullptr, "session", "session is null");
     if (!s->cfg.momentum) fail(FPMHIP_UNSUPPORTED, "session has no mPIE momentum state");
     set_device(s->ctx);
-    const size_t bytes = s->model->M() * sizeof(double2);
-    if (velocity) HIP_CHECK(hipMemcpyAsync(velocity, s->b.vel.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
-    if (anchor) HIP_CHECK(hipMemcpyAsync(anchor, s->b.anc.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+    const uint64_t M = s->model->M();
+    const size_t bytes = M * sizeof(double2);
+    for (int k = 0; k < 2; ++k) {
+      double* out = k ? anchor : velocity;
+      if (!out) continue;
+      const DeviceBuffer& src = k ? s->b.anc : s->b.vel;
+      if (s->precision == FPMHIP_PLAIN_C128) {
+        HIP_CHECK(hipMemcpyAsync(out, src.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+      } else {
+        launch_pupil_to_f64(src.ptr, true, s->b.work_m.as<double2>(), M, s->ctx->stream);
+        HIP_CHECK(hipMemcpyAsync(out, s->b.work_m.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+      }
+    }
     HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
     if (counter) *counter = s->counter;
   });

@@ -51,8 +51,8 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   const Measurements& me = *s.meas;
   t = TileDesc{};
   t.spec = s.b.spec.as<double2>();
-  t.vel = s.b.vel.as<double2>();
-  t.anc = s.b.anc.as<double2>();
+  t.vel = s.b.vel.as<float2>();
+  t.anc = s.b.anc.as<float2>();
   t.pupil = s.b.pupil.as<float2>();
   t.extent = s.b.extent.as<uint32_t>();
   t.field = m.nh > kFusedLdsMaxN ? s.b.field.as<float2>() : nullptr;  // the session's N-pixel field buffer

@@ -10,8 +10,8 @@ namespace fpmhip {
 // Everything one workgroup needs to run one sweep of one tile.
 struct TileDesc {
   double2* spec;          // [R] FP64 centred object spectrum
-  double2* vel;           // [R] mPIE velocity (or null)
-  double2* anc;           // [R] mPIE anchor (or null)
+  float2* vel;            // [R] mPIE velocity (or null), complex64 storage
+  float2* anc;            // [R] mPIE anchor (or null), complex64 storage
   const float2* pupil;    // [N] complex64 pupil
   const uint32_t* extent; // [n] per-row support of the pupil: columns [lo, hi) as lo | hi << 16
   float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)

@@ -224,18 +224,28 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
-// Fixed-order workgroup sum of one double per thread.
-__device__ __forceinline__ double wg_sum(double v, double* red) {
+// Fixed-order workgroup sums of two doubles per thread (one pair of
+// barriers for both); red holds 2 * (kThreads / 64) doubles.
+__device__ __forceinline__ double2 wg_sum2(double a, double b, double* red) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
   const int wave = threadIdx.x >> 6;
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  if ((threadIdx.x & 63) == 0) {
+    red[wave] = a;
+    red[kThreads / 64 + wave] = b;
+  }
   __syncthreads();
-  double s = 0.0;
+  double sa = 0.0, sb = 0.0;
   if (threadIdx.x == 0)
-    for (int w = 0; w < kThreads / 64; ++w) s += red[w];
-  return s;  // valid in thread 0
+    for (int w = 0; w < kThreads / 64; ++w) {
+      sa += red[w];
+      sb += red[kThreads / 64 + w];
+    }
+  return make_double2(sa, sb);  // valid in thread 0
 }
 
 // Object-update denominator of common.rs:247-258 as a + b |P|^2 with FP64
@@ -312,6 +322,45 @@ struct CoherentMem {
   }
 };
 
+// Raw buffer accesses of one array (ranged by its byte size): a lane whose
+// byte offset is kSkip (beyond every range) loads zero without touching
+// memory and its store is dropped. Conditional accesses written this way
+// need no branch, so all of a thread's loads can be in flight at once
+// instead of one basic block (and one memory round trip) per condition.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t kSkip = 0x80000000u;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ double2 bld_d2(Rsrc r, uint32_t byte_off) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  const v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+  return make_double2(__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z));
+}
+__device__ __forceinline__ void bst_d2(Rsrc r, uint32_t byte_off, double2 d) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  v4i v;
+  v.x = __double2loint(d.x);
+  v.y = __double2hiint(d.x);
+  v.z = __double2loint(d.y);
+  v.w = __double2hiint(d.y);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, byte_off, 0, 0);
+}
+__device__ __forceinline__ void bst_f2(Rsrc r, uint32_t byte_off, float2 f) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  v2i v;
+  v.x = __float_as_int(f.x);
+  v.y = __float_as_int(f.y);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, 0);
+}
+__device__ __forceinline__ float2 bld_f2(Rsrc r, uint32_t byte_off) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  const v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  return make_float2(__int_as_float(v.x), __int_as_float(v.y));
+}
+
 // Measured value of one pixel (uint16 counts or f64).
 __device__ __forceinline__ float measured(const TileDesc& T, uint64_t i) {
   return T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[i] : (float)static_cast<const double*>(T.frames)[i];
@@ -336,19 +385,60 @@ __device__ __forceinline__ float2 replace_amplitude(const TileDesc& T, uint64_t 
   return f2(fsqrt(target) - fv.x, -fv.y);
 }
 
-// mPIE momentum event over the tile's whole R^2 spectrum (common.rs:308-387);
-// workgroup `member` of `members` takes every members-th block of elements.
+// mPIE momentum event over the tile's whole R^2 spectrum (common.rs:308-387):
+// V <- friction V + (O - A), O <- O + feedback V, A <- O, evaluated in FP64
+// with V and A stored as complex64; workgroup `member` of `members` takes
+// every members-th block of elements.
+__device__ __forceinline__ void momentum_element(const TileDesc& T, double2& o, float2& v, float2& an, bool& bad) {
+  const double2 nv = make_double2(fma(T.friction, (double)v.x, o.x - (double)an.x),
+                                  fma(T.friction, (double)v.y, o.y - (double)an.y));
+  const double2 no = T.feedback == 0.0 ? o : make_double2(fma(T.feedback, nv.x, o.x), fma(T.feedback, nv.y, o.y));
+  bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
+  o = no;
+  v = f2((float)nv.x, (float)nv.y);
+  an = f2((float)no.x, (float)no.y);
+}
+
 template <class MEM = PlainMem>
 __device__ __forceinline__ void momentum_event(const TileDesc& T, int tid, uint32_t member = 0,
                                                uint32_t members = 1) {
   const uint32_t M = T.rh * T.rw;
   double2* __restrict__ spec = T.spec;
-  double2* __restrict__ vel = T.vel;
-  double2* __restrict__ anc = T.anc;
+  float2* __restrict__ vel = T.vel;
+  float2* __restrict__ anc = T.anc;
   bool bad = false;
   constexpr int U = 4;  // loads of U elements in flight before their use
+  if constexpr (!MEM::kCoherent) {
+    // branch-free: elements past M load zero and store nothing
+    const Rsrc rs = make_rsrc(spec, M * 16u), rv = make_rsrc(vel, M * 8u), ra = make_rsrc(anc, M * 8u);
+    for (uint32_t base = tid; base < M; base += U * kThreads) {
+      double2 o[U];
+      float2 v[U], an[U];
+      uint32_t idx[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t i = base + u * kThreads;
+        idx[u] = i < M ? i : kSkip / 16u;
+        o[u] = bld_d2(rs, i < M ? i * 16u : kSkip);
+        v[u] = bld_f2(rv, i < M ? i * 8u : kSkip);
+        an[u] = bld_f2(ra, i < M ? i * 8u : kSkip);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        momentum_element(T, o[u], v[u], an[u], bad);
+        const bool live = idx[u] != kSkip / 16u;
+        bst_f2(rv, live ? idx[u] * 8u : kSkip, v[u]);
+        bst_d2(rs, live ? idx[u] * 16u : kSkip, o[u]);
+        bst_f2(ra, live ? idx[u] * 8u : kSkip, an[u]);
+      }
+    }
+    if (bad) T.status[0] = 1;
+    __syncthreads();
+    return;
+  }
   for (uint32_t base = member * U * kThreads + tid; base < M; base += members * U * kThreads) {
-    double2 o[U], v[U], an[U];
+    double2 o[U];
+    float2 v[U], an[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t i = base + u * kThreads;
@@ -362,14 +452,10 @@ __device__ __forceinline__ void momentum_event(const TileDesc& T, int tid, uint3
     for (int u = 0; u < U; ++u) {
       const uint32_t i = base + u * kThreads;
       if (i < M) {
-        const double2 nv =
-            make_double2(fma(T.friction, v[u].x, o[u].x - an[u].x), fma(T.friction, v[u].y, o[u].y - an[u].y));
-        const double2 no =
-            T.feedback == 0.0 ? o[u] : make_double2(fma(T.feedback, nv.x, o[u].x), fma(T.feedback, nv.y, o[u].y));
-        bad |= !isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y);
-        MEM::st(vel + i, nv);
-        MEM::st(spec + i, no);
-        MEM::st(anc + i, no);
+        momentum_element(T, o[u], v[u], an[u], bad);
+        MEM::st(vel + i, v[u]);
+        MEM::st(spec + i, o[u]);
+        MEM::st(anc + i, an[u]);
       }
     }
   }
@@ -430,6 +516,70 @@ __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, c
   }
 }
 
+// gather_strip through buffer loads, without branches: pixels outside their
+// row's support extent load their pupil value and spectrum taps with offset
+// kSkip (zero, no memory access), so the strip's pupil loads and its STRIP+1
+// rows of taps are all independent of each other and of loaded values.
+// `pofs` is the pupil element index of the strip's first pixel. Inside the
+// extents the arithmetic is gather_strip's.
+template <int STRIP, class Put>
+__device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
+                                               int c, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du, double wll,
+                                               double wlu, double wul, double wuu, Put put) {
+  // sub-blocks of SUB rows: SUB pupil values and SUB (+1) tap rows in flight;
+  // tap row t is the lower row of pixel t and the upper row of pixel t - 1,
+  // so each sub-block carries its last tap row into the next.
+  constexpr int SUB = STRIP < 4 ? STRIP : 4;
+  static_assert(STRIP % SUB == 0, "strip");
+  double2 clo = make_double2(0.0, 0.0), chi = clo;  // carried tap row
+#pragma unroll
+  for (int j0 = 0; j0 < STRIP; j0 += SUB) {
+    float2 P[SUB];
+    double2 lo[SUB + 1], hi[SUB + 1];
+#pragma unroll
+    for (int k = 0; k < SUB; ++k)
+      P[k] = bld_f2(rpup, in_extent(ext[j0 + k], c) ? (pofs + (j0 + k) * pstride) * 8u : kSkip);
+#pragma unroll
+    for (int k = 0; k <= SUB; ++k) {
+      const int t = j0 + k;
+      if (k == 0 && j0 > 0) {
+        lo[0] = clo;
+        hi[0] = chi;
+        continue;
+      }
+      const bool need = (t < STRIP && in_extent(ext[t], c)) || (du != 0 && t > 0 && in_extent(ext[t - 1], c));
+      const uint32_t o = off + t * rw;
+      const uint32_t ol = need ? o * 16u : kSkip;
+      lo[k] = bld_d2(rspec, ol);
+      hi[k] = bld_d2(rspec, dc != 0 ? ol + 16u : kSkip);
+    }
+    clo = lo[SUB];
+    chi = hi[SUB];
+    // An axis without a fraction has an upper weight of exactly zero, so its
+    // (skipped, zero) upper taps add nothing, as the lower taps repeated do.
+#pragma unroll
+    for (int k = 0; k < SUB; ++k) {
+      const double2 a = lo[k], b = hi[k], c2 = lo[k + 1], d = hi[k + 1];
+      const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
+                                     fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
+      put(j0 + k, fmul(f2((float)v.x, (float)v.y), P[k]));  // P and the taps are zero off the extent
+    }
+  }
+}
+
+// copy_strip through buffer loads, without branches (see gather_strip_b).
+template <int STRIP, class Put>
+__device__ __forceinline__ void copy_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
+                                             int c, uint32_t off, uint32_t rw, Put put) {
+#pragma unroll
+  for (int j = 0; j < STRIP; ++j) {
+    const bool in = in_extent(ext[j], c);
+    const float2 P = bld_f2(rpup, in ? (pofs + j * pstride) * 8u : kSkip);
+    const double2 v = bld_d2(rspec, in ? (off + j * rw) * 16u : kSkip);
+    put(j, fmul(f2((float)v.x, (float)v.y), P));
+  }
+}
+
 // Integer-offset crop: the patch is a plain copy of the window (crop.rs:79-102).
 template <int STRIP, class MEM = PlainMem, class Put>
 __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
@@ -455,8 +605,9 @@ __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, con
 // non-zero. U is zero outside the pupil's support extents `ext`, so pixels
 // beyond them are skipped without reading U.
 template <int N, int THREADS, class MEM = PlainMem, class GetU>
-__device__ __forceinline__ void insert_window(double2* __restrict__ spec, const SrcParam& sp, uint32_t rw,
-                                              double scale, int tid, int a0, int a1, const uint32_t* ext, GetU u) {
+__device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
+                                              uint32_t rw, double scale, int tid, int a0, int a1, const uint32_t* ext,
+                                              GetU u) {
   const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
   // destination columns that can receive a non-zero update: the union of the
   // support extents of tap rows a - 1 and a, widened by one for a column tap
@@ -517,14 +668,25 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, const 
       hit[k] = qd < total && (ax != 0.0f || ay != 0.0f);
     }
     double2 old[UNROLL];
+    if constexpr (!MEM::kCoherent) {
+      // branch-free: every read of the group is in flight at once
+      const Rsrc rs = make_rsrc(spec, spec_elems * 16u);
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k)
-      if (hit[k]) old[k] = MEM::ld(spec + off[k]);
+      for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2(rs, hit[k] ? off[k] * 16u : kSkip);
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k)
-      if (hit[k])
-        MEM::st(spec + off[k],
-                make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
+      for (int k = 0; k < UNROLL; ++k)
+        bst_d2(rs, hit[k] ? off[k] * 16u : kSkip,
+               make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
+    } else {
+#pragma unroll
+      for (int k = 0; k < UNROLL; ++k)
+        if (hit[k]) old[k] = MEM::ld(spec + off[k]);
+#pragma unroll
+      for (int k = 0; k < UNROLL; ++k)
+        if (hit[k])
+          MEM::st(spec + off[k],
+                  make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
+    }
   }
 }
 
@@ -548,6 +710,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   const double eps = T.eps;
   const float eps_f = (float)eps;
   const Denominator den = make_denominator(T, fmax(T.max_p, eps));
+  const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
   uint64_t counter = *T.counter;
   for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
     double obj_sum = 0.0, w_sum = 0.0;
@@ -572,13 +735,13 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           const int cs = (cc + HALF) % N;
           auto put = [&](int j, float2 v) { lds[((rb + j + HALF) % N) * LD + cs] = v; };
           if (sp.zero) {
-            copy_strip<STRIP>(spec, pupil + rb * N + cc, N, ext + rb, cc,
-                              ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw, put);
+            copy_strip_b<STRIP>(rspec, rpup, rb * N + cc, N, ext + rb, cc,
+                                ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw, put);
           } else {
-            gather_strip<STRIP>(spec, pupil + rb * N + cc, N, ext + rb, cc,
-                                (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw,
-                                sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl,
-                                sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
+            gather_strip_b<STRIP>(rspec, rpup, rb * N + cc, N, ext + rb, cc,
+                                  (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw, sp.ac.upper - sp.ac.lower,
+                                  (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu,
+                                  sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
           }
         }
       }
@@ -615,8 +778,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       }
       // per-thread partials (at most 16 terms) in FP32, workgroup reduction in FP64
       const double lsum = (double)lsum_f, lcnt = (double)lcnt_i;
-      const double ls = wg_sum(lsum, red);
-      const double lc = wg_sum(lcnt, red);
+      const double2 lsc = wg_sum2(lsum, lcnt, red);
+      const double ls = lsc.x, lc = lsc.y;
       if (tid == 0) {
         const double L = lc > 0.0 ? ls / lc : 0.0;
         T.frame_loss[pos] = L;
@@ -637,7 +800,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           const float2 s0 = lds[l1], s1 = lds[l0];
           // U is zero outside the support extents: no pupil read there
           const bool in0 = in_extent(ext[r], c), in1 = in_extent(ext[r1], c1);
-          const float2 P0 = in0 ? pupil[r * N + c] : f2(0.0f, 0.0f), P1 = in1 ? pupil[r1 * N + c1] : f2(0.0f, 0.0f);
+          const float2 P0 = bld_f2(rpup, in0 ? (r * N + c) * 8u : kSkip);
+          const float2 P1 = bld_f2(rpup, in1 ? (r1 * N + c1) * 8u : kSkip);
           const float k0 = inv_n / fmaf(den.b, fmaf(P0.x, P0.x, P0.y * P0.y), den.a);
           const float k1 = inv_n / fmaf(den.b, fmaf(P1.x, P1.x, P1.y * P1.y), den.a);
           const float2 u0 = fmulc(P0, s0), u1 = fmulc(P1, s1);
@@ -650,16 +814,27 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       //    pixels that receive a non-zero update (U is zero outside the pupil)
       const double scale = fw * T.object_step;
       if (sp.zero) {
-        for (int p = tid; p < NN; p += kThreads) {
-          const int r = p / N, c = p % N;
-          const float2 u = lds[r * LD + c];
-          if (u.x != 0.0f || u.y != 0.0f) {
-            double2& d = spec[(sp.sr + r) * T.rw + sp.sc + c];
-            d = make_double2(fma(scale, (double)u.x, d.x), fma(scale, (double)u.y, d.y));
+        // branch-free RMW in groups of G pixels per thread
+        constexpr int G = NN / kThreads < 8 ? (NN / kThreads > 0 ? NN / kThreads : 1) : 8;
+        for (int p0 = tid; p0 < NN; p0 += G * kThreads) {
+          float2 u[G];
+          uint32_t off[G];
+          double2 d[G];
+#pragma unroll
+          for (int k = 0; k < G; ++k) {
+            const int p = p0 + k * kThreads, r = p / N, c = p % N;
+            u[k] = p < NN ? lds[r * LD + c] : f2(0.0f, 0.0f);
+            off[k] = (u[k].x != 0.0f || u[k].y != 0.0f) ? (uint32_t)(((sp.sr + r) * T.rw + sp.sc + c) * 16u) : kSkip;
+            d[k] = bld_d2(rspec, off[k]);
           }
+#pragma unroll
+          for (int k = 0; k < G; ++k)
+            bst_d2(rspec, off[k],
+                   make_double2(fma(scale, (double)u[k].x, d[k].x), fma(scale, (double)u[k].y, d[k].y)));
         }
       } else {
-        insert_window<N, kThreads>(spec, sp, T.rw, scale, tid, 0, N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0), ext,
+        insert_window<N, kThreads>(spec, T.rh * T.rw, sp, T.rw, scale, tid, 0,
+                                   N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0), ext,
                                    [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
       }
       __syncthreads();
@@ -892,8 +1067,8 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         __syncthreads();
       }
       {
-        double ls = wg_sum(lsum, red);
-        double lc = wg_sum((double)lcnt, red);
+        const double2 lsc = wg_sum2(lsum, (double)lcnt, red);
+        double ls = lsc.x, lc = lsc.y;
         if (COOP) {
           if (tid == 0) MEM::st(T.partial + T.member, make_double2(ls, lc));
           if (!group_barrier(T, passed, bflag)) return;
@@ -954,7 +1129,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         } else {
           const int a1 = a0 + CH + ((q == NCH - 1 && sp.ar.upper != sp.ar.lower) ? 1 : 0);
           const float2* above = COOP ? T.edge + (q - 1) * N : prev;
-          insert_window<N, kThreads, MEM>(spec, sp, T.rw, scale, opaque_tid(), a0, a1, ext, [&](int sr_, int sc_) {
+          insert_window<N, kThreads, MEM>(spec, T.rh * T.rw, sp, T.rw, scale, opaque_tid(), a0, a1, ext, [&](int sr_, int sc_) {
             return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)]
                              : (COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF]);
           });
@@ -992,7 +1167,7 @@ __global__ void __launch_bounds__(kThreads)
   float2* twf = prev + (N > (int)kFusedLdsMaxN ? N : 0);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
-  int* bflag = reinterpret_cast<int*>(red + 16);
+  int* bflag = reinterpret_cast<int*>(red + 2 * (kThreads / 64));
   uint32_t* ext = reinterpret_cast<uint32_t*>(bflag + 4);
   const float2* src = tw_all + tw_offset(N);
   for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) twf[j] = src[j];
@@ -1034,7 +1209,7 @@ __global__ void __launch_bounds__(kThreads)
 
 size_t sweep_lds_bytes(uint32_t nmax) {
   return ((size_t)sweep_field_elems(nmax) + (nmax > kFusedLdsMaxN ? nmax : 0) + 2 * (size_t)nmax) * sizeof(float2) +
-         16 * sizeof(double) + 16 + (size_t)nmax * sizeof(uint32_t);
+         2 * (kThreads / 64) * sizeof(double) + 16 + (size_t)nmax * sizeof(uint32_t);
 }
 
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {

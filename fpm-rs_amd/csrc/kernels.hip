@@ -308,17 +308,20 @@ __global__ void k_pupil_step(V* __restrict__ pupil, const V* __restrict__ grad, 
 }
 
 // apply_momentum_event (common.rs:355-387)
-__global__ void k_momentum(double2* __restrict__ obj, double2* __restrict__ vel, double2* __restrict__ anc,
-                           uint64_t n, double friction, double feedback, int* __restrict__ flag) {
+// The velocity and anchor are stored as W (complex128, or complex64 in
+// MIXED_F32 sessions); the recurrence is evaluated in FP64 either way.
+template <class W>
+__global__ void k_momentum(double2* __restrict__ obj, W* __restrict__ vel, W* __restrict__ anc, uint64_t n,
+                           double friction, double feedback, int* __restrict__ flag) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double2 o = obj[i];
-  const double2 nv = D::add(D::scale(friction, vel[i]), D::sub(o, anc[i]));
+  const double2 nv = D::add(D::scale(friction, to_d2(vel[i])), D::sub(o, to_d2(anc[i])));
   const double2 no = feedback == 0.0 ? o : D::add(o, D::scale(feedback, nv));
   if (!isfinite(nv.x) || !isfinite(nv.y) || !isfinite(no.x) || !isfinite(no.y)) flag[0] = 1;
-  vel[i] = nv;
+  vel[i] = from_d2<W>(nv);
   obj[i] = no;
-  anc[i] = no;
+  anc[i] = from_d2<W>(no);
 }
 
 __global__ void k_flag_reset(int* flag) {
@@ -388,9 +391,10 @@ __global__ void k_shift(const double2* __restrict__ src, double2* __restrict__ d
   dst[(uint64_t)((r + rs) % h) * w + (c + cs) % w] = src[i];
 }
 
-__global__ void k_fill_zero(double2* dst, uint64_t n) {
+template <class V>
+__global__ void k_fill_zero(V* dst, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = make_double2(0.0, 0.0);
+  if (i < n) dst[i] = V{};
 }
 
 template <class V>
@@ -553,9 +557,9 @@ void launch_pupil_step(const fpmhip_session& s, double step) {
 
 void launch_momentum(const fpmhip_session& s) {
   const uint64_t M = s.model->M();
-  hipLaunchKernelGGL(k_momentum, dim3(blocks_for(M)), dim3(kBlock), 0, s.ctx->stream, s.b.spec.as<double2>(),
-                     s.b.vel.as<double2>(), s.b.anc.as<double2>(), M, s.cfg.friction, s.cfg.feedback,
-                     s.b.flag.as<int>());
+  DISPATCH_V(s, hipLaunchKernelGGL(k_momentum<V>, dim3(blocks_for(M)), dim3(kBlock), 0, s.ctx->stream,
+                                   s.b.spec.as<double2>(), s.b.vel.as<V>(), s.b.anc.as<V>(), M, s.cfg.friction,
+                                   s.cfg.feedback, s.b.flag.as<int>()));
   HIP_CHECK(hipGetLastError());
 }
 
@@ -639,7 +643,12 @@ void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t 
 }
 
 void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream) {
-  hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(n)), dim3(kBlock), 0, stream, dst, n);
+  hipLaunchKernelGGL(k_fill_zero<double2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, dst, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_fill_zero(float2* dst, uint64_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fill_zero<float2>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, dst, n);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -110,9 +110,9 @@ namespace {
 uint64_t job_bytes(const fpmhip_model_desc& d, uint64_t iterations) {
   const uint64_t N = d.image_h * d.image_w, M = d.recon_h * d.recon_w;
   const uint64_t F = d.mux_row_ptr ? d.frame_count : d.source_count;
-  // spectrum, velocity, anchor, scratch (FP64 complex) + per-pixel session
-  // buffers + counts + model arrays + per-tile scratch
-  return 4 * M * 16 + 12 * N * 16 + F * N * 2 + N * 25 + F * 64 + 16 + (iterations + F) * 8;
+  // spectrum and scratch (FP64 complex), velocity and anchor (FP32 complex)
+  // + per-pixel session buffers + counts + model arrays + per-tile scratch
+  return 2 * M * 16 + 2 * M * 8 + 12 * N * 16 + F * N * 2 + N * 25 + F * 64 + 16 + (iterations + F) * 8;
 }
 
 // Speed-up of a cooperative group over one workgroup on a 256^2 tile

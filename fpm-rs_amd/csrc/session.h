@@ -38,7 +38,7 @@ Config make_config(const fpmhip_algorithm& a);
 
 struct SessionBuffers {
   DeviceBuffer spec;      // double2[M] centred object spectrum (FP64 accumulation)
-  DeviceBuffer vel, anc;  // double2[M] mPIE velocity/anchor
+  DeviceBuffer vel, anc;  // V[M] mPIE velocity/anchor (momentum arithmetic is FP64 either way)
   DeviceBuffer pupil;     // V[N]
   DeviceBuffer patch;     // V[K*N]
   DeviceBuffer field;     // V[K*N]
@@ -110,6 +110,7 @@ void launch_convert_pupil(const double2* src, void* dst, bool to_f32, uint64_t n
 void launch_pupil_extent(const fpmhip_session& s);
 void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream);
 void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream);
+void launch_fill_zero(float2* dst, uint64_t n, hipStream_t stream);
 void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields,
                           double* out, hipStream_t stream);
 

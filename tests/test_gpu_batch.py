@@ -228,3 +228,29 @@ def test_invalid_schedule_is_rejected(ctx):
         with pytest.raises(runtime.FpmError) as e:
             b.set_schedule(**kw)
         assert e.value.kind == "InvalidParameter"
+
+
+def test_bench_workload_full_length_precision(ctx):
+    """The measured workload itself, at full length: 225 LEDs, 128^2 images
+    (375^2 object), Poisson counts at 1e4 peak as uint16 (seed 7), mPIE 50
+    sweeps with interval 30 (375 momentum events), friction/feedback 0.5.
+    The batched FP32 result stays within the precision form's bound of the
+    FP64 oracle run on the same inputs: amplitude and phase relative L2
+    <= 1e-5, and every iteration objective of the trace within 1e-5."""
+    model = runtime.compile_model(syn.base_optics(), syn.planar_array(15), (128, 128))
+    obj = syn.random_field_object(model.reconstruction_shape, 7)
+    I = runtime.forward_stack(model, syn.object_spectrum(obj))
+    counts, ppp = syn.poisson_counts(I, 1e4, 7)
+    model.with_frame_gains(np.full(model.frame_count, ppp))
+    meas = MeasurementStack(counts=counts)
+    alg = Mpie(iterations=50, momentum_friction=0.5, momentum_feedback=0.5)
+    s = runtime.Session(ReconstructionProblem(meas, model), alg, precision="f32")
+    trace = runtime.Batch([s]).run(50)[0]
+    say("bench workload: oracle, 50 sweeps")
+    ref = oracle.run(model, meas, alg.to_c(), 50)
+    ea, ep = amp_phase_err(s.object(), ref["object"])
+    es = np.linalg.norm(s.spectrum() - ref["object_spectrum"]) / np.linalg.norm(ref["object_spectrum"])
+    te = np.abs(trace - ref["trace"]) / np.abs(ref["trace"])
+    say("50 sweeps: amp", ea, "phase", ep, "spec", es, "trace", te.max())
+    assert ea <= 1e-5 and ep <= 1e-5 and es <= 1e-5
+    assert te.max() <= 1e-5
