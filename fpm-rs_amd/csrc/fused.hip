@@ -288,12 +288,39 @@ __device__ __forceinline__ float point_loss_f(float predicted, float measured, i
 // device-coherent level, so that workgroups of one cooperative tile group on
 // different CUs / XCDs see each other's writes after a group barrier without
 // invalidating or writing back whole caches.
+// A pointer read from a tile descriptor, as a global-memory pointer: its
+// accesses become global ones, not flat ones (a flat access also counts
+// against the LDS counter, so waiting for LDS data would wait for it too).
+template <class P>
+__device__ __forceinline__ const __attribute__((address_space(1))) P* gptr(const P* p) {
+  return (const __attribute__((address_space(1))) P*)p;
+}
+template <class P>
+__device__ __forceinline__ __attribute__((address_space(1))) P* gptr_mut(P* p) {
+  return (__attribute__((address_space(1))) P*)p;
+}
+
 struct PlainMem {
   static constexpr bool kCoherent = false;
-  __device__ static __forceinline__ double2 ld(const double2* p) { return *p; }
-  __device__ static __forceinline__ void st(double2* p, double2 v) { *p = v; }
-  __device__ static __forceinline__ float2 ld(const float2* p) { return *p; }
-  __device__ static __forceinline__ void st(float2* p, float2 v) { *p = v; }
+  // component-wise through global pointers (the pair is one wide access)
+  __device__ static __forceinline__ double2 ld(const double2* p) {
+    const auto q = gptr(reinterpret_cast<const double*>(p));
+    return make_double2(q[0], q[1]);
+  }
+  __device__ static __forceinline__ void st(double2* p, double2 v) {
+    const auto q = gptr_mut(reinterpret_cast<double*>(p));
+    q[0] = v.x;
+    q[1] = v.y;
+  }
+  __device__ static __forceinline__ float2 ld(const float2* p) {
+    const auto q = gptr(reinterpret_cast<const float*>(p));
+    return make_float2(q[0], q[1]);
+  }
+  __device__ static __forceinline__ void st(float2* p, float2 v) {
+    const auto q = gptr_mut(reinterpret_cast<float*>(p));
+    q[0] = v.x;
+    q[1] = v.y;
+  }
 };
 
 struct CoherentMem {
@@ -348,6 +375,14 @@ __device__ __forceinline__ void bst_d2(Rsrc r, uint32_t byte_off, double2 d) {
   v.w = __double2hiint(d.y);
   __builtin_amdgcn_raw_buffer_store_b128(v, r, byte_off, 0, 0);
 }
+__device__ __forceinline__ float bld_u16f(Rsrc r, uint32_t byte_off) {
+  return (float)(unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0);
+}
+__device__ __forceinline__ float bld_f64f(Rsrc r, uint32_t byte_off) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  const v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  return (float)__hiloint2double(v.y, v.x);
+}
 __device__ __forceinline__ void bst_f2(Rsrc r, uint32_t byte_off, float2 f) {
   typedef int v2i __attribute__((ext_vector_type(2)));
   v2i v;
@@ -363,7 +398,8 @@ __device__ __forceinline__ float2 bld_f2(Rsrc r, uint32_t byte_off) {
 
 // Measured value of one pixel (uint16 counts or f64).
 __device__ __forceinline__ float measured(const TileDesc& T, uint64_t i) {
-  return T.is_u16 ? (float)static_cast<const uint16_t*>(T.frames)[i] : (float)static_cast<const double*>(T.frames)[i];
+  return T.is_u16 ? (float)gptr(static_cast<const uint16_t*>(T.frames))[i]
+                  : (float)gptr(static_cast<const double*>(T.frames))[i];
 }
 
 // Amplitude replacement of pixel p of frame f (fbase = f * NN) kept as the
@@ -372,9 +408,9 @@ __device__ __forceinline__ float measured(const TileDesc& T, uint64_t i) {
 __device__ __forceinline__ float2 replace_amplitude(const TileDesc& T, uint64_t fbase, uint64_t NN, int p, float2 fv,
                                                     float y, float gain_f, float inv_gain, float eps_f, float& lsum,
                                                     int& lcnt) {
-  if (T.masks && T.masks[fbase + p] == 0) return f2(0.0f, 0.0f);
+  if (T.masks && gptr(T.masks)[fbase + p] == 0) return f2(0.0f, 0.0f);
   const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
-  const float b = T.bg_len == 0 ? 0.0f : (float)T.bg[T.bg_len == NN ? (uint64_t)p : fbase + p];
+  const float b = T.bg_len == 0 ? 0.0f : (float)gptr(T.bg)[T.bg_len == NN ? (uint64_t)p : fbase + p];
   lsum += point_loss_f(fmaf(gain_f, pred, b), y, T.loss);
   lcnt += 1;
   const float target = fmaxf((y - b) * inv_gain, 0.0f);
@@ -711,11 +747,13 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   const float eps_f = (float)eps;
   const Denominator den = make_denominator(T, fmax(T.max_p, eps));
   const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
+  // the common case: every pixel counts, no background, amplitude MSE
+  const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   uint64_t counter = *T.counter;
   for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
     double obj_sum = 0.0, w_sum = 0.0;
     for (uint32_t pos = 0; pos < T.F; ++pos) {
-      const uint32_t f = T.order ? T.order[pos] : pos;
+      const uint32_t f = __builtin_amdgcn_readfirstlane(T.order ? T.order[pos] : pos);  // uniform
       const double fw = T.weights[f];
       const int tid = opaque_tid();
       if (fw == 0.0) {
@@ -756,7 +794,36 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       const uint64_t fbase = (uint64_t)f * NN;
       float lsum_f = 0.0f;
       int lcnt_i = 0;
-      {
+      if (plain_frames) {
+        // branch-free: measurement loads at constant offsets, both cases of
+        // the ratio (common.rs:180-190) evaluated and selected
+        static_assert(NN % kThreads == 0, "pixels per thread");
+        constexpr int PP = NN / kThreads, RSTEP = kThreads / N;
+        const int row0 = tid / N, col = tid % N;
+        float y[PP];
+        if (T.is_u16) {
+          const Rsrc rm = make_rsrc(static_cast<const uint16_t*>(T.frames) + fbase, NN * 2u);
+#pragma unroll
+          for (int j = 0; j < PP; ++j) y[j] = bld_u16f(rm, (uint32_t)(tid + j * kThreads) * 2u);
+        } else {
+          const Rsrc rm = make_rsrc(static_cast<const double*>(T.frames) + fbase, NN * 8u);
+#pragma unroll
+          for (int j = 0; j < PP; ++j) y[j] = bld_f64f(rm, (uint32_t)(tid + j * kThreads) * 8u);
+        }
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+          const int lp = (row0 + j * RSTEP) * LD + col;
+          const float2 fv = lds[lp];
+          const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
+          const float r = fsqrt(fmaxf(gain_f * pred, 0.0f)) - fsqrt(fmaxf(y[j], 0.0f));  // point_loss, b = 0
+          lsum_f += r * r;
+          const float target = fmaxf(y[j] * inv_gain, 0.0f);
+          const bool big = pred > eps_f;
+          const float q = fsqrt(big ? target * frcp(pred) : target);
+          lds[lp] = big ? f2(fv.x * (q - 1.0f), fv.y * (q - 1.0f)) : f2(q - fv.x, -fv.y);
+        }
+        lcnt_i = PP;
+      } else {
         constexpr int PP = (NN + kThreads - 1) / kThreads;  // pixels per thread
         float y[PP];
         // issue every measurement load of the thread before using any of them
