@@ -726,6 +726,36 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
   }
 }
 
+// Per-LED record staged in LDS, as 32-bit words: the source's crop
+// parameters (SrcParam), the frame weight and gain, and the frame index.
+// Each LED's record is fetched from global memory during the previous LED
+// (double buffer), so no LED starts with a wait on global loads.
+constexpr int kRecSrc = (int)(sizeof(SrcParam) / 4);
+constexpr int kRecWords = kRecSrc + 5;  // + weight (2), gain (2), frame (1)
+constexpr int kRecPitch = 24;           // words per slot
+static_assert(sizeof(SrcParam) % 4 == 0 && kRecWords <= kRecPitch, "LED record");
+
+__device__ __forceinline__ uint32_t frame_at(const TileDesc& T, uint32_t pos) {
+  return T.order ? gptr(T.order)[pos] : pos;
+}
+// Word t (< kRecWords) of the record of frame f.
+__device__ __forceinline__ uint32_t rec_word(const TileDesc& T, uint32_t f, int t) {
+  if (t < kRecSrc) return gptr(reinterpret_cast<const uint32_t*>(T.src + f))[t];
+  if (t < kRecSrc + 2) return gptr(reinterpret_cast<const uint32_t*>(T.weights + f))[t - kRecSrc];
+  if (t < kRecSrc + 4) return gptr(reinterpret_cast<const uint32_t*>(T.gains + f))[t - kRecSrc - 2];
+  return f;
+}
+__device__ __forceinline__ double rec_double(const uint32_t* r, int t) {
+  return __hiloint2double((int)__builtin_amdgcn_readfirstlane(r[t + 1]), (int)__builtin_amdgcn_readfirstlane(r[t]));
+}
+__device__ __forceinline__ SrcParam rec_src(const uint32_t* r) {
+  SrcParam sp;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&sp);
+#pragma unroll
+  for (int t = 0; t < kRecSrc; ++t) w[t] = r[t];
+  return sp;
+}
+
 // One sweep of one tile. Thread t owns a strip of centred pixels (rows
 // 8*(t/N) .. +7 of column t%N for N = 128) and their swap partners
 // ((r + N/2) mod N, (c + N/2) mod N): for even N, fftshift and ifftshift are
@@ -733,7 +763,7 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
 // inside the owning thread, and vertically adjacent pixels share bilinear taps.
 template <int N, int RADIX>
 __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi,
-                                           double* red, const uint32_t* ext) {
+                                           double* red, const uint32_t* ext, uint32_t* rec) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int LD = pitch<N>();
@@ -750,19 +780,31 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   // the common case: every pixel counts, no background, amplitude MSE
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   uint64_t counter = *T.counter;
+  if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
+  __syncthreads();
+  uint32_t slot = 0;
   for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
     double obj_sum = 0.0, w_sum = 0.0;
-    for (uint32_t pos = 0; pos < T.F; ++pos) {
-      const uint32_t f = __builtin_amdgcn_readfirstlane(T.order ? T.order[pos] : pos);  // uniform
-      const double fw = T.weights[f];
+    for (uint32_t pos = 0; pos < T.F; ++pos, slot ^= 1) {
+      const uint32_t* r = rec + slot * kRecPitch;
+      uint32_t* rn = rec + (slot ^ 1) * kRecPitch;
+      const uint32_t f = __builtin_amdgcn_readfirstlane(r[kRecWords - 1]);
+      const double fw = rec_double(r, kRecSrc);
+      // the next LED: the first of the next sweep after the last
+      const bool more = pos + 1 < T.F || sweep + 1 < T.sweeps;
+      const uint32_t npos = pos + 1 < T.F ? pos + 1 : 0;
+      const bool fetch = more && threadIdx.x < kRecWords;
       const int tid = opaque_tid();
       if (fw == 0.0) {
         if (tid == 0) T.frame_loss[pos] = 0.0;
+        if (fetch) rn[threadIdx.x] = rec_word(T, frame_at(T, npos), threadIdx.x);
+        __syncthreads();
         continue;
       }
-      const SrcParam sp = T.src[f];
-      const float gain_f = (float)T.gains[f];
-      const float inv_gain = (float)(1.0 / T.gains[f]);
+      const SrcParam sp = rec_src(r);
+      const double gain = rec_double(r, kRecSrc + 2);
+      const float gain_f = (float)gain;
+      const float inv_gain = (float)(1.0 / gain);
       // 1. patch x pupil on a strip and its partner strip, written swapped (ifftshift)
       if (tid < COLS_THREADS) {
         const int c = tid % N, r0 = (tid / N) * STRIP;
@@ -784,8 +826,13 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         }
       }
       __syncthreads();
+      // the next LED's frame index, then its record words (written to the
+      // other slot after the forward transform)
+      uint32_t next_f = 0, next_w = 0;
+      if (fetch) next_f = frame_at(T, npos);
       // 2. inverse FFT (unscaled)
       fft2_lds<N, true, RADIX>(lds, twi, tid);
+      if (fetch) next_w = rec_word(T, next_f, threadIdx.x);
       // 3. amplitude replacement and objective terms (pointwise, natural order).
       //    The LDS field becomes psi' - psi: the forward transform is linear,
       //    so FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
@@ -856,6 +903,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       __syncthreads();
       // 4. forward FFT (x 1/N applied in phase 5)
       fft2_lds<N, false, RADIX>(lds, twf, tid);
+      if (fetch) rn[threadIdx.x] = next_w;
       // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
       if (tid < COLS_THREADS) {
         const int c = tid % N, r0 = (tid / N) * STRIP;
@@ -1236,6 +1284,7 @@ __global__ void __launch_bounds__(kThreads)
   double* red = reinterpret_cast<double*>(twi + N);
   int* bflag = reinterpret_cast<int*>(red + 2 * (kThreads / 64));
   uint32_t* ext = reinterpret_cast<uint32_t*>(bflag + 4);
+  uint32_t* rec = ext + N;  // two LED records
   const float2* src = tw_all + tw_offset(N);
   for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) twf[j] = src[j];
   for (int j = threadIdx.x; j < N; j += blockDim.x) ext[j] = T.extent[j];
@@ -1243,23 +1292,23 @@ __global__ void __launch_bounds__(kThreads)
   switch (N) {
     case 32:
       if (T.plan == 4)
-        sweep_tile<32, 4>(T, lds, twf, twi, red, ext);
+        sweep_tile<32, 4>(T, lds, twf, twi, red, ext, rec);
       else
-        sweep_tile<32, 8>(T, lds, twf, twi, red, ext);
+        sweep_tile<32, 8>(T, lds, twf, twi, red, ext, rec);
       break;
     case 64:
       if (T.plan == 4)
-        sweep_tile<64, 4>(T, lds, twf, twi, red, ext);
+        sweep_tile<64, 4>(T, lds, twf, twi, red, ext, rec);
       else if (T.plan == 16)
-        sweep_tile<64, 16>(T, lds, twf, twi, red, ext);
+        sweep_tile<64, 16>(T, lds, twf, twi, red, ext, rec);
       else
-        sweep_tile<64, 8>(T, lds, twf, twi, red, ext);
+        sweep_tile<64, 8>(T, lds, twf, twi, red, ext, rec);
       break;
     case 128:
       if (T.plan == 8)
-        sweep_tile<128, 8>(T, lds, twf, twi, red, ext);
+        sweep_tile<128, 8>(T, lds, twf, twi, red, ext, rec);
       else
-        sweep_tile<128, 16>(T, lds, twf, twi, red, ext);
+        sweep_tile<128, 16>(T, lds, twf, twi, red, ext, rec);
       break;
     case 256:
       if (T.members > 1)
@@ -1276,7 +1325,8 @@ __global__ void __launch_bounds__(kThreads)
 
 size_t sweep_lds_bytes(uint32_t nmax) {
   return ((size_t)sweep_field_elems(nmax) + (nmax > kFusedLdsMaxN ? nmax : 0) + 2 * (size_t)nmax) * sizeof(float2) +
-         2 * (kThreads / 64) * sizeof(double) + 16 + (size_t)nmax * sizeof(uint32_t);
+         2 * (kThreads / 64) * sizeof(double) + 16 + (size_t)nmax * sizeof(uint32_t) +
+         2 * kRecPitch * sizeof(uint32_t);
 }
 
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
