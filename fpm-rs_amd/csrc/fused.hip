@@ -361,11 +361,13 @@ constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, 
 __device__ __forceinline__ Rsrc make_rsrc(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
 }
+template <int AUX = 0>
 __device__ __forceinline__ double2 bld_d2(Rsrc r, uint32_t byte_off) {
   typedef int v4i __attribute__((ext_vector_type(4)));
-  const v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+  const v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, AUX);
   return make_double2(__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z));
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_d2(Rsrc r, uint32_t byte_off, double2 d) {
   typedef int v4i __attribute__((ext_vector_type(4)));
   v4i v;
@@ -373,7 +375,7 @@ __device__ __forceinline__ void bst_d2(Rsrc r, uint32_t byte_off, double2 d) {
   v.y = __double2hiint(d.x);
   v.z = __double2loint(d.y);
   v.w = __double2hiint(d.y);
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, byte_off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, byte_off, 0, AUX);
 }
 __device__ __forceinline__ float bld_u16f(Rsrc r, uint32_t byte_off) {
   return (float)(unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0);
@@ -383,18 +385,25 @@ __device__ __forceinline__ float bld_f64f(Rsrc r, uint32_t byte_off) {
   const v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
   return (float)__hiloint2double(v.y, v.x);
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bst_f2(Rsrc r, uint32_t byte_off, float2 f) {
   typedef int v2i __attribute__((ext_vector_type(2)));
   v2i v;
   v.x = __float_as_int(f.x);
   v.y = __float_as_int(f.y);
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ float2 bld_f2(Rsrc r, uint32_t byte_off) {
   typedef int v2i __attribute__((ext_vector_type(2)));
-  const v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  const v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, AUX);
   return make_float2(__int_as_float(v.x), __int_as_float(v.y));
 }
+
+// Cache policy bits of a buffer access under a memory policy: the coherent
+// policy's accesses carry sc1, as its device-scope atomics do.
+template <class MEM>
+constexpr int kAux = MEM::kCoherent ? 16 : 0;
 
 // Measured value of one pixel (uint16 counts or f64).
 __device__ __forceinline__ float measured(const TileDesc& T, uint64_t i) {
@@ -438,61 +447,31 @@ __device__ __forceinline__ void momentum_element(const TileDesc& T, double2& o, 
 template <class MEM = PlainMem>
 __device__ __forceinline__ void momentum_event(const TileDesc& T, int tid, uint32_t member = 0,
                                                uint32_t members = 1) {
+  constexpr int AUX = kAux<MEM>;
   const uint32_t M = T.rh * T.rw;
-  double2* __restrict__ spec = T.spec;
-  float2* __restrict__ vel = T.vel;
-  float2* __restrict__ anc = T.anc;
   bool bad = false;
   constexpr int U = 4;  // loads of U elements in flight before their use
-  if constexpr (!MEM::kCoherent) {
-    // branch-free: elements past M load zero and store nothing
-    const Rsrc rs = make_rsrc(spec, M * 16u), rv = make_rsrc(vel, M * 8u), ra = make_rsrc(anc, M * 8u);
-    for (uint32_t base = tid; base < M; base += U * kThreads) {
-      double2 o[U];
-      float2 v[U], an[U];
-      uint32_t idx[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t i = base + u * kThreads;
-        idx[u] = i < M ? i : kSkip / 16u;
-        o[u] = bld_d2(rs, i < M ? i * 16u : kSkip);
-        v[u] = bld_f2(rv, i < M ? i * 8u : kSkip);
-        an[u] = bld_f2(ra, i < M ? i * 8u : kSkip);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        momentum_element(T, o[u], v[u], an[u], bad);
-        const bool live = idx[u] != kSkip / 16u;
-        bst_f2(rv, live ? idx[u] * 8u : kSkip, v[u]);
-        bst_d2(rs, live ? idx[u] * 16u : kSkip, o[u]);
-        bst_f2(ra, live ? idx[u] * 8u : kSkip, an[u]);
-      }
-    }
-    if (bad) T.status[0] = 1;
-    __syncthreads();
-    return;
-  }
+  // branch-free: elements past M load zero and store nothing
+  const Rsrc rs = make_rsrc(T.spec, M * 16u), rv = make_rsrc(T.vel, M * 8u), ra = make_rsrc(T.anc, M * 8u);
   for (uint32_t base = member * U * kThreads + tid; base < M; base += members * U * kThreads) {
     double2 o[U];
     float2 v[U], an[U];
+    bool live[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t i = base + u * kThreads;
-      if (i < M) {
-        o[u] = MEM::ld(spec + i);
-        v[u] = MEM::ld(vel + i);
-        an[u] = MEM::ld(anc + i);
-      }
+      live[u] = i < M;
+      o[u] = bld_d2<AUX>(rs, live[u] ? i * 16u : kSkip);
+      v[u] = bld_f2<AUX>(rv, live[u] ? i * 8u : kSkip);
+      an[u] = bld_f2<AUX>(ra, live[u] ? i * 8u : kSkip);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t i = base + u * kThreads;
-      if (i < M) {
-        momentum_element(T, o[u], v[u], an[u], bad);
-        MEM::st(vel + i, v[u]);
-        MEM::st(spec + i, o[u]);
-        MEM::st(anc + i, an[u]);
-      }
+      momentum_element(T, o[u], v[u], an[u], bad);
+      bst_f2<AUX>(rv, live[u] ? i * 8u : kSkip, v[u]);
+      bst_d2<AUX>(rs, live[u] ? i * 16u : kSkip, o[u]);
+      bst_f2<AUX>(ra, live[u] ? i * 8u : kSkip, an[u]);
     }
   }
   if (bad) T.status[0] = 1;
@@ -558,7 +537,7 @@ __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, c
 // rows of taps are all independent of each other and of loaded values.
 // `pofs` is the pupil element index of the strip's first pixel. Inside the
 // extents the arithmetic is gather_strip's.
-template <int STRIP, class Put>
+template <int STRIP, int AUX = 0, class Put>
 __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
                                                int c, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du, double wll,
                                                double wlu, double wul, double wuu, Put put) {
@@ -586,8 +565,8 @@ __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t p
       const bool need = (t < STRIP && in_extent(ext[t], c)) || (du != 0 && t > 0 && in_extent(ext[t - 1], c));
       const uint32_t o = off + t * rw;
       const uint32_t ol = need ? o * 16u : kSkip;
-      lo[k] = bld_d2(rspec, ol);
-      hi[k] = bld_d2(rspec, dc != 0 ? ol + 16u : kSkip);
+      lo[k] = bld_d2<AUX>(rspec, ol);
+      hi[k] = bld_d2<AUX>(rspec, dc != 0 ? ol + 16u : kSkip);
     }
     clo = lo[SUB];
     chi = hi[SUB];
@@ -604,14 +583,14 @@ __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t p
 }
 
 // copy_strip through buffer loads, without branches (see gather_strip_b).
-template <int STRIP, class Put>
+template <int STRIP, int AUX = 0, class Put>
 __device__ __forceinline__ void copy_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
                                              int c, uint32_t off, uint32_t rw, Put put) {
 #pragma unroll
   for (int j = 0; j < STRIP; ++j) {
     const bool in = in_extent(ext[j], c);
     const float2 P = bld_f2(rpup, in ? (pofs + j * pstride) * 8u : kSkip);
-    const double2 v = bld_d2(rspec, in ? (off + j * rw) * 16u : kSkip);
+    const double2 v = bld_d2<AUX>(rspec, in ? (off + j * rw) * 16u : kSkip);
     put(j, fmul(f2((float)v.x, (float)v.y), P));
   }
 }
@@ -640,7 +619,7 @@ __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, con
 // coordinates) and is read-modified-written in FP64 only when that sum is
 // non-zero. U is zero outside the pupil's support extents `ext`, so pixels
 // beyond them are skipped without reading U.
-template <int N, int THREADS, class MEM = PlainMem, class GetU>
+template <int N, int THREADS, class MEM = PlainMem, int UNROLL = 4, class GetU>
 __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
                                               uint32_t rw, double scale, int tid, int a0, int a1, const uint32_t* ext,
                                               GetU u) {
@@ -664,7 +643,6 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
   const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
               wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
   const int total = (a1 - a0) * ww;
-  constexpr int UNROLL = 4;
   for (int base = tid; base < total; base += UNROLL * THREADS) {
     float2 add[UNROLL];
     uint32_t off[UNROLL];
@@ -703,26 +681,16 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
       add[k] = f2(ax, ay);
       hit[k] = qd < total && (ax != 0.0f || ay != 0.0f);
     }
+    // branch-free: every read of the group is in flight at once
+    constexpr int AUX = kAux<MEM>;
+    const Rsrc rs = make_rsrc(spec, spec_elems * 16u);
     double2 old[UNROLL];
-    if constexpr (!MEM::kCoherent) {
-      // branch-free: every read of the group is in flight at once
-      const Rsrc rs = make_rsrc(spec, spec_elems * 16u);
 #pragma unroll
-      for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2(rs, hit[k] ? off[k] * 16u : kSkip);
+    for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2<AUX>(rs, hit[k] ? off[k] * 16u : kSkip);
 #pragma unroll
-      for (int k = 0; k < UNROLL; ++k)
-        bst_d2(rs, hit[k] ? off[k] * 16u : kSkip,
-               make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
-    } else {
-#pragma unroll
-      for (int k = 0; k < UNROLL; ++k)
-        if (hit[k]) old[k] = MEM::ld(spec + off[k]);
-#pragma unroll
-      for (int k = 0; k < UNROLL; ++k)
-        if (hit[k])
-          MEM::st(spec + off[k],
+    for (int k = 0; k < UNROLL; ++k)
+      bst_d2<AUX>(rs, hit[k] ? off[k] * 16u : kSkip,
                   make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
-    }
   }
 }
 
@@ -1078,7 +1046,8 @@ __host__ __device__ constexpr uint32_t sweep_field_elems(uint32_t n) {
 
 template <int N, class MEM>
 __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, float2* prev, const float2* twf,
-                                               const float2* twi, double* red, int* bflag, const uint32_t* ext) {
+                                               const float2* twi, double* red, int* bflag, const uint32_t* ext,
+                                               uint32_t* rec) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int CH = BigPlan<N>::CH;
@@ -1100,20 +1069,38 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   const float2* __restrict__ pupil = T.pupil;
   const float eps_f = (float)T.eps;
   const Denominator den = make_denominator(T, fmax(T.max_p, T.eps));
+  constexpr int AUX = kAux<MEM>;
+  const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
+  const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   uint64_t counter = *T.counter;
+  if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
+  __syncthreads();
+  uint32_t slot = 0;
   for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
     double obj_sum = 0.0, w_sum = 0.0;
-    for (uint32_t pos = 0; pos < T.F; ++pos) {
-      const uint32_t f = T.order ? T.order[pos] : pos;
-      const double fw = T.weights[f];
+    for (uint32_t pos = 0; pos < T.F; ++pos, slot ^= 1) {
+      const uint32_t* r = rec + slot * kRecPitch;
+      uint32_t* rn = rec + (slot ^ 1) * kRecPitch;
+      const uint32_t f = __builtin_amdgcn_readfirstlane(r[kRecWords - 1]);
+      const double fw = rec_double(r, kRecSrc);
+      const bool more = pos + 1 < T.F || sweep + 1 < T.sweeps;
+      const uint32_t npos = pos + 1 < T.F ? pos + 1 : 0;
+      const bool fetch = more && threadIdx.x < kRecWords;
       int tid = opaque_tid();
       if (fw == 0.0) {
         if (lead && tid == 0) T.frame_loss[pos] = 0.0;
+        if (fetch) rn[threadIdx.x] = rec_word(T, frame_at(T, npos), threadIdx.x);
+        __syncthreads();
         continue;
       }
-      const SrcParam sp = T.src[f];
-      const float gain_f = (float)T.gains[f];
-      const float inv_gain = (float)(1.0 / T.gains[f]);
+      const SrcParam sp = rec_src(r);
+      const double gain = rec_double(r, kRecSrc + 2);
+      const float gain_f = (float)gain;
+      const float inv_gain = (float)(1.0 / gain);
+      // the next LED's frame index (during A), then its record words (during
+      // B), written to the other slot after B
+      uint32_t next_f = 0, next_w = 0;
+      if (fetch) next_f = frame_at(T, npos);
       // A. shifted row r' holds centred row r' ^ HALF (ifftshift for even N)
       for (int k = first; k < NCH; k += step) {
         tid = opaque_tid();
@@ -1123,14 +1110,13 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           const int cs = c ^ HALF;
           auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
           if (sp.zero) {
-            copy_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, ext + rc, c,
-                                   ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw, put);
+            copy_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, c,
+                                     ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw, put);
           } else {
-            gather_strip<STRIP, MEM>(spec, pupil + rc * N + c, N, ext + rc, c,
-                                     (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
-                                     sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw,
-                                     sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl,
-                                     sp.ar.wu * sp.ac.wu, put);
+            gather_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, c,
+                                       (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw, sp.ac.upper - sp.ac.lower,
+                                       (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu,
+                                       sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
           }
         }
         __syncthreads();
@@ -1144,6 +1130,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       const uint64_t fbase = (uint64_t)f * NN;
       double lsum = 0.0;
       int lcnt = 0;
+      if (fetch) next_w = rec_word(T, next_f, threadIdx.x);
       for (int jc = first; jc < NCH; jc += step) {
         tid = opaque_tid();
         const int c0 = jc * CH;
@@ -1151,7 +1138,38 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         __syncthreads();
         stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
         stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
-        {
+        if (plain_frames) {
+          // branch-free (as in sweep_tile): pixel (row0 + j RSTEP, c0 + col)
+          constexpr int PP = N * CH / kThreads, RSTEP = kThreads / CH;
+          const int tid = opaque_tid();
+          const int row0 = tid / CH, col = tid % CH;
+          float y[PP];
+          const uint32_t p0 = (uint32_t)(row0 * N + c0 + col);
+          if (T.is_u16) {
+            const Rsrc rm = make_rsrc(static_cast<const uint16_t*>(T.frames) + fbase, NN * 2u);
+#pragma unroll
+            for (int j = 0; j < PP; ++j) y[j] = bld_u16f(rm, (p0 + j * RSTEP * N) * 2u);
+          } else {
+            const Rsrc rm = make_rsrc(static_cast<const double*>(T.frames) + fbase, NN * 8u);
+#pragma unroll
+            for (int j = 0; j < PP; ++j) y[j] = bld_f64f(rm, (p0 + j * RSTEP * N) * 8u);
+          }
+          float lsum_f = 0.0f;
+#pragma unroll
+          for (int j = 0; j < PP; ++j) {
+            const int lp = (row0 + j * RSTEP) * LDC + col;
+            const float2 fv = lds[lp];
+            const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
+            const float rr = fsqrt(fmaxf(gain_f * pred, 0.0f)) - fsqrt(fmaxf(y[j], 0.0f));
+            lsum_f += rr * rr;
+            const float target = fmaxf(y[j] * inv_gain, 0.0f);
+            const bool big = pred > eps_f;
+            const float qv = fsqrt(big ? target * frcp(pred) : target);
+            lds[lp] = big ? f2(fv.x * (qv - 1.0f), fv.y * (qv - 1.0f)) : f2(qv - fv.x, -fv.y);
+          }
+          lsum += (double)lsum_f;
+          lcnt += PP;
+        } else {
           // measurement loads in groups of G issued before their use
           constexpr int PP = N * CH / kThreads;
           constexpr int G = 8;
@@ -1181,6 +1199,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         copy_out<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid());
         __syncthreads();
       }
+      if (fetch) rn[threadIdx.x] = next_w;
       {
         const double2 lsc = wg_sum2(lsum, (double)lcnt, red);
         double ls = lsc.x, lc = lsc.y;
@@ -1217,7 +1236,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         for (int i = tid; i < CH * N; i += kThreads) {
           const int lr = i / N, kc = i % N;
           const bool in = in_extent(ext[q * CH + lr], kc ^ HALF);  // U is zero outside the support
-          const float2 P = in ? pupil[(q * CH + lr) * N + (kc ^ HALF)] : f2(0.0f, 0.0f);
+          const float2 P = bld_f2(rpup, in ? (uint32_t)((q * CH + lr) * N + (kc ^ HALF)) * 8u : kSkip);
           const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
           const float2 u = fmulc(P, lds[lr * LDR + kc]);
           lds[lr * LDR + kc] = in ? f2(u.x * kk, u.y * kk) : f2(0.0f, 0.0f);
@@ -1312,9 +1331,9 @@ __global__ void __launch_bounds__(kThreads)
       break;
     case 256:
       if (T.members > 1)
-        sweep_tile_big<256, CoherentMem>(T, lds, prev, twf, twi, red, bflag, ext);
+        sweep_tile_big<256, CoherentMem>(T, lds, prev, twf, twi, red, bflag, ext, rec);
       else
-        sweep_tile_big<256, PlainMem>(T, lds, prev, twf, twi, red, bflag, ext);
+        sweep_tile_big<256, PlainMem>(T, lds, prev, twf, twi, red, bflag, ext, rec);
       break;
     default:
       break;
