@@ -55,6 +55,7 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.anc = s.b.anc.as<float2>();
   t.pupil = s.b.pupil.as<float2>();
   t.extent = s.b.extent.as<uint32_t>();
+  t.dest = s.b.dest.as<uint32_t>();
   t.field = m.nh > kFusedLdsMaxN ? s.b.field.as<float2>() : nullptr;  // the session's N-pixel field buffer
   t.group = s.b.group.as<uint64_t>();
   t.partial = s.b.group.ptr ? reinterpret_cast<double2*>(s.b.group.as<char>() + 64) : nullptr;

@@ -694,6 +694,69 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
   }
 }
 
+// Adjoint insert of one patch update U (crop.rs:188-256) over the tile's
+// list of live destinations `dl` (count nd; a | b << 16 in row-major order,
+// (a, b) in the (n+1)^2 window): each destination sums its <= 4 weighted taps
+// of U (u(row, col) in centred patch coordinates) and is read-modified-
+// written in FP64 when that sum is non-zero. The list holds every window
+// pixel that a tap of the support can reach, so no live destination is
+// missed and no thread iterates over dead ones.
+template <int N, class GetU>
+__device__ __forceinline__ void insert_list(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
+                                            uint32_t rw, double scale, int tid, const uint32_t* dl, uint32_t nd,
+                                            GetU u) {
+  const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
+  const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
+              wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
+  const Rsrc rs = make_rsrc(spec, spec_elems * 16u), rd = make_rsrc(dl, nd * 4u);
+  constexpr int UNROLL = 4;
+  for (uint32_t base = (uint32_t)tid; base < nd; base += UNROLL * kThreads) {
+    uint32_t e[UNROLL];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+      const uint32_t i = base + k * kThreads;
+      e[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rd, i < nd ? i * 4u : kSkip, 0, 0);
+    }
+    float2 add[UNROLL];
+    uint32_t off[UNROLL];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) {
+      const int a = (int)(e[k] & 0xFFFFu), b = (int)(e[k] >> 16);
+      float ax = 0.0f, ay = 0.0f;
+      auto tap = [&](int sr_, int sc_, float wt) {
+        if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
+          const float2 v = u(sr_, sc_);
+          ax = fmaf(v.x, wt, ax);
+          ay = fmaf(v.y, wt, ay);
+        }
+      };
+      if (fr && fc) {
+        tap(a - 1, b - 1, wuu);
+        tap(a - 1, b, wul);
+        tap(a, b - 1, wlu);
+        tap(a, b, wll);
+      } else if (fr) {
+        tap(a - 1, b, wul);
+        tap(a, b, wll);
+      } else if (fc) {
+        tap(a, b - 1, wlu);
+        tap(a, b, wll);
+      } else {
+        tap(a, b, wll);
+      }
+      add[k] = f2(ax, ay);
+      const bool hit = base + k * kThreads < nd && (ax != 0.0f || ay != 0.0f);
+      off[k] = hit ? ((sp.ar.lower + (uint32_t)a) * rw + sp.ac.lower + (uint32_t)b) * 16u : kSkip;
+    }
+    double2 old[UNROLL];
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2(rs, off[k]);
+#pragma unroll
+    for (int k = 0; k < UNROLL; ++k)
+      bst_d2(rs, off[k], make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
+  }
+}
+
 // Per-LED record staged in LDS, as 32-bit words: the source's crop
 // parameters (SrcParam), the frame weight and gain, and the frame index.
 // Each LED's record is fetched from global memory during the previous LED
@@ -747,6 +810,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
   // the common case: every pixel counts, no background, amplitude MSE
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
+  const uint32_t ndest = __builtin_amdgcn_readfirstlane(gptr(T.dest)[0]);
   uint64_t counter = *T.counter;
   if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
   __syncthreads();
@@ -893,33 +957,11 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         }
       }
       __syncthreads();
-      // 6. adjoint insert: gather over the destination window, FP64 RMW of the
+      // 6. adjoint insert over the tile's live destinations: FP64 RMW of the
       //    pixels that receive a non-zero update (U is zero outside the pupil)
       const double scale = fw * T.object_step;
-      if (sp.zero) {
-        // branch-free RMW in groups of G pixels per thread
-        constexpr int G = NN / kThreads < 8 ? (NN / kThreads > 0 ? NN / kThreads : 1) : 8;
-        for (int p0 = tid; p0 < NN; p0 += G * kThreads) {
-          float2 u[G];
-          uint32_t off[G];
-          double2 d[G];
-#pragma unroll
-          for (int k = 0; k < G; ++k) {
-            const int p = p0 + k * kThreads, r = p / N, c = p % N;
-            u[k] = p < NN ? lds[r * LD + c] : f2(0.0f, 0.0f);
-            off[k] = (u[k].x != 0.0f || u[k].y != 0.0f) ? (uint32_t)(((sp.sr + r) * T.rw + sp.sc + c) * 16u) : kSkip;
-            d[k] = bld_d2(rspec, off[k]);
-          }
-#pragma unroll
-          for (int k = 0; k < G; ++k)
-            bst_d2(rspec, off[k],
-                   make_double2(fma(scale, (double)u[k].x, d[k].x), fma(scale, (double)u[k].y, d[k].y)));
-        }
-      } else {
-        insert_window<N, kThreads>(spec, T.rh * T.rw, sp, T.rw, scale, tid, 0,
-                                   N + ((sp.ar.upper != sp.ar.lower) ? 1 : 0), ext,
-                                   [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
-      }
+      insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 1, ndest,
+                     [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
       __syncthreads();
       // 7. mPIE momentum over the whole tile spectrum (common.rs:308-387)
       if (T.momentum) {

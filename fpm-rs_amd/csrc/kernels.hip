@@ -624,12 +624,55 @@ __global__ void k_pupil_extent(const float2* __restrict__ pupil, uint32_t h, uin
   extent[r] = lo | (hi << 16);
 }
 
+// Live destinations of the fused sweep's adjoint insert for a square n x n
+// tile: window row a (0..n) can receive a non-zero update only in columns
+// [lo, hi + 1) (clamped to n + 1), where [lo, hi) is the union of the
+// support extents of pupil rows a - 1 and a: the bilinear taps of a
+// destination reach back one row and one column, and the update U is zero
+// outside the support. dest[0] = count, then a | b << 16 in row-major order.
+constexpr uint32_t kDestMaxRows = 129;
+__global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, uint32_t* __restrict__ dest) {
+  __shared__ uint32_t lo_s[kDestMaxRows], len_s[kDestMaxRows], start_s[kDestMaxRows + 1];
+  const uint32_t rows = n + 1;
+  for (uint32_t a = threadIdx.x; a < rows; a += blockDim.x) {
+    uint32_t lo = rows, hi = 0;
+    for (uint32_t r = (a > 0 ? a - 1 : 0); r <= a && r < n; ++r) {
+      const uint32_t e = extent[r], elo = e & 0xFFFFu, ehi = e >> 16;
+      if (ehi > elo) {
+        lo = elo < lo ? elo : lo;
+        hi = ehi > hi ? ehi : hi;
+      }
+    }
+    if (hi > lo) hi = hi + 1 < rows ? hi + 1 : rows;
+    lo_s[a] = lo;
+    len_s[a] = hi > lo ? hi - lo : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t a = 0; a < rows; ++a) {
+      start_s[a] = acc;
+      acc += len_s[a];
+    }
+    start_s[rows] = acc;
+    dest[0] = acc;
+  }
+  __syncthreads();
+  for (uint32_t a = 0; a < rows; ++a)
+    for (uint32_t j = threadIdx.x; j < len_s[a]; j += blockDim.x) dest[1 + start_s[a] + j] = a | ((lo_s[a] + j) << 16);
+}
+
 void launch_pupil_extent(const fpmhip_session& s) {
   const Model& m = *s.model;
   if (s.precision != FPMHIP_MIXED_F32 || !s.b.extent.ptr) return;
   hipLaunchKernelGGL(k_pupil_extent, dim3((m.nh + 63) / 64), dim3(64), 0, s.ctx->stream, s.b.pupil.as<float2>(),
                      (uint32_t)m.nh, (uint32_t)m.nw, s.b.extent.as<uint32_t>());
   HIP_CHECK(hipGetLastError());
+  if (s.b.dest.ptr && m.nh + 1 <= kDestMaxRows) {
+    hipLaunchKernelGGL(k_dest_table, dim3(1), dim3(256), 0, s.ctx->stream, s.b.extent.as<uint32_t>(), (uint32_t)m.nh,
+                       s.b.dest.as<uint32_t>());
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream) {
