@@ -736,7 +736,8 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     launch_convert_pupil(m->d_pupil64.as<double2>(), b.pupil.ptr, precision == FPMHIP_MIXED_F32, N, m->ctx->stream);
     if (precision == FPMHIP_MIXED_F32) {
       b.extent.allocate(m->nh * sizeof(uint32_t));
-      if (m->nh == m->nw && m->nh <= kFusedLdsMaxN) b.dest.allocate((1 + (m->nh + 1) * (m->nw + 1)) * sizeof(uint32_t));
+      if (m->nh == m->nw && m->nh <= kFusedLdsMaxN)
+        b.dest.allocate((2 + (m->nh + 1) * (m->nw + 1) + m->nh * m->nw) * sizeof(uint32_t));
       launch_pupil_extent(*s);
     }
     launch_flag_reset(*s);

@@ -694,6 +694,67 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
   }
 }
 
+// Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
+// pixels `sl` (count ns; r | c << 16, centred patch coordinates), written to
+// the N x N LDS field at the ifftshifted position; the rest of the field is
+// zero (written first, then a barrier). Each pixel's pupil value and four
+// spectrum taps are independent loads, G pixels per thread in flight; an
+// axis without a fraction has an upper weight of exactly zero and its
+// (skipped, zero) upper taps add nothing.
+template <int N, int G = 3>
+__device__ __forceinline__ void gather_list(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
+                                            const uint32_t* sl, uint32_t ns, float2* lds) {
+  constexpr int LD = pitch<N>(), HALF = N / 2, NN = N * N;
+  static_assert(NN % kThreads == 0, "pixels per thread");
+  {
+    const int row0 = tid / N, col = tid % N;
+#pragma unroll
+    for (int j = 0; j < NN / kThreads; ++j) lds[(row0 + j * (kThreads / N)) * LD + col] = f2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
+  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+               wuu = sp.ar.wu * sp.ac.wu;
+  const Rsrc rl = make_rsrc(sl, ns * 4u);
+  // list entries of the next group are loaded with the current group's taps
+  uint32_t en[G];
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    const uint32_t i = (uint32_t)tid + k * kThreads;
+    en[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i < ns ? i * 4u : kSkip, 0, 0);
+  }
+  for (uint32_t base = (uint32_t)tid; base < ns; base += G * kThreads) {
+    uint32_t e[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      e[k] = en[k];
+      const uint32_t i = base + (G + k) * kThreads;
+      en[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i < ns ? i * 4u : kSkip, 0, 0);
+    }
+    float2 P[G];
+    double2 ll[G], lu[G], ul[G], uu[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const bool live = base + k * kThreads < ns;
+      const uint32_t r = e[k] & 0xFFFFu, c = e[k] >> 16;
+      const uint32_t o = (sp.ar.lower + r) * rw + sp.ac.lower + c;
+      P[k] = bld_f2(rpup, live ? (r * N + c) * 8u : kSkip);
+      ll[k] = bld_d2(rspec, live ? o * 16u : kSkip);
+      lu[k] = bld_d2(rspec, live && fc ? (o + 1) * 16u : kSkip);
+      ul[k] = bld_d2(rspec, live && fr ? (o + rw) * 16u : kSkip);
+      uu[k] = bld_d2(rspec, live && fr && fc ? (o + rw + 1) * 16u : kSkip);
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const double2 v = make_double2(fma(uu[k].x, wuu, fma(ul[k].x, wul, fma(lu[k].x, wlu, ll[k].x * wll))),
+                                     fma(uu[k].y, wuu, fma(ul[k].y, wul, fma(lu[k].y, wlu, ll[k].y * wll))));
+      const uint32_t r = e[k] & 0xFFFFu, c = e[k] >> 16;
+      if (base + k * kThreads < ns)
+        lds[((r + HALF) % N) * LD + (c + HALF) % N] = fmul(f2((float)v.x, (float)v.y), P[k]);
+    }
+  }
+}
+
 // Adjoint insert of one patch update U (crop.rs:188-256) over the tile's
 // list of live destinations `dl` (count nd; a | b << 16 in row-major order,
 // (a, b) in the (n+1)^2 window): each destination sums its <= 4 weighted taps
@@ -701,21 +762,37 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
 // written in FP64 when that sum is non-zero. The list holds every window
 // pixel that a tap of the support can reach, so no live destination is
 // missed and no thread iterates over dead ones.
+constexpr int kInsertGroup = 4;  // destinations per thread in flight
+
+// The first group of a thread's destination entries (issued ahead of the
+// insert, so that its read-modify-writes need not wait for them).
+__device__ __forceinline__ void insert_list_first(const uint32_t* dl, uint32_t nd, int tid,
+                                                  uint32_t (&en)[kInsertGroup]) {
+  const Rsrc rd = make_rsrc(dl, nd * 4u);
+#pragma unroll
+  for (int k = 0; k < kInsertGroup; ++k) {
+    const uint32_t i = (uint32_t)tid + k * kThreads;
+    en[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rd, i < nd ? i * 4u : kSkip, 0, 0);
+  }
+}
+
 template <int N, class GetU>
 __device__ __forceinline__ void insert_list(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
                                             uint32_t rw, double scale, int tid, const uint32_t* dl, uint32_t nd,
-                                            GetU u) {
+                                            uint32_t (&en)[kInsertGroup], GetU u) {
   const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
   const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
               wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
   const Rsrc rs = make_rsrc(spec, spec_elems * 16u), rd = make_rsrc(dl, nd * 4u);
-  constexpr int UNROLL = 4;
+  constexpr int UNROLL = kInsertGroup;
+  // list entries of the next group are loaded with the current group's reads
   for (uint32_t base = (uint32_t)tid; base < nd; base += UNROLL * kThreads) {
     uint32_t e[UNROLL];
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) {
-      const uint32_t i = base + k * kThreads;
-      e[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rd, i < nd ? i * 4u : kSkip, 0, 0);
+      e[k] = en[k];
+      const uint32_t i = base + (UNROLL + k) * kThreads;
+      en[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rd, i < nd ? i * 4u : kSkip, 0, 0);
     }
     float2 add[UNROLL];
     uint32_t off[UNROLL];
@@ -811,6 +888,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   // the common case: every pixel counts, no background, amplitude MSE
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   const uint32_t ndest = __builtin_amdgcn_readfirstlane(gptr(T.dest)[0]);
+  const uint32_t nsrc = __builtin_amdgcn_readfirstlane(gptr(T.dest)[1]);
   uint64_t counter = *T.counter;
   if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
   __syncthreads();
@@ -837,26 +915,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       const double gain = rec_double(r, kRecSrc + 2);
       const float gain_f = (float)gain;
       const float inv_gain = (float)(1.0 / gain);
-      // 1. patch x pupil on a strip and its partner strip, written swapped (ifftshift)
-      if (tid < COLS_THREADS) {
-        const int c = tid % N, r0 = (tid / N) * STRIP;
-        const int c1 = (c + HALF) % N;
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          const int cc = half ? c1 : c, rb = half ? r0 + HALF : r0;
-          const int cs = (cc + HALF) % N;
-          auto put = [&](int j, float2 v) { lds[((rb + j + HALF) % N) * LD + cs] = v; };
-          if (sp.zero) {
-            copy_strip_b<STRIP>(rspec, rpup, rb * N + cc, N, ext + rb, cc,
-                                ((uint32_t)sp.sr + rb) * T.rw + (uint32_t)sp.sc + cc, T.rw, put);
-          } else {
-            gather_strip_b<STRIP>(rspec, rpup, rb * N + cc, N, ext + rb, cc,
-                                  (sp.ar.lower + rb) * T.rw + sp.ac.lower + cc, T.rw, sp.ac.upper - sp.ac.lower,
-                                  (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu,
-                                  sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
-          }
-        }
-      }
+      // 1. patch x pupil over the support pixels, written ifftshifted
+      gather_list<N>(rspec, rpup, sp, T.rw, tid, T.dest + 2 + (N + 1) * (N + 1), nsrc, lds);
       __syncthreads();
       // the next LED's frame index, then its record words (written to the
       // other slot after the forward transform)
@@ -937,6 +997,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       fft2_lds<N, false, RADIX>(lds, twf, tid);
       if (fetch) rn[threadIdx.x] = next_w;
       // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
+      uint32_t dfirst[kInsertGroup];
+      insert_list_first(T.dest + 2, ndest, tid, dfirst);
       if (tid < COLS_THREADS) {
         const int c = tid % N, r0 = (tid / N) * STRIP;
         const int c1 = (c + HALF) % N;
@@ -960,7 +1022,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       // 6. adjoint insert over the tile's live destinations: FP64 RMW of the
       //    pixels that receive a non-zero update (U is zero outside the pupil)
       const double scale = fw * T.object_step;
-      insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 1, ndest,
+      insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 2, ndest, dfirst,
                      [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
       __syncthreads();
       // 7. mPIE momentum over the whole tile spectrum (common.rs:308-387)

@@ -624,15 +624,19 @@ __global__ void k_pupil_extent(const float2* __restrict__ pupil, uint32_t h, uin
   extent[r] = lo | (hi << 16);
 }
 
-// Live destinations of the fused sweep's adjoint insert for a square n x n
-// tile: window row a (0..n) can receive a non-zero update only in columns
-// [lo, hi + 1) (clamped to n + 1), where [lo, hi) is the union of the
-// support extents of pupil rows a - 1 and a: the bilinear taps of a
-// destination reach back one row and one column, and the update U is zero
-// outside the support. dest[0] = count, then a | b << 16 in row-major order.
+// Live-pixel lists of the fused sweep for a square n x n tile.
+// Destinations of the adjoint insert: window row a (0..n) can receive a
+// non-zero update only in columns [lo, hi + 1) (clamped to n + 1), where
+// [lo, hi) is the union of the support extents of pupil rows a - 1 and a:
+// the bilinear taps of a destination reach back one row and one column, and
+// the update U is zero outside the support. Support pixels: row r, columns
+// of its extent (the gather reads nothing elsewhere). Layout: dest[0] and
+// dest[1] the two counts, then the destinations a | b << 16 and the support
+// pixels r | c << 16, both row-major.
 constexpr uint32_t kDestMaxRows = 129;
 __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, uint32_t* __restrict__ dest) {
   __shared__ uint32_t lo_s[kDestMaxRows], len_s[kDestMaxRows], start_s[kDestMaxRows + 1];
+  __shared__ uint32_t slen_s[kDestMaxRows], sstart_s[kDestMaxRows];
   const uint32_t rows = n + 1;
   for (uint32_t a = threadIdx.x; a < rows; a += blockDim.x) {
     uint32_t lo = rows, hi = 0;
@@ -646,20 +650,36 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
     if (hi > lo) hi = hi + 1 < rows ? hi + 1 : rows;
     lo_s[a] = lo;
     len_s[a] = hi > lo ? hi - lo : 0;
+    if (a < n) {
+      const uint32_t e = extent[a], elo = e & 0xFFFFu, ehi = e >> 16;
+      slen_s[a] = ehi > elo ? ehi - elo : 0;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t acc = 0;
+    uint32_t acc = 0, sacc = 0;
     for (uint32_t a = 0; a < rows; ++a) {
       start_s[a] = acc;
       acc += len_s[a];
+      if (a < n) {
+        sstart_s[a] = sacc;
+        sacc += slen_s[a];
+      }
     }
     start_s[rows] = acc;
     dest[0] = acc;
+    dest[1] = sacc;
   }
   __syncthreads();
-  for (uint32_t a = 0; a < rows; ++a)
-    for (uint32_t j = threadIdx.x; j < len_s[a]; j += blockDim.x) dest[1 + start_s[a] + j] = a | ((lo_s[a] + j) << 16);
+  uint32_t* dl = dest + 2;
+  uint32_t* sl = dest + 2 + rows * rows;
+  for (uint32_t a = 0; a < rows; ++a) {
+    for (uint32_t j = threadIdx.x; j < len_s[a]; j += blockDim.x) dl[start_s[a] + j] = a | ((lo_s[a] + j) << 16);
+    if (a < n) {
+      const uint32_t elo = extent[a] & 0xFFFFu;
+      for (uint32_t j = threadIdx.x; j < slen_s[a]; j += blockDim.x) sl[sstart_s[a] + j] = a | ((elo + j) << 16);
+    }
+  }
 }
 
 void launch_pupil_extent(const fpmhip_session& s) {

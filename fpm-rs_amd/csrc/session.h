@@ -53,8 +53,10 @@ struct SessionBuffers {
   DeviceBuffer avg, aw;   // double[N] (init)
   DeviceBuffer group;     // cooperative-group state of the fused sweep (large MIXED_F32 tiles)
   DeviceBuffer extent;    // uint32[n_h] per-row support of the complex64 pupil (MIXED_F32)
-  DeviceBuffer dest;      // uint32[1 + (n+1)^2]: count, then the adjoint insert's live destinations
-                          // a | b << 16 (MIXED_F32 tiles up to the LDS-resident size)
+  DeviceBuffer dest;      // uint32[2 + (n+1)^2 + n^2] live-pixel lists of the fused sweep (MIXED_F32
+                          // tiles up to the LDS-resident size): [0] destinations of the adjoint
+                          // insert, [1] support pixels; then the destinations a | b << 16 and the
+                          // support pixels r | c << 16, row-major
   DeviceBuffer bg;        // double[F * N] background of the reconstruction state (bg_len used)
   DeviceBuffer gains;     // double[F] frame gains of the state (1 where the model has none)
   DeviceBuffer calib;     // double[nblk * 2] block partial sums of the calibration updates
