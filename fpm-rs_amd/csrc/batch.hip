@@ -24,6 +24,11 @@ int guarded(F&& body) {
   }
 }
 
+__global__ void k_set_sweeps(TileDesc* tiles, uint32_t count, uint32_t sweeps) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) tiles[i].sweeps = sweeps;
+}
+
 __global__ void k_reset_tiles(TileDesc* tiles, uint32_t count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count && tiles[i].member == 0) {
@@ -177,7 +182,7 @@ struct fpmhip_batch {
   std::vector<fpmhip_session*> sessions;
   std::vector<TileDesc> host_tiles;    // in session order
   std::vector<TileDesc> device_order;  // launch order (largest tiles first, groups expanded)
-  fpmhip_batch_schedule schedule{kGroupMembers, 1, {0, 0, 0, 0}};
+  fpmhip_batch_schedule schedule{kGroupMembers, 1, {0, 0, 0, 0}, 0};
   uint32_t launched = 0;               // workgroups per sweep launch
   bool cooperative = false;
   DeviceBuffer tiles;      // TileDesc[launched]
@@ -251,14 +256,27 @@ void initialize_tiles(fpmhip_batch& b) {
   HIP_CHECK(hipGetLastError());
 }
 
-// `sweeps` graph replays, each bracketed by HIP events on the batch stream.
-void replay(fpmhip_batch& b, uint64_t sweeps, double* mean, double* mx) {
+constexpr uint32_t kDefaultSweepsPerLaunch = 10;
+
+// `sweeps` sweeps as graph replays of up to `per_launch` sweeps each (the
+// sweep count of a launch is set in the device descriptors before it), each
+// replay bracketed by HIP events on the batch stream. mean / mx: per-sweep
+// time (launch time / its sweeps), averaged / the largest.
+void replay(fpmhip_batch& b, uint64_t sweeps, double* mean, double* mx, uint32_t per_launch = 1) {
   hipStream_t st = b.ctx->stream;
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   double total = 0.0, worst = 0.0;
-  for (uint64_t it = 0; it < sweeps; ++it) {
+  uint32_t current = 0;
+  for (uint64_t done = 0; done < sweeps;) {
+    const uint32_t k = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(per_launch, 1), sweeps - done);
+    if (k != current) {
+      hipLaunchKernelGGL(k_set_sweeps, dim3((b.launched + 255) / 256), dim3(256), 0, st, b.tiles.as<TileDesc>(),
+                         b.launched, k);
+      HIP_CHECK(hipGetLastError());
+      current = k;
+    }
     HIP_CHECK(hipEventRecord(e0, st));
     HIP_CHECK(hipGraphLaunch(b.exec, st));
     HIP_CHECK(hipEventRecord(e1, st));
@@ -266,7 +284,13 @@ void replay(fpmhip_batch& b, uint64_t sweeps, double* mean, double* mx) {
     float ms = 0.0f;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     total += ms;
-    worst = std::max(worst, (double)ms);
+    worst = std::max(worst, (double)ms / k);
+    done += k;
+  }
+  if (current != 1) {  // leave single-sweep launches in place
+    hipLaunchKernelGGL(k_set_sweeps, dim3((b.launched + 255) / 256), dim3(256), 0, st, b.tiles.as<TileDesc>(),
+                       b.launched, 1u);
+    HIP_CHECK(hipGetLastError());
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -283,6 +307,8 @@ void validate_schedule(const fpmhip_batch_schedule& s) {
   if (s.group_members != 1 && s.group_members != kGroupMembers)
     fail(FPMHIP_INVALID_PARAMETER, "group_members must be 1 or 4", "group_members");
   if (s.largest_first > 1) fail(FPMHIP_INVALID_PARAMETER, "largest_first must be 0 or 1", "largest_first");
+  if (s.sweeps_per_launch > 1000)
+    fail(FPMHIP_INVALID_PARAMETER, "sweeps_per_launch must be in 0..1000", "sweeps_per_launch");
   const uint32_t sizes[4] = {32, 64, 128, 256};
   for (int k = 0; k < 4; ++k) {
     if (s.radix[k] == 0) continue;
@@ -344,7 +370,9 @@ int fpmhip_batch_run(fpmhip_batch* b, uint64_t iterations, double* trace) {
     const uint32_t count = (uint32_t)b->sessions.size();
     if (iterations > b->trace_cap) build_tiles(*b, (uint32_t)iterations);
     initialize_tiles(*b);
-    replay(*b, iterations, &b->ms_mean, &b->ms_max);
+    const uint32_t per_launch =
+        b->schedule.sweeps_per_launch ? b->schedule.sweeps_per_launch : kDefaultSweepsPerLaunch;
+    replay(*b, iterations, &b->ms_mean, &b->ms_max, per_launch);
     b->sweeps = iterations;
     std::vector<char> per(count * 16);
     HIP_CHECK(hipMemcpyAsync(per.data(), b->per_tile.ptr, per.size(), hipMemcpyDeviceToHost, st));

@@ -364,18 +364,20 @@ class Batch:
     def schedule(self) -> dict:
         """The sweep launch's schedule: group_members (workgroups per 256^2
         tile), largest_first, radix (first-stage FFT radix for 32, 64, 128,
-        256 pixel tiles; 0 = default)."""
+        256 pixel tiles; 0 = default), sweeps_per_launch (0 = default)."""
         sch = BatchSchedule()
         check(lib().fpmhip_batch_get_schedule(self.handle, ct.byref(sch)))
-        return {"group_members": sch.group_members, "largest_first": sch.largest_first, "radix": list(sch.radix)}
+        return {"group_members": sch.group_members, "largest_first": sch.largest_first, "radix": list(sch.radix),
+                "sweeps_per_launch": sch.sweeps_per_launch}
 
-    def set_schedule(self, group_members=None, largest_first=None, radix=None):
+    def set_schedule(self, group_members=None, largest_first=None, radix=None, sweeps_per_launch=None):
         cur = self.schedule()
         sch = BatchSchedule()
         sch.group_members = cur["group_members"] if group_members is None else group_members
         sch.largest_first = cur["largest_first"] if largest_first is None else int(largest_first)
         for i, r in enumerate(cur["radix"] if radix is None else radix):
             sch.radix[i] = r
+        sch.sweeps_per_launch = cur["sweeps_per_launch"] if sweeps_per_launch is None else sweeps_per_launch
         check(lib().fpmhip_batch_set_schedule(self.handle, ct.byref(sch)))
 
     def tune(self, trial_sweeps=2) -> dict:
@@ -387,7 +389,8 @@ class Batch:
 
 
 class BatchSchedule(ct.Structure):
-    _fields_ = [("group_members", ct.c_uint32), ("largest_first", ct.c_uint32), ("radix", ct.c_uint32 * 4)]
+    _fields_ = [("group_members", ct.c_uint32), ("largest_first", ct.c_uint32), ("radix", ct.c_uint32 * 4),
+                ("sweeps_per_launch", ct.c_uint32)]
 
 
 class BatchTuning(ct.Structure):

@@ -186,6 +186,26 @@ def test_radix_plans_match_oracle(ctx, radix):
         assert ea <= 1e-5 and ep <= 1e-5
 
 
+def test_sweeps_per_launch_do_not_change_results(ctx):
+    """7 sweeps as launches of 1, 3 (3 + 3 + 1) and 10 sweeps each: the same
+    spectra and traces, bit for bit (a launch runs whole sweeps; mPIE's
+    counter and the trace carry across launches)."""
+    alg = Mpie(iterations=7, momentum_interval=11, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(64, 7, 141), tile(128, 5, 142), tile(256, 5, 143)]
+    out = []
+    for per in (1, 3, 10):
+        ss = sessions_for(tiles, alg)
+        b = runtime.Batch(ss)
+        b.set_schedule(sweeps_per_launch=per)
+        assert b.schedule()["sweeps_per_launch"] == per
+        tr = b.run(7)
+        out.append(([s.spectrum() for s in ss], tr))
+    for spec, tr in out[1:]:
+        for a, c in zip(out[0][0], spec):
+            assert np.array_equal(a, c)
+        assert np.array_equal(out[0][1], tr)
+
+
 def test_launch_order_does_not_change_results(ctx):
     alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
     tiles = [tile(64, 5, 111), tile(256, 5, 112), tile(32, 5, 113)]
@@ -224,7 +244,7 @@ def test_autotuner_keeps_a_valid_schedule(ctx):
 def test_invalid_schedule_is_rejected(ctx):
     ss = sessions_for([tile(64, 5, 131)], Fpie(iterations=1))
     b = runtime.Batch(ss)
-    for kw in ({"group_members": 3}, {"radix": [0, 32, 0, 0]}, {"largest_first": 2}):
+    for kw in ({"group_members": 3}, {"radix": [0, 32, 0, 0]}, {"largest_first": 2}, {"sweeps_per_launch": 1001}):
         with pytest.raises(runtime.FpmError) as e:
             b.set_schedule(**kw)
         assert e.value.kind == "InvalidParameter"
