@@ -30,15 +30,16 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_sweep_mpie_225led_128px_256t
 
 
 def measured_traffic():
-    """HBM bytes per k_sweep launch from the committed rocprofv3 counter
-    summary, when it was taken with this exact library build (else None)."""
+    """HBM bytes per sweep of every tile (k_sweep dispatch bytes / the sweeps
+    the dispatch ran) from the committed rocprofv3 counter summary, when it
+    was taken with this exact library build (else None)."""
     import hashlib
 
     try:
         lib = open(runtime.LIB_PATH, "rb").read()
         digest = hashlib.sha256(lib).hexdigest()
         for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith("traffic_bytes_per_sweep:"):
                 parts = line.split()
                 if parts[3] == digest:
                     return float(parts[1])
@@ -118,12 +119,12 @@ def run_batched(model, meas, iterations, steps, warmup, tiles):
     sweep_ms = []
     for i in range(steps):
         batch.run(iterations)
-        for s in sessions:
-            s.object()
+        batch.objects()  # every tile's result object, downloaded
         sweep_ms.append(batch.timing()[0])
         log(f"batch {i + 1}/{steps} done")
     ctx.synchronize()
-    return time.perf_counter() - t0, float(np.mean(sweep_ms))
+    per_launch = batch.schedule()["sweeps_per_launch"] or 10
+    return time.perf_counter() - t0, float(np.mean(sweep_ms)), min(per_launch, iterations)
 
 
 def serving(requests, rate, iterations, seed=5, max_tiles=256):
@@ -226,8 +227,7 @@ def run_uneven(iterations=40, steps=1, warmup=1):
     sweep_ms = []
     for i in range(steps):
         batch.run(iterations)
-        for s in sessions:
-            s.object()
+        batch.objects()  # every tile's result object, downloaded
         sweep_ms.append(batch.timing()[0])
         log(f"uneven batch {i + 1}/{steps} done")
     ctx.synchronize()
@@ -316,15 +316,18 @@ def main():
     B = tile_bytes(model, args.iterations, 30, 2, s_o, s_p)
     traffic = None
     if args.mode == "batched":
-        secs, sweep_ms = run_batched(model, meas, args.iterations, args.steps, args.warmup, args.tiles)
+        secs, sweep_ms, per_launch = run_batched(model, meas, args.iterations, args.steps, args.warmup, args.tiles)
         if args.tiles == 256:
             traffic = measured_traffic()
         tiles_per_s = args.steps * args.tiles / secs
-        # dominant kernel: the fused sweep (one launch = one sweep of every tile)
+        # dominant kernel: the fused sweep; one dispatch runs `per_launch`
+        # sweeps of every tile, and bytes and time are both per sweep here
         sweep_bytes = (B - model.frame_count * model.image_len * 2
                        - 4 * model.reconstruction_shape[0] * model.reconstruction_shape[1] * s_o) / args.iterations
         achieved = sweep_bytes * args.tiles / (sweep_ms * 1e-3) / 1e9
-        note = "fused sweep kernel: algorithmic bytes of one sweep of every tile / mean sweep duration (HIP events)"
+        note = (f"fused sweep kernel k_sweep, {per_launch} sweeps of every tile per dispatch: algorithmic bytes "
+                "of one sweep of every tile / mean per-sweep time (HIP events around each dispatch / its sweeps); "
+                "traffic is per sweep too")
     else:
         secs = run_plain(model, meas, args.iterations, args.steps, args.warmup, precision)
         tiles_per_s = args.steps / secs

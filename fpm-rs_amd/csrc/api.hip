@@ -289,15 +289,14 @@ void load_calibration(fpmhip_session& s) {
   const Model& m = *s.model;
   s.has_gain = m.has_gain;
   s.gain = m.gain;
-  std::vector<double> g(m.F, 1.0);
-  if (m.has_gain) g = m.gain;
+  // the model's device gains hold ones where it has none
   if (m.F)
-    HIP_CHECK(hipMemcpyAsync(s.b.gains.ptr, g.data(), m.F * sizeof(double), hipMemcpyHostToDevice, s.ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(s.b.gains.ptr, m.d_gains.ptr, m.F * sizeof(double), hipMemcpyDeviceToDevice,
+                             s.ctx->stream));
   s.bg_len = m.background.size();
   if (s.bg_len)
     HIP_CHECK(hipMemcpyAsync(s.b.bg.ptr, m.d_background.ptr, s.bg_len * sizeof(double), hipMemcpyDeviceToDevice,
                              s.ctx->stream));
-  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
 }
 
 // Sum of the calibration partials of the last launch_calibration_sums, in
@@ -444,6 +443,36 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
 }
 
 }  // namespace
+
+namespace fpmhip {
+
+void session_initialize_launch(fpmhip_session& s) {
+  for (uint64_t f = 0; f < s.model->F; ++f) {
+    const double g = s.model->has_gain ? s.model->gain[f] : 1.0;
+    if (!std::isfinite(g) || g <= 0.0) fail(FPMHIP_INVALID_MODEL, "frame has invalid gain");
+  }
+  s.initialized = false;
+  launch_flag_reset(s);
+  launch_init(s);
+  launch_convert_pupil(s.model->d_pupil64.as<double2>(), s.b.pupil.ptr, s.precision == FPMHIP_MIXED_F32, s.model->N(),
+                       s.ctx->stream);
+  if (!s.extent_from_model) {  // the model's pupil: its extents and lists do not change
+    launch_pupil_extent(s);
+    s.extent_from_model = true;
+  }
+  load_calibration(s);
+  init_aux(s);
+  refresh_max_pupil(s);
+}
+
+void session_initialize_check(fpmhip_session& s) {
+  check_flags(s);  // waits for the launches
+  if (s.cfg.gauge) canonicalize_session(s);
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+  s.initialized = true;
+}
+
+}  // namespace fpmhip
 
 extern "C" {
 
@@ -739,6 +768,7 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       if (m->nh == m->nw && m->nh <= kFusedLdsMaxN)
         b.dest.allocate((2 + (m->nh + 1) * (m->nw + 1) + m->nh * m->nw) * sizeof(uint32_t));
       launch_pupil_extent(*s);
+      s->extent_from_model = true;
     }
     launch_flag_reset(*s);
     HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
@@ -754,26 +784,13 @@ int fpmhip_session_destroy(fpmhip_session* s) {
   });
 }
 
+
 int fpmhip_session_initialize(fpmhip_session* s) {
   return guarded([&] {
     require(s != nullptr, "session", "session is null");
     set_device(s->ctx);
-    for (uint64_t f = 0; f < s->model->F; ++f) {
-      const double g = s->model->has_gain ? s->model->gain[f] : 1.0;
-      if (!std::isfinite(g) || g <= 0.0) fail(FPMHIP_INVALID_MODEL, "frame has invalid gain");
-    }
-    launch_flag_reset(*s);
-    launch_init(*s);
-    launch_convert_pupil(s->model->d_pupil64.as<double2>(), s->b.pupil.ptr, s->precision == FPMHIP_MIXED_F32,
-                         s->model->N(), s->ctx->stream);
-    launch_pupil_extent(*s);
-    load_calibration(*s);
-    check_flags(*s);
-    init_aux(*s);
-    refresh_max_pupil(*s);
-    if (s->cfg.gauge) canonicalize_session(*s);
-    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
-    s->initialized = true;
+    session_initialize_launch(*s);
+    session_initialize_check(*s);
   });
 }
 
@@ -792,6 +809,7 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
     HIP_CHECK(hipMemcpyAsync(s->b.spec.ptr, spectrum, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
     upload_pupil64(*s, pupil);
     launch_pupil_extent(*s);
+    s->extent_from_model = false;
     if (frame_gains) {
       for (uint64_t f = 0; f < s->model->F; ++f)
         if (!std::isfinite(frame_gains[f]) || frame_gains[f] <= 0.0)

@@ -246,9 +246,9 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
 // a reset of the per-tile sweep state.
 void initialize_tiles(fpmhip_batch& b) {
   hipStream_t st = b.ctx->stream;
-  for (fpmhip_session* s : b.sessions) {
-    if (int rc = fpmhip_session_initialize(s)) throw Failure{rc};
-  }
+  // every tile's launches first, then the checks (one wait in all)
+  for (fpmhip_session* s : b.sessions) session_initialize_launch(*s);
+  for (fpmhip_session* s : b.sessions) session_initialize_check(*s);
   for (size_t i = 0; i < b.sessions.size(); ++i)
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, b.host_tiles[i].counter, b.sessions[i]->counter);
   hipLaunchKernelGGL(k_reset_tiles, dim3((b.launched + 255) / 256), dim3(256), 0, st, b.tiles.as<TileDesc>(),
@@ -475,6 +475,22 @@ int fpmhip_batch_tune(fpmhip_batch* b, uint32_t trial_sweeps, fpmhip_batch_tunin
       report->first_ms = first;
       report->best_ms = best_ms;
     }
+  });
+}
+
+int fpmhip_batch_objects(fpmhip_batch* b, double* out) {
+  return guarded([&] {
+    if (!b || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
+    HIP_CHECK(hipSetDevice(b->ctx->device));
+    for (fpmhip_session* s : b->sessions) launch_object(*s, s->b.work_m.as<double2>());
+    uint64_t off = 0;
+    for (fpmhip_session* s : b->sessions) {
+      const uint64_t M = s->model->M();
+      HIP_CHECK(hipMemcpyAsync(out + 2 * off, s->b.work_m.ptr, M * sizeof(double2), hipMemcpyDeviceToHost,
+                               b->ctx->stream));
+      off += M;
+    }
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
   });
 }
 

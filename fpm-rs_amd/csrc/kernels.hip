@@ -336,25 +336,39 @@ __global__ void k_init_average(const void* __restrict__ frames, int is_u16, cons
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   double a = 0.0, w = 0.0;
-  for (uint64_t f = 0; f < F; ++f) {
-    const double fw = weights[f];
-    if (fw == 0.0) continue;
-    if (masks && masks[f * N + p] == 0) continue;
-    const double b = bg_value(bg, bg_len, N, f, p);
-    const double amp = sqrt(fmax((meas_value(frames, is_u16, f * N + p) - b) / gains[f], 0.0));
-    a += fw * amp;
-    w += fw;
+  // frames in groups of G: the group's loads are issued before the
+  // in-order accumulation uses them
+  constexpr uint64_t G = 8;
+  for (uint64_t f0 = 0; f0 < F; f0 += G) {
+    double y[G];
+    uint8_t keep[G];
+#pragma unroll
+    for (uint64_t k = 0; k < G; ++k) {
+      const uint64_t f = f0 + k;
+      y[k] = f < F ? meas_value(frames, is_u16, f * N + p) : 0.0;
+      keep[k] = f < F && (!masks || masks[f * N + p] != 0);
+    }
+#pragma unroll
+    for (uint64_t k = 0; k < G; ++k) {
+      const uint64_t f = f0 + k;
+      if (f >= F || !keep[k]) continue;
+      const double fw = weights[f];
+      if (fw == 0.0) continue;
+      const double b = bg_value(bg, bg_len, N, f, p);
+      const double amp = sqrt(fmax((y[k] - b) / gains[f], 0.0));
+      a += fw * amp;
+      w += fw;
+    }
   }
   avg[p] = a;
   aw[p] = w;
 }
 
-// fallback amplitude (state.rs:314-326) and nearest-neighbour upsample (:327-336)
-__global__ void __launch_bounds__(1024) k_init_object(const double* __restrict__ avg, const double* __restrict__ aw,
-                                                      uint32_t nh, uint32_t nw, uint32_t rh, uint32_t rw,
-                                                      double2* __restrict__ object, int* __restrict__ flag) {
+// fallback amplitude (state.rs:314-326) into *fallback
+__global__ void __launch_bounds__(1024) k_init_fallback(const double* __restrict__ avg, const double* __restrict__ aw,
+                                                        uint32_t N, double* __restrict__ fallback_out,
+                                                        int* __restrict__ flag) {
   __shared__ double sa[1024], sw[1024];
-  const uint32_t N = nh * nw;
   double ta = 0.0, tw = 0.0;
   for (uint32_t p = threadIdx.x; p < N; p += blockDim.x) {
     ta += avg[p];
@@ -371,16 +385,23 @@ __global__ void __launch_bounds__(1024) k_init_object(const double* __restrict__
     __syncthreads();
   }
   const double total_w = sw[0];
-  const double fallback = total_w > 0.0 ? sa[0] / total_w : 0.0;
-  if (threadIdx.x == 0 && total_w == 0.0) flag[1] = 1;
-  const uint64_t M = (uint64_t)rh * rw;
-  for (uint64_t i = threadIdx.x; i < M; i += blockDim.x) {
-    const uint32_t r = (uint32_t)(i / rw), c = (uint32_t)(i % rw);
-    const uint32_t lr = (uint32_t)((uint64_t)r * nh / rh), lc = (uint32_t)((uint64_t)c * nw / rw);
-    const uint32_t lp = lr * nw + lc;
-    const double v = aw[lp] > 0.0 ? avg[lp] / aw[lp] : fallback;
-    object[i] = make_double2(v, 0.0);
+  if (threadIdx.x == 0) {
+    *fallback_out = total_w > 0.0 ? sa[0] / total_w : 0.0;
+    if (total_w == 0.0) flag[1] = 1;
   }
+}
+
+// nearest-neighbour upsample of the mean amplitude (state.rs:327-336)
+__global__ void k_init_object(const double* __restrict__ avg, const double* __restrict__ aw, uint32_t nh, uint32_t nw,
+                              uint32_t rh, uint32_t rw, const double* __restrict__ fallback,
+                              double2* __restrict__ object) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)rh * rw) return;
+  const uint32_t r = (uint32_t)(i / rw), c = (uint32_t)(i % rw);
+  const uint32_t lr = (uint32_t)((uint64_t)r * nh / rh), lc = (uint32_t)((uint64_t)c * nw / rw);
+  const uint32_t lp = lr * nw + lc;
+  const double v = aw[lp] > 0.0 ? avg[lp] / aw[lp] : *fallback;
+  object[i] = make_double2(v, 0.0);
 }
 
 __global__ void k_shift(const double2* __restrict__ src, double2* __restrict__ dst, uint32_t h, uint32_t w,
@@ -578,8 +599,12 @@ void launch_init(const fpmhip_session& s) {
                      m.d_background.as<double>(), (uint64_t)m.background.size(), m.F, (uint32_t)N,
                      s.b.avg.as<double>(), s.b.aw.as<double>());
   HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_init_object, dim3(1), dim3(1024), 0, st, s.b.avg.as<double>(), s.b.aw.as<double>(), m.nh,
-                     m.nw, m.rh, m.rw, s.b.work_m.as<double2>(), s.b.flag.as<int>());
+  // scal[2]: the fallback amplitude
+  hipLaunchKernelGGL(k_init_fallback, dim3(1), dim3(1024), 0, st, s.b.avg.as<double>(), s.b.aw.as<double>(),
+                     (uint32_t)N, s.b.scal.as<double>() + 2, s.b.flag.as<int>());
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_init_object, dim3(blocks_for(m.M())), dim3(kBlock), 0, st, s.b.avg.as<double>(),
+                     s.b.aw.as<double>(), m.nh, m.nw, m.rh, m.rw, s.b.scal.as<double>() + 2, s.b.work_m.as<double2>());
   HIP_CHECK(hipGetLastError());
   fft2_f64(*m.fft_high, s.b.work_m.as<double2>(), 1, false, st);
   launch_shift(s.b.work_m.as<double2>(), s.b.spec.as<double2>(), m.rh, m.rw, m.rh / 2, m.rw / 2, st);

@@ -46,7 +46,7 @@ struct SessionBuffers {
   DeviceBuffer proj;      // V[N]
   DeviceBuffer U;         // V[N]
   DeviceBuffer grad;      // V[N]
-  DeviceBuffer scal;      // double[8]: [0] max|P|^2, [1] max|patch|^2
+  DeviceBuffer scal;      // double[8]: [0] max|P|^2, [1] max|patch|^2, [2] initial fallback amplitude
   DeviceBuffer partials;  // double[F * nblk * 2]
   DeviceBuffer flag;      // int[4]
   DeviceBuffer work_m;    // double2[M] (init / result scratch)
@@ -78,6 +78,7 @@ struct fpmhip_session {
   bool has_gain = false;
   std::vector<double> gain;
   uint64_t bg_len = 0;  // background of the state: 0 (none), N (shared) or F * N (per frame)
+  bool extent_from_model = false;  // b.extent / b.dest describe the model's pupil
   fpmhip::SessionBuffers b;
   size_t vbytes() const { return precision == FPMHIP_PLAIN_C128 ? sizeof(double2) : sizeof(float2); }
 };
@@ -106,6 +107,11 @@ void launch_pupil_step(const fpmhip_session& s, double step);
 void launch_momentum(const fpmhip_session& s);
 void launch_flag_reset(const fpmhip_session& s);
 void launch_init(const fpmhip_session& s);
+// ReconstructionState::initialize_with_backend in two parts: every device
+// launch (no host synchronisation), then the host-side checks and the EPRY
+// gauge (api.hip). A batch launches all of its tiles before checking any.
+void session_initialize_launch(fpmhip_session& s);
+void session_initialize_check(fpmhip_session& s);
 void launch_object(const fpmhip_session& s, double2* out_dev);
 void launch_shift(const double2* src, double2* dst, uint32_t h, uint32_t w, uint32_t rs, uint32_t cs,
                   hipStream_t stream);

@@ -300,6 +300,11 @@ int fpmhip_batch_run(fpmhip_batch* batch, uint64_t iterations, double* trace);
  * with HIP events on the batch stream: mean and max per sweep (ms), count. */
 int fpmhip_batch_timing(const fpmhip_batch* batch, double* sweep_ms_mean, double* sweep_ms_max,
                         uint64_t* sweeps);
+/* Every tile's result object (state_object, result.rs:397-413), in session
+ * order, concatenated into `out` (sum of the tiles' R_h * R_w complex128
+ * values, interleaved): fpmhip_session_get_object for all tiles, with the
+ * launches of all tiles queued before the copies and one wait at the end. */
+int fpmhip_batch_objects(fpmhip_batch* batch, double* out);
 
 /* Schedule of the batch's sweep launch (no counterpart in the reference; the
  * reference runs one tile at a time on the host). */

@@ -206,6 +206,20 @@ def test_sweeps_per_launch_do_not_change_results(ctx):
         assert np.array_equal(out[0][1], tr)
 
 
+def test_batch_objects_match_session_objects(ctx):
+    """fpmhip_batch_objects returns each tile's state_object, bit for bit the
+    per-session download, for tiles of different shapes."""
+    alg = Fpie(iterations=2)
+    tiles = [tile(64, 5, 151), tile(128, 5, 152), tile(32, 5, 153)]
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    b.run(2)
+    objs = b.objects()
+    for s, o in zip(ss, objs):
+        assert o.shape == tuple(s.problem.model.reconstruction_shape)
+        assert np.array_equal(o, s.object())
+
+
 def test_launch_order_does_not_change_results(ctx):
     alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
     tiles = [tile(64, 5, 111), tile(256, 5, 112), tile(32, 5, 113)]
