@@ -119,7 +119,7 @@ def run_batched(model, meas, iterations, steps, warmup, tiles):
     sweep_ms = []
     for i in range(steps):
         batch.run(iterations)
-        batch.objects()  # every tile's result object, downloaded
+        batch.objects(pinned=True)  # every tile's result object, downloaded to page-locked host memory
         sweep_ms.append(batch.timing()[0])
         log(f"batch {i + 1}/{steps} done")
     ctx.synchronize()
@@ -227,7 +227,7 @@ def run_uneven(iterations=40, steps=1, warmup=1):
     sweep_ms = []
     for i in range(steps):
         batch.run(iterations)
-        batch.objects()  # every tile's result object, downloaded
+        batch.objects(pinned=True)  # every tile's result object, downloaded to page-locked host memory
         sweep_ms.append(batch.timing()[0])
         log(f"uneven batch {i + 1}/{steps} done")
     ctx.synchronize()

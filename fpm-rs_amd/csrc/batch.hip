@@ -195,11 +195,14 @@ struct fpmhip_batch {
   uint64_t fmax = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  double* pinned = nullptr;  // page-locked result objects (fpmhip_batch_objects_pinned)
+  size_t pinned_bytes = 0;
   double ms_mean = 0.0, ms_max = 0.0;
   uint64_t sweeps = 0;
   ~fpmhip_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (pinned) (void)hipHostFree(pinned);
   }
 };
 
@@ -478,19 +481,48 @@ int fpmhip_batch_tune(fpmhip_batch* b, uint32_t trial_sweeps, fpmhip_batch_tunin
   });
 }
 
+namespace {
+
+void batch_objects_into(fpmhip_batch& b, double* out) {
+  HIP_CHECK(hipSetDevice(b.ctx->device));
+  for (fpmhip_session* s : b.sessions) launch_object(*s, s->b.work_m.as<double2>());
+  uint64_t off = 0;
+  for (fpmhip_session* s : b.sessions) {
+    const uint64_t M = s->model->M();
+    HIP_CHECK(hipMemcpyAsync(out + 2 * off, s->b.work_m.ptr, M * sizeof(double2), hipMemcpyDeviceToHost,
+                             b.ctx->stream));
+    off += M;
+  }
+  HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
+}
+
+}  // namespace
+
 int fpmhip_batch_objects(fpmhip_batch* b, double* out) {
   return guarded([&] {
     if (!b || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
-    HIP_CHECK(hipSetDevice(b->ctx->device));
-    for (fpmhip_session* s : b->sessions) launch_object(*s, s->b.work_m.as<double2>());
-    uint64_t off = 0;
-    for (fpmhip_session* s : b->sessions) {
-      const uint64_t M = s->model->M();
-      HIP_CHECK(hipMemcpyAsync(out + 2 * off, s->b.work_m.ptr, M * sizeof(double2), hipMemcpyDeviceToHost,
-                               b->ctx->stream));
-      off += M;
+    batch_objects_into(*b, out);
+  });
+}
+
+int fpmhip_batch_objects_pinned(fpmhip_batch* b, const double** out) {
+  return guarded([&] {
+    if (!b || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
+    size_t bytes = 0;
+    for (fpmhip_session* s : b->sessions) bytes += s->model->M() * sizeof(double2);
+    if (b->pinned_bytes < bytes) {
+      HIP_CHECK(hipSetDevice(b->ctx->device));
+      if (b->pinned) HIP_CHECK(hipHostFree(b->pinned));
+      b->pinned = nullptr;
+      b->pinned_bytes = 0;
+      void* p = nullptr;
+      if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p)
+        fail(FPMHIP_OUT_OF_MEMORY, "page-locked result buffer could not be allocated");
+      b->pinned = static_cast<double*>(p);
+      b->pinned_bytes = bytes;
     }
-    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    batch_objects_into(*b, b->pinned);
+    *out = b->pinned;
   });
 }
 

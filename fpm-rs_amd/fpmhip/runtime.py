@@ -27,7 +27,7 @@ EXPORTS = [
     "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
     "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
-    "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
+    "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_objects_pinned", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
     "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
     "fpmhip_server_poll", "fpmhip_server_wait", "fpmhip_server_result", "fpmhip_server_stats",
 ]
@@ -356,11 +356,20 @@ class Batch:
         check(lib().fpmhip_batch_run(self.handle, ct.c_uint64(iterations), _cd(trace)))
         return trace
 
-    def objects(self):
-        """Every tile's result object (one launch queue, one wait)."""
+    def objects(self, pinned=False):
+        """Every tile's result object (one launch queue, one wait). With
+        pinned=True the arrays are views of the batch's page-locked result
+        buffer, valid until the next objects() call or the batch's release
+        (copy them to keep them)."""
         shapes = [tuple(s.problem.model.reconstruction_shape) for s in self.sessions]
-        flat = np.zeros(sum(h * w for h, w in shapes), np.complex128)
-        check(lib().fpmhip_batch_objects(self.handle, _cd(flat)))
+        total = sum(h * w for h, w in shapes)
+        if pinned:
+            p = ct.POINTER(ct.c_double)()
+            check(lib().fpmhip_batch_objects_pinned(self.handle, ct.byref(p)))
+            flat = np.ctypeslib.as_array(p, shape=(2 * total,)).view(np.complex128)
+        else:
+            flat = np.zeros(total, np.complex128)
+            check(lib().fpmhip_batch_objects(self.handle, _cd(flat)))
         out, off = [], 0
         for h, w in shapes:
             out.append(flat[off:off + h * w].reshape(h, w))

@@ -305,6 +305,11 @@ int fpmhip_batch_timing(const fpmhip_batch* batch, double* sweep_ms_mean, double
  * values, interleaved): fpmhip_session_get_object for all tiles, with the
  * launches of all tiles queued before the copies and one wait at the end. */
 int fpmhip_batch_objects(fpmhip_batch* batch, double* out);
+/* The same into a page-locked host buffer owned by the batch (allocated on
+ * first use, freed with the batch), for full-rate PCIe copies: *out points
+ * at the concatenated objects and stays valid until the next call on this
+ * batch or its destruction. */
+int fpmhip_batch_objects_pinned(fpmhip_batch* batch, const double** out);
 
 /* Schedule of the batch's sweep launch (no counterpart in the reference; the
  * reference runs one tile at a time on the host). */

@@ -215,9 +215,11 @@ def test_batch_objects_match_session_objects(ctx):
     b = runtime.Batch(ss)
     b.run(2)
     objs = b.objects()
-    for s, o in zip(ss, objs):
+    pinned = b.objects(pinned=True)
+    for s, o, q in zip(ss, objs, pinned):
         assert o.shape == tuple(s.problem.model.reconstruction_shape)
         assert np.array_equal(o, s.object())
+        assert np.array_equal(q, o)
 
 
 def test_launch_order_does_not_change_results(ctx):
