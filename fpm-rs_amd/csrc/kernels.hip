@@ -329,29 +329,34 @@ __global__ void k_flag_reset(int* flag) {
 }
 
 // weighted mean measured amplitude per low-resolution pixel (state.rs:288-313)
-__global__ void k_init_average(const void* __restrict__ frames, int is_u16, const double* __restrict__ weights,
-                               const uint8_t* __restrict__ masks, const double* __restrict__ gains,
-                               const double* __restrict__ bg, uint64_t bg_len, uint64_t F, uint32_t N,
-                               double* __restrict__ avg, double* __restrict__ aw) {
+// Per-pixel partial sums of the weighted mean amplitude over the frames of
+// chunk blockIdx.y ([y * chunk, (y + 1) * chunk), in order): part[(y * N + p)]
+// = (sum w_f sqrt(max((y - bg) / g, 0)), sum w_f) over the chunk's
+// positive-weight, unmasked frames (state.rs:288-313).
+__global__ void k_init_average_part(const void* __restrict__ frames, int is_u16, const double* __restrict__ weights,
+                                    const uint8_t* __restrict__ masks, const double* __restrict__ gains,
+                                    const double* __restrict__ bg, uint64_t bg_len, uint64_t F, uint32_t N,
+                                    uint64_t chunk, double2* __restrict__ part) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
+  const uint64_t fa = (uint64_t)blockIdx.y * chunk, fb = fa + chunk < F ? fa + chunk : F;
   double a = 0.0, w = 0.0;
   // frames in groups of G: the group's loads are issued before the
   // in-order accumulation uses them
   constexpr uint64_t G = 8;
-  for (uint64_t f0 = 0; f0 < F; f0 += G) {
+  for (uint64_t f0 = fa; f0 < fb; f0 += G) {
     double y[G];
     uint8_t keep[G];
 #pragma unroll
     for (uint64_t k = 0; k < G; ++k) {
       const uint64_t f = f0 + k;
-      y[k] = f < F ? meas_value(frames, is_u16, f * N + p) : 0.0;
-      keep[k] = f < F && (!masks || masks[f * N + p] != 0);
+      y[k] = f < fb ? meas_value(frames, is_u16, f * N + p) : 0.0;
+      keep[k] = f < fb && (!masks || masks[f * N + p] != 0);
     }
 #pragma unroll
     for (uint64_t k = 0; k < G; ++k) {
       const uint64_t f = f0 + k;
-      if (f >= F || !keep[k]) continue;
+      if (f >= fb || !keep[k]) continue;
       const double fw = weights[f];
       if (fw == 0.0) continue;
       const double b = bg_value(bg, bg_len, N, f, p);
@@ -359,6 +364,20 @@ __global__ void k_init_average(const void* __restrict__ frames, int is_u16, cons
       a += fw * amp;
       w += fw;
     }
+  }
+  part[(uint64_t)blockIdx.y * N + p] = make_double2(a, w);
+}
+
+// The chunk partials summed in chunk order.
+__global__ void k_init_average_sum(const double2* __restrict__ part, uint32_t chunks, uint32_t N,
+                                   double* __restrict__ avg, double* __restrict__ aw) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double a = 0.0, w = 0.0;
+  for (uint32_t c = 0; c < chunks; ++c) {
+    const double2 v = part[(uint64_t)c * N + p];
+    a += v.x;
+    w += v.y;
   }
   avg[p] = a;
   aw[p] = w;
@@ -594,10 +613,18 @@ void launch_init(const fpmhip_session& s) {
   const Measurements& me = *s.meas;
   const uint64_t N = m.N();
   hipStream_t st = s.ctx->stream;
-  hipLaunchKernelGGL(k_init_average, dim3(blocks_for(N)), dim3(kBlock), 0, st, me.d_frames.ptr, me.is_u16 ? 1 : 0,
-                     me.d_weights.as<double>(), me.d_masks.as<uint8_t>(), m.d_gains.as<double>(),
-                     m.d_background.as<double>(), (uint64_t)m.background.size(), m.F, (uint32_t)N,
-                     s.b.avg.as<double>(), s.b.aw.as<double>());
+  // frames in up to 8 ordered chunks (partials in the init scratch work_m,
+  // max(M, N) complex128 slots), so that the pass fills the device
+  const uint64_t slots = s.b.work_m.bytes / sizeof(double2);
+  const uint32_t chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({8, slots / N, m.F ? m.F : 1}));
+  const uint64_t chunk = (m.F + chunks - 1) / chunks;
+  hipLaunchKernelGGL(k_init_average_part, dim3(blocks_for(N), chunks), dim3(kBlock), 0, st, me.d_frames.ptr,
+                     me.is_u16 ? 1 : 0, me.d_weights.as<double>(), me.d_masks.as<uint8_t>(), m.d_gains.as<double>(),
+                     m.d_background.as<double>(), (uint64_t)m.background.size(), m.F, (uint32_t)N, chunk,
+                     s.b.work_m.as<double2>());
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_init_average_sum, dim3(blocks_for(N)), dim3(kBlock), 0, st, s.b.work_m.as<double2>(), chunks,
+                     (uint32_t)N, s.b.avg.as<double>(), s.b.aw.as<double>());
   HIP_CHECK(hipGetLastError());
   // scal[2]: the fallback amplitude
   hipLaunchKernelGGL(k_init_fallback, dim3(1), dim3(1024), 0, st, s.b.avg.as<double>(), s.b.aw.as<double>(),
