@@ -967,6 +967,31 @@ int fpmhip_session_get_calibration(fpmhip_session* s, double* frame_gains, doubl
   });
 }
 
+int fpmhip_session_calibration_present(const fpmhip_session* s, int32_t* frame_gains, uint64_t* background_len) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    if (frame_gains) *frame_gains = s->has_gain ? 1 : 0;
+    if (background_len) *background_len = s->bg_len;
+  });
+}
+
+int fpmhip_session_clear_calibration(fpmhip_session* s, int32_t frame_gains, int32_t background) {
+  return guarded([&] {
+    require(s != nullptr, "session", "session is null");
+    if (frame_gains && s->has_gain) {
+      set_device(s->ctx);
+      s->has_gain = false;
+      s->gain.assign(s->model->F, 1.0);
+      if (s->model->F) {
+        HIP_CHECK(hipMemcpyAsync(s->b.gains.ptr, s->gain.data(), s->gain.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, s->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+      }
+    }
+    if (background) s->bg_len = 0;
+  });
+}
+
 int fpmhip_session_shape(const fpmhip_session* s, uint64_t* nh, uint64_t* nw, uint64_t* rh, uint64_t* rw,
                          uint64_t* F) {
   return guarded([&] {

@@ -26,6 +26,7 @@ STATUS_NAMES = {
     11: "DeviceError",
     12: "OutOfMemory",
 }
+STATUS_CODES = {name: code for code, name in STATUS_NAMES.items()}
 
 FORWARD, INVERSE = 0, 1
 AP, FPIE, MPIE, EPRY = 0, 1, 2, 3

@@ -36,11 +36,15 @@ class _Base:
                 setattr(a, f.name, v)
         return a
 
-    def run(self, problem, precision="c128", schedule="sequential", context=None):
-        """Runner::run on the GPU (one session); returns a ReconstructionResult."""
+    def run(self, problem, precision="c128", schedule="sequential", context=None, callbacks=None,
+            resume_from=None):
+        """Runner::run on the GPU (one session); returns a ReconstructionResult.
+        `callbacks` may hold CheckpointEvery; `resume_from` a
+        ReconstructionCheckpoint (python/fpm_rs/__init__.pyi:3316-3336)."""
         from .runtime import run_algorithm
 
-        return run_algorithm(self, problem, precision=precision, schedule=schedule, context=context)
+        return run_algorithm(self, problem, precision=precision, schedule=schedule, context=context,
+                             callbacks=callbacks, resume_from=resume_from)
 
 
 @dataclass

@@ -26,6 +26,7 @@ EXPORTS = [
     "fpmhip_session_destroy", "fpmhip_session_initialize", "fpmhip_session_set_state", "fpmhip_session_sweep",
     "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
+    "fpmhip_session_calibration_present", "fpmhip_session_clear_calibration",
     "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
     "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_objects_pinned", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
     "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
@@ -290,6 +291,17 @@ class Session:
             return gains, None
         return gains, (bg[:n_img].reshape(self.N) if n.value == n_img else bg.reshape((self.F,) + self.N))
 
+    def calibration_present(self):
+        """(state holds frame gains, background length: 0, N or F*N)."""
+        g = ct.c_int32(0)
+        n = ct.c_uint64(0)
+        check(lib().fpmhip_session_calibration_present(self.handle, ct.byref(g), ct.byref(n)))
+        return bool(g.value), n.value
+
+    def clear_calibration(self, frame_gains=True, background=True):
+        check(lib().fpmhip_session_clear_calibration(self.handle, ct.c_int32(int(bool(frame_gains))),
+                                                     ct.c_int32(int(bool(background)))))
+
     def momentum(self):
         v = np.zeros(self.R, np.complex128)
         a = np.zeros(self.R, np.complex128)
@@ -311,16 +323,98 @@ class ReconstructionResult:
     extra: dict = field(default_factory=dict)
 
 
+def mpie_parameters(alg: abi.Algorithm) -> dict:
+    """The mPIE parameters a checkpoint's auxiliary state records (state.rs:34-53)."""
+    return dict(object_step=alg.object_step, stability=alg.stability, epsilon=alg.epsilon,
+                momentum_interval=int(alg.momentum_interval), momentum_friction=alg.momentum_friction,
+                momentum_feedback=alg.momentum_feedback)
+
+
+def _restore(s: Session, alg: abi.Algorithm, problem: ReconstructionProblem, ck):
+    """ReconstructionState::from_checkpoint (state.rs:398-419) and the mPIE
+    auxiliary check of prepare_auxiliary (algorithms/mpie.rs:166-222)."""
+    from .checkpoint import MpieAuxiliary
+
+    ck.validate_for_problem(problem)
+    if ck.illumination_corrections is not None or ck.physical_illumination_calibration is not None:
+        raise FpmError(abi.STATUS_CODES["Unsupported"],
+                       "checkpoints with illumination corrections or physical calibration resume only in "
+                       "fpm-rs's calibration solvers")
+    aux = ck.algorithm_auxiliary
+    v = a = None
+    counter = 0
+    if alg.kind == abi.MPIE and aux is not None:
+        if not isinstance(aux, MpieAuxiliary):
+            raise FpmError(abi.STATUS_CODES["InvalidModel"],
+                           "mPIE cannot resume auxiliary state owned by another algorithm")
+        mine = mpie_parameters(alg)
+        for name in ("object_step", "stability", "epsilon", "momentum_friction", "momentum_feedback"):
+            cur, stored = float(mine[name]), float(getattr(aux, name))
+            if np.float64(cur).view(np.uint64) != np.float64(stored).view(np.uint64):
+                raise FpmError(abi.STATUS_CODES["InvalidParameter"],
+                               f"value {cur} differs from checkpointed mPIE value {stored}", name)
+        if mine["momentum_interval"] != aux.momentum_interval:
+            raise FpmError(abi.STATUS_CODES["InvalidParameter"],
+                           f"value {mine['momentum_interval']} differs from checkpointed mPIE value "
+                           f"{aux.momentum_interval}", "momentum_interval")
+        v, a, counter = aux.velocity, aux.anchor, aux.effective_frames_since_momentum
+    s.set_state(ck.object_spectrum, ck.pupil_values, velocity=v, anchor=a, counter=counter,
+                frame_gains=ck.frame_gains, background=ck.background)
+    s.clear_calibration(frame_gains=ck.frame_gains is None, background=ck.background is None)
+
+
 def run_algorithm(algorithm, problem: ReconstructionProblem, precision="c128", schedule="sequential",
-                  context=None) -> ReconstructionResult:
-    """Runner::run (src/reconstruction/runner.rs:76-285) on one device session."""
+                  context=None, callbacks=None, resume_from=None) -> ReconstructionResult:
+    """Runner::run (src/reconstruction/runner.rs:76-285) on one device session.
+
+    `resume_from` restores a ReconstructionCheckpoint and continues from its
+    completed iterations up to `algorithm.iterations` (runner.rs:94-121);
+    `callbacks` takes CheckpointEvery (callbacks/checkpoint.rs). The trace
+    holds the checkpoint's records followed by the new ones, with elapsed
+    seconds continuing from the last restored record."""
+    import time
+
+    from .checkpoint import CheckpointEvery, ReconstructionCheckpoint
+
     alg = algorithm.to_c()
     validate(alg)
+    callbacks = list(callbacks or [])
+    for cb in callbacks:
+        if not isinstance(cb, CheckpointEvery):
+            raise FpmError(abi.STATUS_CODES["Unsupported"], f"unsupported callback {type(cb).__name__}")
+    started = time.perf_counter()
     s = Session(problem, alg, precision=precision, context=context)
-    s.initialize()
-    trace = s.run(int(algorithm.iterations), schedule)
+    if resume_from is not None:
+        _restore(s, alg, problem, resume_from)
+        records = list(resume_from.trace)
+        metrics = list(resume_from.algorithm_metrics)
+        first = int(resume_from.completed_iterations)
+        carried_aux = None if alg.kind == abi.MPIE else resume_from.algorithm_auxiliary
+    else:
+        s.initialize()
+        records, metrics, first, carried_aux = [], [], 0, None
+    previous = records[-1][2] if records else 0.0
+    for cb in callbacks:
+        cb.on_start()
+    total = int(algorithm.iterations)
+    mparams = mpie_parameters(alg) if alg.kind == abi.MPIE else None
+    if callbacks:
+        for it in range(first, total):
+            obj = float(s.run(1, schedule)[0])
+            records.append((it + 1, obj, previous + time.perf_counter() - started))
+            for cb in callbacks:
+                if cb.due(it + 1):
+                    ck = ReconstructionCheckpoint.capture(it + 1, s, records, mparams, metrics, carried_aux)
+                    cb.write(ck)
+    elif total > first:
+        objs = s.run(total - first, schedule)
+        end = previous + time.perf_counter() - started
+        # one timestamp for the whole device run: the records share it
+        records += [(first + k + 1, float(o), end) for k, o in enumerate(objs)]
+    trace = np.array([r[1] for r in records], np.float64)
     obj = s.object()
     res = ReconstructionResult(obj, np.abs(obj), np.angle(obj), s.spectrum(), s.pupil(), trace)
+    res.extra.update(trace_records=records)
     if alg.kind == abi.MPIE:
         v, a, c = s.momentum()
         res.extra.update(velocity=v, anchor=a, counter=c)

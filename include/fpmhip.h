@@ -282,6 +282,14 @@ int fpmhip_session_get_momentum(fpmhip_session* session, double* velocity, doubl
                                 uint64_t* counter);
 int fpmhip_session_get_calibration(fpmhip_session* session, double* frame_gains,
                                    double* background, uint64_t* background_len);
+/* Whether the state holds frame gains (ReconstructionState::frame_gains() is
+ * Some, state.rs:216) and its background length (0 = None, state.rs:221). */
+int fpmhip_session_calibration_present(const fpmhip_session* session, int32_t* frame_gains,
+                                       uint64_t* background_len);
+/* Drops the state's frame gains (all frames then use gain 1) and/or its
+ * background (0), as a checkpoint with `frame_gains` / `background` None
+ * restores them (state.rs:411-412). */
+int fpmhip_session_clear_calibration(fpmhip_session* session, int32_t frame_gains, int32_t background);
 int fpmhip_session_shape(const fpmhip_session* session, uint64_t* image_h, uint64_t* image_w,
                          uint64_t* recon_h, uint64_t* recon_w, uint64_t* frame_count);
 
