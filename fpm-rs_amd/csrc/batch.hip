@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <utility>
 
 #include "fused.h"
 
@@ -40,10 +41,6 @@ __global__ void k_reset_tiles(TileDesc* tiles, uint32_t count) {
       tiles[i].group[2] = 0;
     }
   }
-}
-
-__global__ void k_set_counter(uint64_t* counter, uint64_t value) {
-  if (threadIdx.x == 0) *counter = value;
 }
 
 }  // namespace
@@ -197,6 +194,21 @@ struct fpmhip_batch {
   hipGraphExec_t exec = nullptr;
   double* pinned = nullptr;  // page-locked result objects (fpmhip_batch_objects_pinned)
   size_t pinned_bytes = 0;
+  // batched initialisation and results: one InitTile per session, and one
+  // R-array slot per tile in `slots`, tiles of equal R consecutive so that
+  // each group is transformed by one batched 2-D FFT
+  struct SlotGroup {
+    const Fft2* plan;
+    double2* base;
+    uint32_t count;
+  };
+  DeviceBuffer init_tiles;  // InitTile[count], session order
+  DeviceBuffer slots;       // double2[sum of R_h * R_w]
+  DeviceBuffer init_flags;  // int[2 * count]
+  std::vector<SlotGroup> slot_groups;
+  std::vector<double2*> slot_of;  // per session
+  uint32_t init_nmax = 0, init_mmax = 0;
+  uint64_t init_cmax = 0;
   double ms_mean = 0.0, ms_max = 0.0;
   uint64_t sweeps = 0;
   ~fpmhip_batch() {
@@ -207,6 +219,82 @@ struct fpmhip_batch {
 };
 
 namespace {
+
+// InitTile table and result slots of the batch (the per-tile counters live
+// in per_tile, which build_tiles reallocates: rebuilt with it).
+void build_init(fpmhip_batch& b) {
+  const uint32_t count = (uint32_t)b.sessions.size();
+  std::vector<std::pair<uint32_t, uint32_t>> shapes;  // distinct R, in order of first appearance
+  for (fpmhip_session* s : b.sessions) {
+    const std::pair<uint32_t, uint32_t> r{s->model->rh, s->model->rw};
+    if (std::find(shapes.begin(), shapes.end(), r) == shapes.end()) shapes.push_back(r);
+  }
+  uint64_t total = 0;
+  for (fpmhip_session* s : b.sessions) total += s->model->M();
+  if (b.slots.bytes < total * sizeof(double2)) b.slots.allocate(total * sizeof(double2));
+  b.init_flags.allocate(std::max<size_t>(2 * (size_t)count, 1) * sizeof(int));
+  b.slot_of.assign(count, nullptr);
+  b.slot_groups.clear();
+  uint64_t off = 0;
+  for (const auto& r : shapes) {
+    fpmhip_batch::SlotGroup g{nullptr, b.slots.as<double2>() + off, 0};
+    for (uint32_t i = 0; i < count; ++i) {
+      const Model& m = *b.sessions[i]->model;
+      if (m.rh != r.first || m.rw != r.second) continue;
+      if (!g.plan) g.plan = &*m.fft_high;
+      b.slot_of[i] = b.slots.as<double2>() + off;
+      off += m.M();
+      ++g.count;
+    }
+    b.slot_groups.push_back(g);
+  }
+  std::vector<InitTile> tiles(count);
+  b.init_nmax = b.init_mmax = 0;
+  b.init_cmax = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    fpmhip_session& s = *b.sessions[i];
+    const Model& m = *s.model;
+    const Measurements& me = *s.meas;
+    InitTile& t = tiles[i];
+    t = InitTile{};
+    t.frames = me.d_frames.ptr;
+    t.weights = me.d_weights.as<double>();
+    t.masks = me.d_masks.as<uint8_t>();
+    t.model_gains = m.d_gains.as<double>();
+    t.model_bg = m.d_background.as<double>();
+    t.model_pupil = m.d_pupil64.as<double2>();
+    t.part = s.b.work_m.as<double2>();
+    t.avg = s.b.avg.as<double>();
+    t.aw = s.b.aw.as<double>();
+    t.scal = s.b.scal.as<double>();
+    t.flag = s.b.flag.as<int>();
+    t.counter = b.host_tiles[i].counter;
+    t.slot = b.slot_of[i];
+    t.spec = s.b.spec.as<double2>();
+    t.pupil = s.b.pupil.as<float2>();
+    t.vel = s.cfg.momentum ? s.b.vel.as<float2>() : nullptr;
+    t.anc = s.cfg.momentum ? s.b.anc.as<float2>() : nullptr;
+    t.gains = s.b.gains.as<double>();
+    t.bg = s.b.bg.as<double>();
+    t.F = m.F;
+    t.chunks = init_chunks(s);
+    t.chunk = (m.F + t.chunks - 1) / t.chunks;
+    t.bg_len = m.background.size();
+    t.N = (uint32_t)m.N();
+    t.nh = m.nh;
+    t.nw = m.nw;
+    t.rh = m.rh;
+    t.rw = m.rw;
+    t.is_u16 = me.is_u16 ? 1 : 0;
+    b.init_nmax = std::max<uint32_t>(b.init_nmax, t.N);
+    b.init_mmax = std::max<uint32_t>(b.init_mmax, (uint32_t)m.M());
+    b.init_cmax = std::max<uint64_t>({b.init_cmax, (uint64_t)t.N, t.F, t.bg_len});
+  }
+  b.init_tiles.allocate(std::max<size_t>(count, 1) * sizeof(InitTile));
+  HIP_CHECK(hipMemcpyAsync(b.init_tiles.ptr, tiles.data(), count * sizeof(InitTile), hipMemcpyHostToDevice,
+                           b.ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(b.ctx->stream));  // `tiles` is pageable host memory
+}
 
 void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
   const uint32_t count = (uint32_t)b.sessions.size();
@@ -243,17 +331,48 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
   launch_sweep(b.tiles.as<TileDesc>(), b.launched, b.tw.as<float2>(), b.nmax, b.cooperative, b.ctx->stream);
   HIP_CHECK(hipStreamEndCapture(b.ctx->stream, &b.graph));
   HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+  build_init(b);
 }
 
 // ReconstructionState::initialize (+ mPIE auxiliary state) of every tile and
-// a reset of the per-tile sweep state.
+// a reset of the per-tile sweep state: each pass one launch for all tiles,
+// one batched FFT per distinct R, one wait for the checks.
 void initialize_tiles(fpmhip_batch& b) {
   hipStream_t st = b.ctx->stream;
-  // every tile's launches first, then the checks (one wait in all)
-  for (fpmhip_session* s : b.sessions) session_initialize_launch(*s);
-  for (fpmhip_session* s : b.sessions) session_initialize_check(*s);
-  for (size_t i = 0; i < b.sessions.size(); ++i)
-    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, b.host_tiles[i].counter, b.sessions[i]->counter);
+  const uint32_t count = (uint32_t)b.sessions.size();
+  for (fpmhip_session* s : b.sessions) {
+    const Model& m = *s->model;
+    for (uint64_t f = 0; f < m.F; ++f) {
+      const double g = m.has_gain ? m.gain[f] : 1.0;
+      if (!std::isfinite(g) || g <= 0.0) fail(FPMHIP_INVALID_MODEL, "frame has invalid gain");
+    }
+    s->initialized = false;
+  }
+  const InitTile* tiles = b.init_tiles.as<InitTile>();
+  launch_init_batch_pre(tiles, count, b.init_nmax, b.init_mmax, b.init_cmax, st);
+  for (fpmhip_session* s : b.sessions)
+    if (!s->extent_from_model) {  // after the pupil copy: extents of the model's pupil
+      launch_pupil_extent(*s);
+      s->extent_from_model = true;
+    }
+  for (const fpmhip_batch::SlotGroup& g : b.slot_groups) fft2_f64(*g.plan, g.base, g.count, false, st);
+  launch_init_batch_post(tiles, count, b.init_mmax, st);
+  launch_init_batch_flags(tiles, count, b.init_flags.as<int>(), st);
+  std::vector<int> flags(2 * (size_t)count);
+  HIP_CHECK(hipMemcpyAsync(flags.data(), b.init_flags.ptr, flags.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  for (uint32_t i = 0; i < count; ++i) {
+    if (flags[2 * i]) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
+    if (flags[2 * i + 1]) fail(FPMHIP_INVALID_MEASUREMENTS, "no positive-weight, unmasked measurements are available");
+  }
+  for (fpmhip_session* s : b.sessions) {  // host side of load_calibration and init_aux
+    const Model& m = *s->model;
+    s->has_gain = m.has_gain;
+    s->gain = m.gain;
+    s->bg_len = m.background.size();
+    s->counter = 0;
+    s->initialized = true;
+  }
   hipLaunchKernelGGL(k_reset_tiles, dim3((b.launched + 255) / 256), dim3(256), 0, st, b.tiles.as<TileDesc>(),
                      b.launched);
   HIP_CHECK(hipGetLastError());
@@ -483,14 +602,21 @@ int fpmhip_batch_tune(fpmhip_batch* b, uint32_t trial_sweeps, fpmhip_batch_tunin
 
 namespace {
 
-void batch_objects_into(fpmhip_batch& b, double* out) {
+// state_object of every tile (result.rs:397-413) into its slot: one shift
+// launch for all tiles and one inverse FFT per distinct R.
+void batch_objects_compute(fpmhip_batch& b) {
   HIP_CHECK(hipSetDevice(b.ctx->device));
-  for (fpmhip_session* s : b.sessions) launch_object(*s, s->b.work_m.as<double2>());
+  hipStream_t st = b.ctx->stream;
+  launch_object_batch_shift(b.init_tiles.as<InitTile>(), (uint32_t)b.sessions.size(), b.init_mmax, st);
+  for (const fpmhip_batch::SlotGroup& g : b.slot_groups) fft2_f64(*g.plan, g.base, g.count, true, st);
+}
+
+void batch_objects_into(fpmhip_batch& b, double* out) {
+  batch_objects_compute(b);
   uint64_t off = 0;
-  for (fpmhip_session* s : b.sessions) {
-    const uint64_t M = s->model->M();
-    HIP_CHECK(hipMemcpyAsync(out + 2 * off, s->b.work_m.ptr, M * sizeof(double2), hipMemcpyDeviceToHost,
-                             b.ctx->stream));
+  for (size_t i = 0; i < b.sessions.size(); ++i) {
+    const uint64_t M = b.sessions[i]->model->M();
+    HIP_CHECK(hipMemcpyAsync(out + 2 * off, b.slot_of[i], M * sizeof(double2), hipMemcpyDeviceToHost, b.ctx->stream));
     off += M;
   }
   HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
@@ -502,6 +628,15 @@ int fpmhip_batch_objects(fpmhip_batch* b, double* out) {
   return guarded([&] {
     if (!b || !out) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
     batch_objects_into(*b, out);
+  });
+}
+
+int fpmhip_batch_objects_device(fpmhip_batch* b, const double** objects) {
+  return guarded([&] {
+    if (!b || !objects) fail(FPMHIP_INVALID_PARAMETER, "null argument", "batch");
+    batch_objects_compute(*b);
+    HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    for (size_t i = 0; i < b->sessions.size(); ++i) objects[i] = reinterpret_cast<const double*>(b->slot_of[i]);
   });
 }
 

@@ -281,10 +281,9 @@ __global__ void k_insert(double2* __restrict__ spec, uint32_t rw, const SrcParam
   dst = acc;
 }
 
-// max |v|^2 over n values (single block; max is order independent)
+// max |v|^2 over n values by one 1024-thread block (max is order independent)
 template <class V>
-__global__ void __launch_bounds__(1024) k_max_abs2(const V* __restrict__ v, uint64_t n, double* __restrict__ out) {
-  __shared__ double sh[1024];
+__device__ inline void block_max_abs2(const V* __restrict__ v, uint64_t n, double* __restrict__ out, double* sh) {
   double m = 0.0;
   for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, D::norm2(to_d2(v[i])));
   sh[threadIdx.x] = m;
@@ -294,6 +293,12 @@ __global__ void __launch_bounds__(1024) k_max_abs2(const V* __restrict__ v, uint
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = sh[0];
+}
+
+template <class V>
+__global__ void __launch_bounds__(1024) k_max_abs2(const V* __restrict__ v, uint64_t n, double* __restrict__ out) {
+  __shared__ double sh[1024];
+  block_max_abs2(v, n, out, sh);
 }
 
 // P += w_f beta g; support projection (common.rs:287-300, pupil.rs:171-182)
@@ -330,27 +335,25 @@ __global__ void k_flag_reset(int* flag) {
 
 // weighted mean measured amplitude per low-resolution pixel (state.rs:288-313)
 // Per-pixel partial sums of the weighted mean amplitude over the frames of
-// chunk blockIdx.y ([y * chunk, (y + 1) * chunk), in order): part[(y * N + p)]
+// chunk y ([y * chunk, (y + 1) * chunk), in order): part[(y * N + p)]
 // = (sum w_f sqrt(max((y - bg) / g, 0)), sum w_f) over the chunk's
 // positive-weight, unmasked frames (state.rs:288-313).
-__global__ void k_init_average_part(const void* __restrict__ frames, int is_u16, const double* __restrict__ weights,
+__device__ inline void average_part(const void* __restrict__ frames, int is_u16, const double* __restrict__ weights,
                                     const uint8_t* __restrict__ masks, const double* __restrict__ gains,
                                     const double* __restrict__ bg, uint64_t bg_len, uint64_t F, uint32_t N,
-                                    uint64_t chunk, double2* __restrict__ part) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const uint64_t fa = (uint64_t)blockIdx.y * chunk, fb = fa + chunk < F ? fa + chunk : F;
+                                    uint64_t chunk, uint32_t y, uint32_t p, double2* __restrict__ part) {
+  const uint64_t fa = (uint64_t)y * chunk, fb = fa + chunk < F ? fa + chunk : F;
   double a = 0.0, w = 0.0;
   // frames in groups of G: the group's loads are issued before the
   // in-order accumulation uses them
   constexpr uint64_t G = 8;
   for (uint64_t f0 = fa; f0 < fb; f0 += G) {
-    double y[G];
+    double yv[G];
     uint8_t keep[G];
 #pragma unroll
     for (uint64_t k = 0; k < G; ++k) {
       const uint64_t f = f0 + k;
-      y[k] = f < fb ? meas_value(frames, is_u16, f * N + p) : 0.0;
+      yv[k] = f < fb ? meas_value(frames, is_u16, f * N + p) : 0.0;
       keep[k] = f < fb && (!masks || masks[f * N + p] != 0);
     }
 #pragma unroll
@@ -360,19 +363,26 @@ __global__ void k_init_average_part(const void* __restrict__ frames, int is_u16,
       const double fw = weights[f];
       if (fw == 0.0) continue;
       const double b = bg_value(bg, bg_len, N, f, p);
-      const double amp = sqrt(fmax((y[k] - b) / gains[f], 0.0));
+      const double amp = sqrt(fmax((yv[k] - b) / gains[f], 0.0));
       a += fw * amp;
       w += fw;
     }
   }
-  part[(uint64_t)blockIdx.y * N + p] = make_double2(a, w);
+  part[(uint64_t)y * N + p] = make_double2(a, w);
+}
+
+__global__ void k_init_average_part(const void* __restrict__ frames, int is_u16, const double* __restrict__ weights,
+                                    const uint8_t* __restrict__ masks, const double* __restrict__ gains,
+                                    const double* __restrict__ bg, uint64_t bg_len, uint64_t F, uint32_t N,
+                                    uint64_t chunk, double2* __restrict__ part) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  average_part(frames, is_u16, weights, masks, gains, bg, bg_len, F, N, chunk, blockIdx.y, p, part);
 }
 
 // The chunk partials summed in chunk order.
-__global__ void k_init_average_sum(const double2* __restrict__ part, uint32_t chunks, uint32_t N,
+__device__ inline void average_sum(const double2* __restrict__ part, uint32_t chunks, uint32_t N, uint32_t p,
                                    double* __restrict__ avg, double* __restrict__ aw) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
   double a = 0.0, w = 0.0;
   for (uint32_t c = 0; c < chunks; ++c) {
     const double2 v = part[(uint64_t)c * N + p];
@@ -383,11 +393,18 @@ __global__ void k_init_average_sum(const double2* __restrict__ part, uint32_t ch
   aw[p] = w;
 }
 
-// fallback amplitude (state.rs:314-326) into *fallback
-__global__ void __launch_bounds__(1024) k_init_fallback(const double* __restrict__ avg, const double* __restrict__ aw,
-                                                        uint32_t N, double* __restrict__ fallback_out,
-                                                        int* __restrict__ flag) {
-  __shared__ double sa[1024], sw[1024];
+__global__ void k_init_average_sum(const double2* __restrict__ part, uint32_t chunks, uint32_t N,
+                                   double* __restrict__ avg, double* __restrict__ aw) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  average_sum(part, chunks, N, p, avg, aw);
+}
+
+// fallback amplitude (state.rs:314-326) into *fallback_out, by one
+// 1024-thread block
+__device__ inline void block_fallback(const double* __restrict__ avg, const double* __restrict__ aw, uint32_t N,
+                                      double* __restrict__ fallback_out, int* __restrict__ flag, double* sa,
+                                      double* sw) {
   double ta = 0.0, tw = 0.0;
   for (uint32_t p = threadIdx.x; p < N; p += blockDim.x) {
     ta += avg[p];
@@ -410,25 +427,120 @@ __global__ void __launch_bounds__(1024) k_init_fallback(const double* __restrict
   }
 }
 
+__global__ void __launch_bounds__(1024) k_init_fallback(const double* __restrict__ avg, const double* __restrict__ aw,
+                                                        uint32_t N, double* __restrict__ fallback_out,
+                                                        int* __restrict__ flag) {
+  __shared__ double sa[1024], sw[1024];
+  block_fallback(avg, aw, N, fallback_out, flag, sa, sw);
+}
+
 // nearest-neighbour upsample of the mean amplitude (state.rs:327-336)
+__device__ inline double2 object_value(const double* __restrict__ avg, const double* __restrict__ aw, uint32_t nh,
+                                       uint32_t nw, uint32_t rh, uint32_t rw, double fallback, uint64_t i) {
+  const uint32_t r = (uint32_t)(i / rw), c = (uint32_t)(i % rw);
+  const uint32_t lr = (uint32_t)((uint64_t)r * nh / rh), lc = (uint32_t)((uint64_t)c * nw / rw);
+  const uint32_t lp = lr * nw + lc;
+  const double v = aw[lp] > 0.0 ? avg[lp] / aw[lp] : fallback;
+  return make_double2(v, 0.0);
+}
+
 __global__ void k_init_object(const double* __restrict__ avg, const double* __restrict__ aw, uint32_t nh, uint32_t nw,
                               uint32_t rh, uint32_t rw, const double* __restrict__ fallback,
                               double2* __restrict__ object) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint64_t)rh * rw) return;
-  const uint32_t r = (uint32_t)(i / rw), c = (uint32_t)(i % rw);
-  const uint32_t lr = (uint32_t)((uint64_t)r * nh / rh), lc = (uint32_t)((uint64_t)c * nw / rw);
-  const uint32_t lp = lr * nw + lc;
-  const double v = aw[lp] > 0.0 ? avg[lp] / aw[lp] : *fallback;
-  object[i] = make_double2(v, 0.0);
+  object[i] = object_value(avg, aw, nh, nw, rh, rw, *fallback, i);
+}
+
+// destination of element i under a cyclic shift by (rs, cs) of an h x w array
+__device__ inline uint64_t shifted(uint64_t i, uint32_t h, uint32_t w, uint32_t rs, uint32_t cs) {
+  const uint32_t r = (uint32_t)(i / w), c = (uint32_t)(i % w);
+  return (uint64_t)((r + rs) % h) * w + (c + cs) % w;
 }
 
 __global__ void k_shift(const double2* __restrict__ src, double2* __restrict__ dst, uint32_t h, uint32_t w,
                         uint32_t rs, uint32_t cs) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint64_t)h * w) return;
-  const uint32_t r = (uint32_t)(i / w), c = (uint32_t)(i % w);
-  dst[(uint64_t)((r + rs) % h) * w + (c + cs) % w] = src[i];
+  dst[shifted(i, h, w, rs, cs)] = src[i];
+}
+
+// ---- batched forms over InitTile arrays (grid y or z = tile) ----
+
+__global__ void k_binit_reset(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.x];
+  if (threadIdx.x < 4) T.flag[threadIdx.x] = 0;
+  if (threadIdx.x == 0 && T.counter) *T.counter = 0;
+}
+
+__global__ void k_binit_average_part(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.z];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.y >= T.chunks || p >= T.N) return;
+  average_part(T.frames, (int)T.is_u16, T.weights, T.masks, T.model_gains, T.model_bg, T.bg_len, T.F, T.N, T.chunk,
+               blockIdx.y, p, T.part);
+}
+
+__global__ void k_binit_average_sum(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.y];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= T.N) return;
+  average_sum(T.part, T.chunks, T.N, p, T.avg, T.aw);
+}
+
+__global__ void __launch_bounds__(1024) k_binit_fallback(const InitTile* __restrict__ tiles) {
+  __shared__ double sa[1024], sw[1024];
+  const InitTile& T = tiles[blockIdx.x];
+  block_fallback(T.avg, T.aw, T.N, T.scal + 2, T.flag, sa, sw);
+}
+
+__global__ void k_binit_object(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.y];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)T.rh * T.rw) return;
+  T.slot[i] = object_value(T.avg, T.aw, T.nh, T.nw, T.rh, T.rw, T.scal[2], i);
+}
+
+// pupil (complex64), gains and background of the state from the model
+__global__ void k_binit_pupil(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.y];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < T.N) T.pupil[i] = from_d2<float2>(T.model_pupil[i]);
+  if (i < T.F) T.gains[i] = T.model_gains[i];
+  if (i < T.bg_len) T.bg[i] = T.model_bg[i];
+}
+
+__global__ void __launch_bounds__(1024) k_binit_max_pupil(const InitTile* __restrict__ tiles) {
+  __shared__ double sh[1024];
+  const InitTile& T = tiles[blockIdx.x];
+  block_max_abs2(T.pupil, T.N, T.scal, sh);
+}
+
+__global__ void k_binit_finish(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.y];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)T.rh * T.rw) return;
+  const double2 v = T.slot[i];
+  const uint64_t j = shifted(i, T.rh, T.rw, T.rh / 2, T.rw / 2);
+  T.spec[j] = v;
+  if (T.vel) {
+    T.anc[j] = from_d2<float2>(v);
+    T.vel[j] = make_float2(0.0f, 0.0f);
+  }
+}
+
+__global__ void k_binit_flags(const InitTile* __restrict__ tiles, uint32_t count, int* __restrict__ flags) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  flags[2 * t] = tiles[t].flag[0];
+  flags[2 * t + 1] = tiles[t].flag[1];
+}
+
+__global__ void k_bobject_shift(const InitTile* __restrict__ tiles) {
+  const InitTile& T = tiles[blockIdx.y];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)T.rh * T.rw) return;
+  T.slot[shifted(i, T.rh, T.rw, (T.rh + 1) / 2, (T.rw + 1) / 2)] = T.spec[i];
 }
 
 template <class V>
@@ -608,15 +720,20 @@ void launch_flag_reset(const fpmhip_session& s) {
   HIP_CHECK(hipGetLastError());
 }
 
+// frames in up to 8 ordered chunks (partials in the init scratch work_m,
+// max(M, N) complex128 slots), so that the pass fills the device
+uint32_t init_chunks(const fpmhip_session& s) {
+  const Model& m = *s.model;
+  const uint64_t slots = s.b.work_m.bytes / sizeof(double2);
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({8, slots / m.N(), m.F ? m.F : 1}));
+}
+
 void launch_init(const fpmhip_session& s) {
   const Model& m = *s.model;
   const Measurements& me = *s.meas;
   const uint64_t N = m.N();
   hipStream_t st = s.ctx->stream;
-  // frames in up to 8 ordered chunks (partials in the init scratch work_m,
-  // max(M, N) complex128 slots), so that the pass fills the device
-  const uint64_t slots = s.b.work_m.bytes / sizeof(double2);
-  const uint32_t chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({8, slots / N, m.F ? m.F : 1}));
+  const uint32_t chunks = init_chunks(s);
   const uint64_t chunk = (m.F + chunks - 1) / chunks;
   hipLaunchKernelGGL(k_init_average_part, dim3(blocks_for(N), chunks), dim3(kBlock), 0, st, me.d_frames.ptr,
                      me.is_u16 ? 1 : 0, me.d_weights.as<double>(), me.d_masks.as<uint8_t>(), m.d_gains.as<double>(),
@@ -635,6 +752,43 @@ void launch_init(const fpmhip_session& s) {
   HIP_CHECK(hipGetLastError());
   fft2_f64(*m.fft_high, s.b.work_m.as<double2>(), 1, false, st);
   launch_shift(s.b.work_m.as<double2>(), s.b.spec.as<double2>(), m.rh, m.rw, m.rh / 2, m.rw / 2, st);
+}
+
+void launch_init_batch_pre(const InitTile* tiles, uint32_t count, uint32_t nmax, uint32_t mmax, uint64_t cmax,
+                           hipStream_t st) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_binit_reset, dim3(count), dim3(64), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_binit_average_part, dim3(blocks_for(nmax), 8, count), dim3(kBlock), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_binit_average_sum, dim3(blocks_for(nmax), count), dim3(kBlock), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_binit_fallback, dim3(count), dim3(1024), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_binit_object, dim3(blocks_for(mmax), count), dim3(kBlock), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_binit_pupil, dim3(blocks_for(cmax), count), dim3(kBlock), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_binit_max_pupil, dim3(count), dim3(1024), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_init_batch_post(const InitTile* tiles, uint32_t count, uint32_t mmax, hipStream_t st) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_binit_finish, dim3(blocks_for(mmax), count), dim3(kBlock), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_init_batch_flags(const InitTile* tiles, uint32_t count, int* flags, hipStream_t st) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_binit_flags, dim3(blocks_for(count)), dim3(kBlock), 0, st, tiles, count, flags);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_object_batch_shift(const InitTile* tiles, uint32_t count, uint32_t mmax, hipStream_t st) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_bobject_shift, dim3(blocks_for(mmax), count), dim3(kBlock), 0, st, tiles);
+  HIP_CHECK(hipGetLastError());
 }
 
 void launch_object(const fpmhip_session& s, double2* out_dev) {

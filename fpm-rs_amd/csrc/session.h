@@ -124,4 +124,49 @@ void launch_fill_zero(float2* dst, uint64_t n, hipStream_t stream);
 void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields,
                           double* out, hipStream_t stream);
 
+// ---- batched initialisation and result transform of MIXED_F32 tiles ----
+// One entry per tile, in a device array: every pass below is one launch for
+// all tiles (grid y or z = tile), computing what session_initialize_launch
+// and launch_object compute for each tile, element for element.
+struct InitTile {
+  const void* frames;
+  const double* weights;
+  const uint8_t* masks;
+  const double* model_gains;   // F (ones where the model has none)
+  const double* model_bg;      // bg_len
+  const double2* model_pupil;  // N, complex128
+  double2* part;               // chunk partials: the session's init scratch
+  double* avg;
+  double* aw;
+  double* scal;
+  int* flag;
+  uint64_t* counter;  // the batch's per-tile mPIE counter
+  double2* slot;      // the tile's R-array in the batch scratch
+  double2* spec;
+  float2* pupil;
+  float2* vel;  // null without momentum
+  float2* anc;
+  double* gains;
+  double* bg;
+  uint64_t F, chunk, bg_len;
+  uint32_t N, nh, nw, rh, rw, chunks, is_u16, pad;
+};
+// Chunking of the initial mean amplitude for a session (also used by
+// launch_init, so that both sum the frames in the same order).
+uint32_t init_chunks(const fpmhip_session& s);
+// Reset, mean amplitude, fallback and nearest-neighbour object into each
+// tile's slot (forward transform not included); pupil, gains and background
+// copies and the pupil maximum. `nmax`, `mmax`: largest N and R of the tiles;
+// `cmax`: largest of N, F and the background length.
+void launch_init_batch_pre(const InitTile* tiles, uint32_t count, uint32_t nmax, uint32_t mmax, uint64_t cmax,
+                           hipStream_t stream);
+// Each slot (transformed) shifted into its tile's spectrum, with the mPIE
+// anchor (= spectrum, complex64) and zero velocity.
+void launch_init_batch_post(const InitTile* tiles, uint32_t count, uint32_t mmax, hipStream_t stream);
+// flags[2 t], flags[2 t + 1] = tile t's flag[0], flag[1].
+void launch_init_batch_flags(const InitTile* tiles, uint32_t count, int* flags, hipStream_t stream);
+// Each tile's spectrum, inverse-shifted into its slot (the inverse transform
+// of the slots completes the object, as launch_object does).
+void launch_object_batch_shift(const InitTile* tiles, uint32_t count, uint32_t mmax, hipStream_t stream);
+
 }  // namespace fpmhip

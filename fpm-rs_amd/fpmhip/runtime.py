@@ -28,7 +28,7 @@ EXPORTS = [
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
     "fpmhip_session_calibration_present", "fpmhip_session_clear_calibration",
     "fpmhip_session_shape", "fpmhip_batch_create", "fpmhip_batch_destroy", "fpmhip_batch_run",
-    "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_objects_pinned", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
+    "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_objects_pinned", "fpmhip_batch_objects_device", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
     "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
     "fpmhip_server_poll", "fpmhip_server_wait", "fpmhip_server_result", "fpmhip_server_stats",
 ]
@@ -469,6 +469,14 @@ class Batch:
             out.append(flat[off:off + h * w].reshape(h, w))
             off += h * w
         return out
+
+    def objects_device(self):
+        """Every tile's result object left in device memory: a list of
+        (device address, (R_h, R_w)) per tile, complex128 row-major, valid
+        until the next run() or objects call on this batch."""
+        ptrs = (ct.c_void_p * len(self.sessions))()
+        check(lib().fpmhip_batch_objects_device(self.handle, ptrs))
+        return [(int(p or 0), tuple(s.problem.model.reconstruction_shape)) for p, s in zip(ptrs, self.sessions)]
 
     def timing(self):
         mean, mx, n = ct.c_double(), ct.c_double(), ct.c_uint64()

@@ -310,9 +310,14 @@ int fpmhip_batch_timing(const fpmhip_batch* batch, double* sweep_ms_mean, double
                         uint64_t* sweeps);
 /* Every tile's result object (state_object, result.rs:397-413), in session
  * order, concatenated into `out` (sum of the tiles' R_h * R_w complex128
- * values, interleaved): fpmhip_session_get_object for all tiles, with the
- * launches of all tiles queued before the copies and one wait at the end. */
+ * values, interleaved): fpmhip_session_get_object for all tiles, computed by
+ * one launch for all tiles and one batched FFT per distinct R, then copied. */
 int fpmhip_batch_objects(fpmhip_batch* batch, double* out);
+/* The same objects left in device memory (HBM): objects[count] receives one
+ * device pointer per tile, in session order, to its R_h x R_w row-major
+ * complex128 object (interleaved). Returns once they are complete; valid
+ * until the next run or objects call on this batch, or its destruction. */
+int fpmhip_batch_objects_device(fpmhip_batch* batch, const double** objects);
 /* The same into a page-locked host buffer owned by the batch (allocated on
  * first use, freed with the batch), for full-rate PCIe copies: *out points
  * at the concatenated objects and stays valid until the next call on this

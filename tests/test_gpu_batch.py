@@ -222,6 +222,64 @@ def test_batch_objects_match_session_objects(ctx):
         assert np.array_equal(q, o)
 
 
+def test_batched_initialisation_matches_session_initialisation(ctx):
+    """The batch initialises all tiles with one launch per pass and one FFT
+    per distinct R; every tile's state (spectrum, pupil, velocity, anchor,
+    counter) is bit for bit what Session.initialize gives, for uneven tiles
+    (two 64^2 tiles around a 128^2 one, a 256^2 tile with masks, weights and
+    a per-frame background, a 32^2 tile with a shared background)."""
+    say("batched init vs session init")
+    alg = Mpie(iterations=1, momentum_interval=11, momentum_friction=0.5, momentum_feedback=0.5)
+    tiles = [tile(64, 5, 161), tile(128, 7, 162), tile(64, 7, 163), tile(256, 5, 164), tile(32, 5, 165)]
+    rng = np.random.default_rng(166)
+    model, meas = tiles[3]
+    F = model.frame_count
+    model.with_background(rng.uniform(0.0, 3.0, (F,) + tuple(model.image_shape)))
+    w = np.ones(F)
+    w[2] = 0.0
+    w[4] = 0.25
+    masks = (rng.random((F,) + tuple(model.image_shape)) > 0.05).astype(np.uint8)
+    tiles[3] = (model, MeasurementStack(counts=meas.counts, weights=w, masks=masks))
+    tiles[4][0].with_background(np.full(tiles[4][0].image_shape, 1.5))
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    for rep in range(2):  # a second run re-initialises from scratch
+        b.run(1 if rep else 0)
+        if rep:
+            b.run(0)
+        for (mo, me), s in zip(tiles, ss):
+            ref = runtime.Session(ReconstructionProblem(me, mo), alg, precision="f32")
+            ref.initialize()
+            assert np.array_equal(s.spectrum(), ref.spectrum()), mo.image_shape
+            assert np.array_equal(s.pupil(), ref.pupil())
+            v, a, c = s.momentum()
+            rv, ra, rc = ref.momentum()
+            assert np.array_equal(v, rv) and np.array_equal(a, ra) and c == rc == 0
+            assert np.array_equal(s.object(), ref.object())
+
+
+def test_device_objects_are_the_host_objects(ctx):
+    """fpmhip_batch_objects_device leaves each tile's object in HBM: copied
+    back with hipMemcpy it is the object fpmhip_batch_objects returns."""
+    say("device-resident objects")
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    hip.hipMemcpy.restype = ct.c_int
+    hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    alg = Fpie(iterations=2)
+    tiles = [tile(64, 5, 171), tile(128, 5, 172), tile(64, 5, 173)]
+    ss = sessions_for(tiles, alg)
+    b = runtime.Batch(ss)
+    b.run(2)
+    host = [o.copy() for o in b.objects()]
+    dev = b.objects_device()
+    assert len({p for p, _ in dev}) == len(dev) and all(p for p, _ in dev)
+    for (ptr, shape), h in zip(dev, host):
+        out = np.empty(shape, np.complex128)
+        assert hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0  # hipMemcpyDeviceToHost
+        assert np.array_equal(out, h)
+
+
 def test_launch_order_does_not_change_results(ctx):
     alg = Mpie(iterations=2, momentum_friction=0.5, momentum_feedback=0.5)
     tiles = [tile(64, 5, 111), tile(256, 5, 112), tile(32, 5, 113)]
