@@ -134,28 +134,54 @@ __host__ __device__ constexpr int pitch() {
   return N + 1;
 }
 
+// The lines of a stage that carry data: nl lines starting at line l0,
+// cyclically (all lines when LIVE is off).
+struct LiveRows {
+  int nl;     // number of live lines
+  int l0;     // first live line
+  float rcp;  // 1 / nl
+};
+
 // One Stockham stage (radix P, stride S) over LINES lines of length LEN in
 // LDS; sample e of line l sits at l * LS + e * ES. Consecutive lanes take
 // consecutive lines, so with LS = 1 or LS = an odd number of 8-byte words
 // every LDS access of a wave is conflict free. Loads complete (barrier)
-// before any store, so the stage runs in place in one buffer.
-template <int LEN, int LINES, int P, int S, int LS, int ES, bool INV>
-__device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid) {
+// before any store, so the stage runs in place in one buffer. With LIVE,
+// only the live lines are transformed: the work items are packed onto the
+// first threads, so whole waves skip the stage.
+template <int LEN, int LINES, int P, int S, int LS, int ES, bool INV, bool LIVE = false>
+__device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid, LiveRows lr = {LINES, 0, 0.0f}) {
   asm volatile("" : "+v"(tid));
   constexpr int M = LEN / (P * S);
   constexpr int BPL = LEN / P;
   constexpr int TOTAL = LINES * BPL;
   constexpr int PER = (TOTAL + kThreads - 1) / kThreads;
+  const int total = LIVE ? lr.nl * BPL : TOTAL;
   float2 a[PER][P];
   int line[PER], pp[PER], q[PER];
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int b = tid + u * kThreads;
-    line[u] = b % LINES;
-    const int t = b / LINES;
+    int t;
+    if (LIVE) {
+      // b = t * nl + li, the quotient from the FP32 reciprocal, corrected
+      t = (int)((float)b * lr.rcp);
+      int li = b - t * lr.nl;
+      if (li < 0) {
+        --t;
+        li += lr.nl;
+      } else if (li >= lr.nl) {
+        ++t;
+        li -= lr.nl;
+      }
+      line[u] = (lr.l0 + li) & (LINES - 1);
+    } else {
+      line[u] = b % LINES;
+      t = b / LINES;
+    }
     pp[u] = t / S;
     q[u] = t % S;
-    if (b < TOTAL) {
+    if (b < total) {
 #pragma unroll
       for (int r = 0; r < P; ++r) {
         const int e = q[u] + S * (pp[u] + r * M);
@@ -167,7 +193,7 @@ __device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid) {
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int b = tid + u * kThreads;
-    if (b < TOTAL) {
+    if (b < total) {
       reg_dft<P, INV>(a[u]);
 #pragma unroll
       for (int k = 0; k < P; ++k) {
@@ -203,16 +229,26 @@ struct DefaultPlan<256> {
   static constexpr int P0 = 16;
 };
 
-// Unscaled 2-D DFT in LDS with the direction given by `tw`.
+// Unscaled 2-D DFT in LDS with the direction given by `tw`. Only the rows
+// in `live` can be non-zero on input (inverse) or are read on output
+// (forward): the inverse transforms those rows and then every column; the
+// forward transforms every column and then those rows.
 template <int N, bool INV, int P0>
-__device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw, int tid) {
+__device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw, int tid, LiveRows live) {
   constexpr int P1 = N / P0;
   constexpr int LD = pitch<N>();
   static_assert(P0 * P1 == N && P0 <= 16 && P1 <= 16, "plan");
-  stage<N, N, P0, 1, LD, 1, INV>(lds, tw, tid);
-  stage<N, N, P1, P0, LD, 1, INV>(lds, tw, tid);
-  stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
-  stage<N, N, P1, P0, 1, LD, INV>(lds, tw, tid);
+  if (INV) {
+    stage<N, N, P0, 1, LD, 1, INV, true>(lds, tw, tid, live);
+    stage<N, N, P1, P0, LD, 1, INV, true>(lds, tw, tid, live);
+    stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
+    stage<N, N, P1, P0, 1, LD, INV>(lds, tw, tid);
+  } else {
+    stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
+    stage<N, N, P1, P0, 1, LD, INV>(lds, tw, tid);
+    stage<N, N, P0, 1, LD, 1, INV, true>(lds, tw, tid, live);
+    stage<N, N, P1, P0, LD, 1, INV, true>(lds, tw, tid, live);
+  }
 }
 
 // The thread index, hidden from the optimiser so that per-thread addresses
@@ -890,6 +926,24 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   const uint32_t ndest = __builtin_amdgcn_readfirstlane(gptr(T.dest)[0]);
   const uint32_t nsrc = __builtin_amdgcn_readfirstlane(gptr(T.dest)[1]);
   uint64_t counter = *T.counter;
+  // Rows of the transforms that carry data: the centred rows rlo..rhi with
+  // pupil support, i.e. the field rows (rlo + N/2 + i) mod N, i < nl (the
+  // patch x pupil is zero elsewhere and U is read only there).
+  LiveRows live;
+  {
+    int rlo = N, rhi = -1;
+    for (int r = 0; r < N; ++r) {
+      const uint32_t e = ext[r];
+      if ((e & 0xFFFFu) < (e >> 16)) {
+        rlo = rlo < r ? rlo : r;
+        rhi = r;
+      }
+    }
+    const int nl = rhi >= rlo ? rhi - rlo + 1 : 0;
+    live.nl = __builtin_amdgcn_readfirstlane(nl);
+    live.l0 = __builtin_amdgcn_readfirstlane((rlo + HALF) & (N - 1));
+    live.rcp = live.nl ? 1.0f / (float)live.nl : 0.0f;
+  }
   if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
   __syncthreads();
   uint32_t slot = 0;
@@ -923,7 +977,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       uint32_t next_f = 0, next_w = 0;
       if (fetch) next_f = frame_at(T, npos);
       // 2. inverse FFT (unscaled)
-      fft2_lds<N, true, RADIX>(lds, twi, tid);
+      fft2_lds<N, true, RADIX>(lds, twi, tid, live);
       if (fetch) next_w = rec_word(T, next_f, threadIdx.x);
       // 3. amplitude replacement and objective terms (pointwise, natural order).
       //    The LDS field becomes psi' - psi: the forward transform is linear,
@@ -994,7 +1048,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       }
       __syncthreads();
       // 4. forward FFT (x 1/N applied in phase 5)
-      fft2_lds<N, false, RADIX>(lds, twf, tid);
+      fft2_lds<N, false, RADIX>(lds, twf, tid, live);
       if (fetch) rn[threadIdx.x] = next_w;
       // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
       uint32_t dfirst[kInsertGroup];
@@ -1177,6 +1231,24 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   uint64_t counter = *T.counter;
+  // Rows of the transforms that carry data: the centred rows rlo..rhi with
+  // pupil support, i.e. the field rows (rlo + N/2 + i) mod N, i < nl (the
+  // patch x pupil is zero elsewhere and U is read only there).
+  LiveRows live;
+  {
+    int rlo = N, rhi = -1;
+    for (int r = 0; r < N; ++r) {
+      const uint32_t e = ext[r];
+      if ((e & 0xFFFFu) < (e >> 16)) {
+        rlo = rlo < r ? rlo : r;
+        rhi = r;
+      }
+    }
+    const int nl = rhi >= rlo ? rhi - rlo + 1 : 0;
+    live.nl = __builtin_amdgcn_readfirstlane(nl);
+    live.l0 = __builtin_amdgcn_readfirstlane((rlo + HALF) & (N - 1));
+    live.rcp = live.nl ? 1.0f / (float)live.nl : 0.0f;
+  }
   if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
   __syncthreads();
   uint32_t slot = 0;
