@@ -446,6 +446,56 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
 
 namespace fpmhip {
 
+uint64_t momentum_span_capacity(const Model& m) { return 1 + (uint64_t)m.rh * ((m.rw + 7 + 63) / 64); }
+
+void upload_momentum_spans(fpmhip_session& s, const double* pupil) {
+  if (!s.b.mspan.ptr) return;
+  const Model& m = *s.model;
+  const int nh = (int)m.nh, nw = (int)m.nw;
+  // columns [lo, hi) of each pupil row that are non-zero in complex64
+  std::vector<int> lo(nh, nw), hi(nh, 0);
+  for (int r = 0; r < nh; ++r)
+    for (int c = 0; c < nw; ++c) {
+      const uint64_t p = (uint64_t)r * nw + c;
+      if ((float)pupil[2 * p] != 0.0f || (float)pupil[2 * p + 1] != 0.0f) {
+        lo[r] = std::min(lo[r], c);
+        hi[r] = c + 1;
+      }
+    }
+  // window row a (0..nh) receives the taps of pupil rows a - 1 and a, reaching
+  // one column further right
+  std::vector<int> wlo(nh + 1, nw + 1), whi(nh + 1, 0);
+  for (int a = 0; a <= nh; ++a)
+    for (int r = a > 0 ? a - 1 : 0; r <= a && r < nh; ++r)
+      if (hi[r] > lo[r]) {
+        wlo[a] = std::min(wlo[a], lo[r]);
+        whi[a] = std::max(whi[a], std::min(hi[r] + 1, nw + 1));
+      }
+  // per object row, the hull of the columns any source's window reaches
+  std::vector<int64_t> ylo(m.rh, std::numeric_limits<int64_t>::max()), yhi(m.rh, 0);
+  for (uint64_t src = 0; src < m.S; ++src)
+    for (int a = 0; a <= nh; ++a) {
+      if (whi[a] <= wlo[a]) continue;
+      const uint64_t y = (uint64_t)m.ax_r[src].lower + a;
+      if (y >= m.rh) continue;
+      const int64_t c0 = (int64_t)m.ax_c[src].lower + wlo[a];
+      const int64_t c1 = std::min<int64_t>((int64_t)m.ax_c[src].lower + whi[a], m.rw);
+      ylo[y] = std::min(ylo[y], c0);
+      yhi[y] = std::max(yhi[y], c1);
+    }
+  std::vector<uint32_t> spans(1, 0);
+  for (uint32_t y = 0; y < m.rh; ++y)
+    for (int64_t c = ylo[y] & ~int64_t{7}; c < yhi[y]; c += 64) {  // chunks start on 128-byte lines
+      const uint32_t len = (uint32_t)std::min<int64_t>(64, yhi[y] - c);
+      spans.push_back((uint32_t)((uint64_t)y * m.rw + (uint64_t)c) | ((len - 1) << 25));
+    }
+  spans[0] = (uint32_t)(spans.size() - 1);
+  if (spans.size() > momentum_span_capacity(m)) fail(FPMHIP_DEVICE_ERROR, "momentum span table overflow");
+  HIP_CHECK(hipMemcpyAsync(s.b.mspan.ptr, spans.data(), spans.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           s.ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));  // `spans` is pageable host memory
+}
+
 void session_initialize_launch(fpmhip_session& s) {
   for (uint64_t f = 0; f < s.model->F; ++f) {
     const double g = s.model->has_gain ? s.model->gain[f] : 1.0;
@@ -458,6 +508,7 @@ void session_initialize_launch(fpmhip_session& s) {
                        s.ctx->stream);
   if (!s.extent_from_model) {  // the model's pupil: its extents and lists do not change
     launch_pupil_extent(s);
+    upload_momentum_spans(s, s.model->pupil.data());
     s.extent_from_model = true;
   }
   load_calibration(s);
@@ -556,6 +607,8 @@ int fpmhip_model_create(fpmhip_context* c, const fpmhip_model_desc* d, fpmhip_mo
       const Axis ar = interpolation_axis(r0, m->nh, m->rh, orow);
       const Axis ac = interpolation_axis(c0, m->nw, m->rw, ocol);
       sp[s] = SrcParam{r0, c0, ar, ac, off_zero(orow, ocol) ? 1 : 0, 0};
+      m->ax_r.push_back(ar);
+      m->ax_c.push_back(ac);
     }
     m->pupil.assign(d->pupil, d->pupil + 2 * N);
     for (double v : m->pupil)
@@ -763,6 +816,8 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
     }
     launch_convert_pupil(m->d_pupil64.as<double2>(), b.pupil.ptr, precision == FPMHIP_MIXED_F32, N, m->ctx->stream);
+    if (precision == FPMHIP_MIXED_F32 && s->cfg.momentum && m->M() < (1ull << 25))
+      b.mspan.allocate(momentum_span_capacity(*m) * sizeof(uint32_t));
     if (precision == FPMHIP_MIXED_F32) {
       b.extent.allocate(m->nh * sizeof(uint32_t));
       if (m->nh == m->nw && m->nh <= kFusedLdsMaxN)
@@ -770,6 +825,7 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       launch_pupil_extent(*s);
       s->extent_from_model = true;
     }
+    upload_momentum_spans(*s, m->pupil.data());
     launch_flag_reset(*s);
     HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
     *out = s.release();
@@ -809,6 +865,7 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
     HIP_CHECK(hipMemcpyAsync(s->b.spec.ptr, spectrum, M * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
     upload_pupil64(*s, pupil);
     launch_pupil_extent(*s);
+    upload_momentum_spans(*s, pupil);
     s->extent_from_model = false;
     if (frame_gains) {
       for (uint64_t f = 0; f < s->model->F; ++f)

@@ -72,6 +72,7 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.bg_len = s.bg_len;
   t.src = m.d_src.as<SrcParam>();
   t.order = nullptr;
+  t.mspan = s.b.mspan.as<uint32_t>();
   t.trace = trace;
   t.frame_loss = frame_loss;
   t.counter = reinterpret_cast<uint64_t*>(per16);
@@ -167,6 +168,10 @@ bool sweep_supports(const fpmhip_session& s, const char** why) {
   }
   if (s.alg.kind == FPMHIP_EPRY) {
     *why = "batched tiles do not support EPRY";
+    return false;
+  }
+  if (s.cfg.momentum && !s.b.mspan.ptr) {  // chunk starts are 25-bit pixel indices
+    *why = "batched mPIE tiles need an object grid below 2^25 pixels";
     return false;
   }
   return true;
@@ -353,6 +358,7 @@ void initialize_tiles(fpmhip_batch& b) {
   for (fpmhip_session* s : b.sessions)
     if (!s->extent_from_model) {  // after the pupil copy: extents of the model's pupil
       launch_pupil_extent(*s);
+      upload_momentum_spans(*s, s->model->pupil.data());
       s->extent_from_model = true;
     }
   for (const fpmhip_batch::SlotGroup& g : b.slot_groups) fft2_f64(*g.plan, g.base, g.count, false, st);

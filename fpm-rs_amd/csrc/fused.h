@@ -25,6 +25,7 @@ struct TileDesc {
   uint64_t bg_len;
   const SrcParam* src;    // [F] per-source crop parameters
   const uint32_t* order;  // [F] frame order, or null for sequential
+  const uint32_t* mspan;  // momentum chunks (null: the event visits every object pixel)
   double* trace;          // [trace_len] iteration objectives
   double* frame_loss;     // [F] objective of each order position, last sweep
   uint64_t* counter;      // mPIE effective-frame counter (persists across sweeps)

@@ -486,28 +486,39 @@ __device__ __forceinline__ void momentum_event(const TileDesc& T, int tid, uint3
   constexpr int AUX = kAux<MEM>;
   const uint32_t M = T.rh * T.rw;
   bool bad = false;
-  constexpr int U = 4;  // loads of U elements in flight before their use
-  // branch-free: elements past M load zero and store nothing
+  constexpr int U = 4;  // chunks in flight per wave before their use
+  // branch-free: pixels past a chunk load zero and store nothing
   const Rsrc rs = make_rsrc(T.spec, M * 16u), rv = make_rsrc(T.vel, M * 8u), ra = make_rsrc(T.anc, M * 8u);
-  for (uint32_t base = member * U * kThreads + tid; base < M; base += members * U * kThreads) {
+  // Only the pixels an LED update can change, as chunks of up to 64 pixels
+  // of one row (TileDesc::mspan): one chunk per wave and step, U chunks in
+  // flight. Elsewhere the velocity stays zero and the object equals the
+  // anchor, so the event would leave them unchanged (common.rs:355-387).
+  const auto span = gptr(T.mspan);
+  const uint32_t nspan = __builtin_amdgcn_readfirstlane(span[0]);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = members * (kThreads / 64);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(member * (kThreads / 64) + (threadIdx.x >> 6));
+  for (uint32_t c0 = wave * U; c0 < nspan; c0 += waves * U) {
     double2 o[U];
     float2 v[U], an[U];
+    uint32_t idx[U];
     bool live[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t i = base + u * kThreads;
-      live[u] = i < M;
-      o[u] = bld_d2<AUX>(rs, live[u] ? i * 16u : kSkip);
-      v[u] = bld_f2<AUX>(rv, live[u] ? i * 8u : kSkip);
-      an[u] = bld_f2<AUX>(ra, live[u] ? i * 8u : kSkip);
+      const uint32_t c = c0 + u;
+      const uint32_t e = span[1 + (c < nspan ? c : nspan - 1)];
+      idx[u] = (e & 0x1FFFFFFu) + lane;
+      live[u] = c < nspan && lane <= (e >> 25);
+      o[u] = bld_d2<AUX>(rs, live[u] ? idx[u] * 16u : kSkip);
+      v[u] = bld_f2<AUX>(rv, live[u] ? idx[u] * 8u : kSkip);
+      an[u] = bld_f2<AUX>(ra, live[u] ? idx[u] * 8u : kSkip);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t i = base + u * kThreads;
       momentum_element(T, o[u], v[u], an[u], bad);
-      bst_f2<AUX>(rv, live[u] ? i * 8u : kSkip, v[u]);
-      bst_d2<AUX>(rs, live[u] ? i * 16u : kSkip, o[u]);
-      bst_f2<AUX>(ra, live[u] ? i * 8u : kSkip, an[u]);
+      bst_f2<AUX>(rv, live[u] ? idx[u] * 8u : kSkip, v[u]);
+      bst_d2<AUX>(rs, live[u] ? idx[u] * 16u : kSkip, o[u]);
+      bst_f2<AUX>(ra, live[u] ? idx[u] * 8u : kSkip, an[u]);
     }
   }
   if (bad) T.status[0] = 1;

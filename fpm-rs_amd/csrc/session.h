@@ -57,6 +57,9 @@ struct SessionBuffers {
                           // tiles up to the LDS-resident size): [0] destinations of the adjoint
                           // insert, [1] support pixels; then the destinations a | b << 16 and the
                           // support pixels r | c << 16, row-major
+  DeviceBuffer mspan;     // uint32 momentum chunks of the fused sweep (MIXED_F32 mPIE): [0] count, then
+                          // start | (length - 1) << 25, up to 64 object pixels of one row each, covering
+                          // every pixel an LED update can change
   DeviceBuffer bg;        // double[F * N] background of the reconstruction state (bg_len used)
   DeviceBuffer gains;     // double[F] frame gains of the state (1 where the model has none)
   DeviceBuffer calib;     // double[nblk * 2] block partial sums of the calibration updates
@@ -123,6 +126,12 @@ void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream);
 void launch_fill_zero(float2* dst, uint64_t n, hipStream_t stream);
 void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields,
                           double* out, hipStream_t stream);
+
+// Capacity (uint32 entries) of a session's momentum chunk table, and its
+// upload for a given pupil (interleaved complex128; the table follows the
+// pupil's complex64 support and the model's source windows).
+uint64_t momentum_span_capacity(const Model& m);
+void upload_momentum_spans(fpmhip_session& s, const double* pupil);
 
 // ---- batched initialisation and result transform of MIXED_F32 tiles ----
 // One entry per tile, in a device array: every pass below is one launch for
