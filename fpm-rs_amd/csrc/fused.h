@@ -14,8 +14,9 @@ struct TileDesc {
   float2* anc;            // [R] mPIE anchor (or null), complex64 storage
   const float2* pupil;    // [N] complex64 pupil
   const uint32_t* extent; // [n] per-row support of the pupil: columns [lo, hi) as lo | hi << 16
-  const uint32_t* dest;   // live-pixel lists, tiles <= 128^2: [0] insert destinations, [1] support pixels,
-                          // then (n+1)^2 slots of destinations a | b << 16 and the support pixels r | c << 16
+  const uint32_t* dest;   // live-pixel lists, tiles <= 128^2: [0] insert destinations, [1] support
+                          // strips, then (n+1)^2 slots of destinations a | b << 16 and the strips
+                          // r | c << 8 | mask << 16 (SessionBuffers::dest)
   float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)
   const void* frames;     // [F*N] uint16 counts or f64 frames
   const uint8_t* masks;   // [F*N] or null

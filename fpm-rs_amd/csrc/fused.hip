@@ -742,17 +742,22 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
 }
 
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
-// pixels `sl` (count ns; r | c << 16, centred patch coordinates), written to
-// the N x N LDS field at the ifftshifted position; the rest of the field is
-// zero (written first, then a barrier). Each pixel's pupil value and four
-// spectrum taps are independent loads, G pixels per thread in flight; an
-// axis without a fraction has an upper weight of exactly zero and its
-// (skipped, zero) upper taps add nothing.
-template <int N, int G = 3>
-__device__ __forceinline__ void gather_list(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
-                                            const uint32_t* sl, uint32_t ns, float2* lds) {
-  constexpr int LD = pitch<N>(), HALF = N / 2, NN = N * N;
+// strips `sl` (count ns; kGatherStrip vertically consecutive pixels of one
+// column, centred patch coordinates, with a mask of the pixels in the
+// support), written to the N x N LDS field at the ifftshifted position; the
+// rest of the field is zero (written first, then a barrier). Consecutive
+// pixels of a strip share a row of taps: S + 1 rows of two spectrum taps and
+// S pupil values per strip, all independent loads issued together. An axis
+// without a fraction has an upper weight of exactly zero and its upper taps
+// are neither loaded nor used (they contribute zero).
+template <int N>
+__device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
+                                              const uint32_t* sl, uint32_t ns, float2* lds) {
+  constexpr int LD = pitch<N>(), HALF = N / 2, NN = N * N, S = (int)kGatherStrip;
   static_assert(NN % kThreads == 0, "pixels per thread");
+  const Rsrc rl = make_rsrc(sl, ns * 4u);
+  // the first strip's entry, in flight during the zero fill
+  uint32_t en = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, (uint32_t)tid < ns ? tid * 4u : kSkip, 0, 0);
   {
     const int row0 = tid / N, col = tid % N;
 #pragma unroll
@@ -762,42 +767,29 @@ __device__ __forceinline__ void gather_list(Rsrc rspec, Rsrc rpup, const SrcPara
   const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
   const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
                wuu = sp.ar.wu * sp.ac.wu;
-  const Rsrc rl = make_rsrc(sl, ns * 4u);
-  // list entries of the next group are loaded with the current group's taps
-  uint32_t en[G];
+  const double2 zero = make_double2(0.0, 0.0);
+  for (uint32_t i = (uint32_t)tid; i < ns; i += kThreads) {
+    const uint32_t e = i == (uint32_t)tid ? en : (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
+    const uint32_t r0 = e & 0xFFu, c = (e >> 8) & 0xFFu, mask = e >> 16;
+    const uint32_t o = (sp.ar.lower + r0) * rw + sp.ac.lower + c;
+    double2 t0[S + 1], t1[S + 1];
+    float2 P[S];
 #pragma unroll
-  for (int k = 0; k < G; ++k) {
-    const uint32_t i = (uint32_t)tid + k * kThreads;
-    en[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i < ns ? i * 4u : kSkip, 0, 0);
-  }
-  for (uint32_t base = (uint32_t)tid; base < ns; base += G * kThreads) {
-    uint32_t e[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-      e[k] = en[k];
-      const uint32_t i = base + (G + k) * kThreads;
-      en[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i < ns ? i * 4u : kSkip, 0, 0);
-    }
-    float2 P[G];
-    double2 ll[G], lu[G], ul[G], uu[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-      const bool live = base + k * kThreads < ns;
-      const uint32_t r = e[k] & 0xFFFFu, c = e[k] >> 16;
-      const uint32_t o = (sp.ar.lower + r) * rw + sp.ac.lower + c;
-      P[k] = bld_f2(rpup, live ? (r * N + c) * 8u : kSkip);
-      ll[k] = bld_d2(rspec, live ? o * 16u : kSkip);
-      lu[k] = bld_d2(rspec, live && fc ? (o + 1) * 16u : kSkip);
-      ul[k] = bld_d2(rspec, live && fr ? (o + rw) * 16u : kSkip);
-      uu[k] = bld_d2(rspec, live && fr && fc ? (o + rw + 1) * 16u : kSkip);
+    for (int j = 0; j <= S; ++j) {
+      // tap row j serves pixel j (lower row) and, with a row fraction, pixel j - 1
+      const bool need = (j < S && (mask >> j & 1u)) || (fr && j > 0 && (mask >> (j - 1) & 1u));
+      t0[j] = bld_d2(rspec, need ? (o + j * rw) * 16u : kSkip);
+      t1[j] = bld_d2(rspec, need && fc ? (o + j * rw + 1) * 16u : kSkip);
     }
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
-      const double2 v = make_double2(fma(uu[k].x, wuu, fma(ul[k].x, wul, fma(lu[k].x, wlu, ll[k].x * wll))),
-                                     fma(uu[k].y, wuu, fma(ul[k].y, wul, fma(lu[k].y, wlu, ll[k].y * wll))));
-      const uint32_t r = e[k] & 0xFFFFu, c = e[k] >> 16;
-      if (base + k * kThreads < ns)
-        lds[((r + HALF) % N) * LD + (c + HALF) % N] = fmul(f2((float)v.x, (float)v.y), P[k]);
+    for (int j = 0; j < S; ++j) P[j] = bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+    const int cc = (c + HALF) % N;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
+      const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
+                                     fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
+      if (mask >> j & 1u) lds[((r0 + j + HALF) % N) * LD + cc] = fmul(f2((float)v.x, (float)v.y), P[j]);
     }
   }
 }
@@ -935,7 +927,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   // the common case: every pixel counts, no background, amplitude MSE
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   const uint32_t ndest = __builtin_amdgcn_readfirstlane(gptr(T.dest)[0]);
-  const uint32_t nsrc = __builtin_amdgcn_readfirstlane(gptr(T.dest)[1]);
+  const uint32_t nstrip = __builtin_amdgcn_readfirstlane(gptr(T.dest)[1]);
   uint64_t counter = *T.counter;
   // Rows of the transforms that carry data: the centred rows rlo..rhi with
   // pupil support, i.e. the field rows (rlo + N/2 + i) mod N, i < nl (the
@@ -981,7 +973,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       const float gain_f = (float)gain;
       const float inv_gain = (float)(1.0 / gain);
       // 1. patch x pupil over the support pixels, written ifftshifted
-      gather_list<N>(rspec, rpup, sp, T.rw, tid, T.dest + 2 + (N + 1) * (N + 1), nsrc, lds);
+      gather_strips<N>(rspec, rpup, sp, T.rw, tid, T.dest + 2 + (N + 1) * (N + 1), nstrip, lds);
       __syncthreads();
       // the next LED's frame index, then its record words (written to the
       // other slot after the forward transform)
@@ -1090,7 +1082,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 2, ndest, dfirst,
                      [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
       __syncthreads();
-      // 7. mPIE momentum over the whole tile spectrum (common.rs:308-387)
+      // 7. mPIE momentum over the tile spectrum's changeable pixels (common.rs:308-387)
       if (T.momentum) {
         counter += 1;
         if (counter >= T.interval) {

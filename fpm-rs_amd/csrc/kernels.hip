@@ -842,7 +842,7 @@ __global__ void k_pupil_extent(const float2* __restrict__ pupil, uint32_t h, uin
 constexpr uint32_t kDestMaxRows = 129;
 __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, uint32_t* __restrict__ dest) {
   __shared__ uint32_t lo_s[kDestMaxRows], len_s[kDestMaxRows], start_s[kDestMaxRows + 1];
-  __shared__ uint32_t slen_s[kDestMaxRows], sstart_s[kDestMaxRows];
+  __shared__ uint32_t top_s[kDestMaxRows], cnt_s[kDestMaxRows], cstart_s[kDestMaxRows];
   const uint32_t rows = n + 1;
   for (uint32_t a = threadIdx.x; a < rows; a += blockDim.x) {
     uint32_t lo = rows, hi = 0;
@@ -856,10 +856,22 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
     if (hi > lo) hi = hi + 1 < rows ? hi + 1 : rows;
     lo_s[a] = lo;
     len_s[a] = hi > lo ? hi - lo : 0;
-    if (a < n) {
-      const uint32_t e = extent[a], elo = e & 0xFFFFu, ehi = e >> 16;
-      slen_s[a] = ehi > elo ? ehi - elo : 0;
+  }
+  // strips: bands of kGatherStrip rows, each over the columns from its
+  // first to its last support pixel (consecutive list entries are adjacent
+  // columns of one band, so a wave's loads are contiguous)
+  const uint32_t bands = (n + kGatherStrip - 1) / kGatherStrip;
+  for (uint32_t k = threadIdx.x; k < bands; k += blockDim.x) {
+    uint32_t lo = n, hi = 0;
+    for (uint32_t r = k * kGatherStrip; r < (k + 1) * kGatherStrip && r < n; ++r) {
+      const uint32_t e = extent[r], elo = e & 0xFFFFu, ehi = e >> 16;
+      if (ehi > elo) {
+        lo = elo < lo ? elo : lo;
+        hi = ehi > hi ? ehi : hi;
+      }
     }
+    top_s[k] = lo;
+    cnt_s[k] = hi > lo ? hi - lo : 0;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -867,10 +879,10 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
     for (uint32_t a = 0; a < rows; ++a) {
       start_s[a] = acc;
       acc += len_s[a];
-      if (a < n) {
-        sstart_s[a] = sacc;
-        sacc += slen_s[a];
-      }
+    }
+    for (uint32_t k = 0; k < bands; ++k) {
+      cstart_s[k] = sacc;
+      sacc += cnt_s[k];
     }
     start_s[rows] = acc;
     dest[0] = acc;
@@ -879,11 +891,18 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
   __syncthreads();
   uint32_t* dl = dest + 2;
   uint32_t* sl = dest + 2 + rows * rows;
-  for (uint32_t a = 0; a < rows; ++a) {
+  for (uint32_t a = 0; a < rows; ++a)
     for (uint32_t j = threadIdx.x; j < len_s[a]; j += blockDim.x) dl[start_s[a] + j] = a | ((lo_s[a] + j) << 16);
-    if (a < n) {
-      const uint32_t elo = extent[a] & 0xFFFFu;
-      for (uint32_t j = threadIdx.x; j < slen_s[a]; j += blockDim.x) sl[sstart_s[a] + j] = a | ((elo + j) << 16);
+  for (uint32_t k = 0; k < bands; ++k) {
+    const uint32_t r0 = k * kGatherStrip;
+    for (uint32_t j = threadIdx.x; j < cnt_s[k]; j += blockDim.x) {
+      const uint32_t c = top_s[k] + j;
+      uint32_t mask = 0;
+      for (uint32_t i = 0; i < kGatherStrip && r0 + i < n; ++i) {
+        const uint32_t e = extent[r0 + i];
+        if (c >= (e & 0xFFFFu) && c < (e >> 16)) mask |= 1u << i;
+      }
+      sl[cstart_s[k] + j] = r0 | (c << 8) | (mask << 16);
     }
   }
 }

@@ -36,6 +36,9 @@ struct Config {
 };
 Config make_config(const fpmhip_algorithm& a);
 
+// Rows per strip of the fused sweep's gather list.
+constexpr uint32_t kGatherStrip = 5;
+
 struct SessionBuffers {
   DeviceBuffer spec;      // double2[M] centred object spectrum (FP64 accumulation)
   DeviceBuffer vel, anc;  // V[M] mPIE velocity/anchor (momentum arithmetic is FP64 either way)
@@ -53,10 +56,12 @@ struct SessionBuffers {
   DeviceBuffer avg, aw;   // double[N] (init)
   DeviceBuffer group;     // cooperative-group state of the fused sweep (large MIXED_F32 tiles)
   DeviceBuffer extent;    // uint32[n_h] per-row support of the complex64 pupil (MIXED_F32)
-  DeviceBuffer dest;      // uint32[2 + (n+1)^2 + n^2] live-pixel lists of the fused sweep (MIXED_F32
-                          // tiles up to the LDS-resident size): [0] destinations of the adjoint
-                          // insert, [1] support pixels; then the destinations a | b << 16 and the
-                          // support pixels r | c << 16, row-major
+  DeviceBuffer dest;      // uint32[2 + (n+1)^2 + (n / kGatherStrip + 1) n] live-pixel lists of the fused
+                          // sweep (MIXED_F32 tiles up to the LDS-resident size): [0] destinations of
+                          // the adjoint insert, [1] support strips; then the destinations a | b << 16
+                          // and the strips r | c << 8 | mask << 16 (kGatherStrip rows r.. of column c,
+                          // bit j: pixel r + j is in the support), band of rows by band, columns in
+                          // order within a band
   DeviceBuffer mspan;     // uint32 momentum chunks of the fused sweep (MIXED_F32 mPIE): [0] count, then
                           // start | (length - 1) << 25, up to 64 object pixels of one row each, covering
                           // every pixel an LED update can change
