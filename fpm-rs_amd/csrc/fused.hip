@@ -801,7 +801,7 @@ __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcPa
 // written in FP64 when that sum is non-zero. The list holds every window
 // pixel that a tap of the support can reach, so no live destination is
 // missed and no thread iterates over dead ones.
-constexpr int kInsertGroup = 4;  // destinations per thread in flight
+constexpr int kInsertGroup = 5;  // destinations per thread in flight
 
 // The first group of a thread's destination entries (issued ahead of the
 // insert, so that its read-modify-writes need not wait for them).
