@@ -744,7 +744,7 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
 // strips `sl` (count ns; kGatherStrip vertically consecutive pixels of one
 // column, centred patch coordinates, with a mask of the pixels in the
-// support), written to the N x N LDS field at the ifftshifted position; the
+// support; `en` is entry tid, loaded by the caller), written to the N x N LDS field at the ifftshifted position; the
 // rest of the field is zero (written first, then a barrier). Consecutive
 // pixels of a strip share a row of taps: S + 1 rows of two spectrum taps and
 // S pupil values per strip, all independent loads issued together. An axis
@@ -752,12 +752,10 @@ __device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32
 // are neither loaded nor used (they contribute zero).
 template <int N>
 __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
-                                              const uint32_t* sl, uint32_t ns, float2* lds) {
+                                              const uint32_t* sl, uint32_t ns, uint32_t en, float2* lds) {
   constexpr int LD = pitch<N>(), HALF = N / 2, NN = N * N, S = (int)kGatherStrip;
   static_assert(NN % kThreads == 0, "pixels per thread");
   const Rsrc rl = make_rsrc(sl, ns * 4u);
-  // the first strip's entry, in flight during the zero fill
-  uint32_t en = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, (uint32_t)tid < ns ? tid * 4u : kSkip, 0, 0);
   {
     const int row0 = tid / N, col = tid % N;
 #pragma unroll
@@ -903,20 +901,18 @@ __device__ __forceinline__ SrcParam rec_src(const uint32_t* r) {
   return sp;
 }
 
-// One sweep of one tile. Thread t owns a strip of centred pixels (rows
-// 8*(t/N) .. +7 of column t%N for N = 128) and their swap partners
-// ((r + N/2) mod N, (c + N/2) mod N): for even N, fftshift and ifftshift are
-// both that swap, so the shifted LDS reads and writes of phases 1 and 5 stay
-// inside the owning thread, and vertically adjacent pixels share bilinear taps.
+// One sweep of one tile. The support pixels are handled in strips of
+// kGatherStrip vertically consecutive pixels (the tile's strip list), one
+// strip per thread: the gather writes each pixel's patch x pupil at its
+// ifftshifted (natural) slot and the update replaces the transform there by
+// U; for even N, fftshift and ifftshift are both the swap
+// ((r + N/2) mod N, (c + N/2) mod N).
 template <int N, int RADIX>
 __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi,
                                            double* red, const uint32_t* ext, uint32_t* rec) {
   constexpr int NN = N * N;
   constexpr int HALF = N / 2;
   constexpr int LD = pitch<N>();
-  constexpr int NPAIR = NN / 2;
-  constexpr int STRIP = NPAIR / N >= (kThreads / N) ? (NPAIR / N) / (kThreads / N) : 1;  // rows per thread
-  constexpr int COLS_THREADS = N * ((HALF + STRIP - 1) / STRIP);  // threads with work
   const float inv_n = 1.0f / (float)NN;
   double2* __restrict__ spec = T.spec;
   const float2* __restrict__ pupil = T.pupil;
@@ -928,6 +924,9 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   const uint32_t ndest = __builtin_amdgcn_readfirstlane(gptr(T.dest)[0]);
   const uint32_t nstrip = __builtin_amdgcn_readfirstlane(gptr(T.dest)[1]);
+  const uint32_t* slist = T.dest + 2 + (N + 1) * (N + 1);
+  // this thread's support strip (the same for every LED): gather and update
+  const uint32_t my_strip = threadIdx.x < nstrip ? gptr(slist)[threadIdx.x] : 0u;
   uint64_t counter = *T.counter;
   // Rows of the transforms that carry data: the centred rows rlo..rhi with
   // pupil support, i.e. the field rows (rlo + N/2 + i) mod N, i < nl (the
@@ -973,7 +972,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       const float gain_f = (float)gain;
       const float inv_gain = (float)(1.0 / gain);
       // 1. patch x pupil over the support pixels, written ifftshifted
-      gather_strips<N>(rspec, rpup, sp, T.rw, tid, T.dest + 2 + (N + 1) * (N + 1), nstrip, lds);
+      gather_strips<N>(rspec, rpup, sp, T.rw, tid, slist, nstrip, my_strip, lds);
       __syncthreads();
       // the next LED's frame index, then its record words (written to the
       // other slot after the forward transform)
@@ -1053,34 +1052,42 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       // 4. forward FFT (x 1/N applied in phase 5)
       fft2_lds<N, false, RADIX>(lds, twf, tid, live);
       if (fetch) rn[threadIdx.x] = next_w;
-      // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den on the strip and its partner
+      // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den at the support pixels,
+      //    in place: U of centred pixel (r, c) replaces the transform value at
+      //    its natural slot ((r + N/2) mod N, (c + N/2) mod N). Outside the
+      //    support the field keeps values that the insert does not read.
       uint32_t dfirst[kInsertGroup];
       insert_list_first(T.dest + 2, ndest, tid, dfirst);
-      if (tid < COLS_THREADS) {
-        const int c = tid % N, r0 = (tid / N) * STRIP;
-        const int c1 = (c + HALF) % N;
+      {
+        constexpr int S = (int)kGatherStrip;
+        const Rsrc rl = make_rsrc(slist, nstrip * 4u);
+        for (uint32_t i = (uint32_t)tid; i < nstrip; i += kThreads) {
+          const uint32_t e = i == (uint32_t)tid ? my_strip : (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
+          const int r0 = (int)(e & 0xFFu), c = (int)((e >> 8) & 0xFFu);
+          const uint32_t mask = e >> 16;
+          const int cn = (c + HALF) % N;
+          float2 P[S];
 #pragma unroll
-        for (int j = 0; j < STRIP; ++j) {
-          const int r = r0 + j, r1 = r + HALF;
-          const int l0 = r * LD + c, l1 = r1 * LD + c1;
-          const float2 s0 = lds[l1], s1 = lds[l0];
-          // U is zero outside the support extents: no pupil read there
-          const bool in0 = in_extent(ext[r], c), in1 = in_extent(ext[r1], c1);
-          const float2 P0 = bld_f2(rpup, in0 ? (r * N + c) * 8u : kSkip);
-          const float2 P1 = bld_f2(rpup, in1 ? (r1 * N + c1) * 8u : kSkip);
-          const float k0 = inv_n / fmaf(den.b, fmaf(P0.x, P0.x, P0.y * P0.y), den.a);
-          const float k1 = inv_n / fmaf(den.b, fmaf(P1.x, P1.x, P1.y * P1.y), den.a);
-          const float2 u0 = fmulc(P0, s0), u1 = fmulc(P1, s1);
-          lds[l0] = in0 ? f2(u0.x * k0, u0.y * k0) : f2(0.0f, 0.0f);
-          lds[l1] = in1 ? f2(u1.x * k1, u1.y * k1) : f2(0.0f, 0.0f);
+          for (int j = 0; j < S; ++j) P[j] = bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            if (mask >> j & 1u) {
+              const int l = ((r0 + j + HALF) % N) * LD + cn;
+              const float k = inv_n / fmaf(den.b, fmaf(P[j].x, P[j].x, P[j].y * P[j].y), den.a);
+              const float2 u = fmulc(P[j], lds[l]);
+              lds[l] = f2(u.x * k, u.y * k);
+            }
+          }
         }
       }
       __syncthreads();
       // 6. adjoint insert over the tile's live destinations: FP64 RMW of the
-      //    pixels that receive a non-zero update (U is zero outside the pupil)
+      //    pixels that receive a non-zero update (U is zero outside the support
+      //    extents, where it is not read)
       const double scale = fw * T.object_step;
-      insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 2, ndest, dfirst,
-                     [&](int sr_, int sc_) { return lds[sr_ * LD + sc_]; });
+      insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 2, ndest, dfirst, [&](int sr_, int sc_) {
+        return in_extent(ext[sr_], sc_) ? lds[((sr_ + HALF) % N) * LD + (sc_ + HALF) % N] : f2(0.0f, 0.0f);
+      });
       __syncthreads();
       // 7. mPIE momentum over the tile spectrum's changeable pixels (common.rs:308-387)
       if (T.momentum) {
