@@ -901,6 +901,23 @@ __device__ __forceinline__ SrcParam rec_src(const uint32_t* r) {
   return sp;
 }
 
+// Amplitude replacement and loss term of one pixel of an unmasked,
+// background-free amplitude-MSE frame (common.rs:123-228), as the field
+// change psi' - psi, with two transcendental operations per pixel:
+// sy = sqrt(max(y, 0)) and rp = 1/sqrt(pred), so that sqrt(g pred) =
+// sqrt(g) pred rp and sqrt(max(y/g, 0) / pred) = sy sqrt(1/g) rp
+// (sg = sqrt(g), sig = sqrt(1/g)).
+__device__ __forceinline__ float2 replace_plain(float2 fv, float y, float sg, float sig, float eps_f, float& lsum) {
+  const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
+  const float sy = fsqrt(fmaxf(y, 0.0f));
+  const float rp = __builtin_amdgcn_rsqf(fmaxf(pred, 1e-30f));
+  const float r = sg * (pred * rp) - sy;  // point_loss with b = 0
+  lsum += r * r;
+  const float a = sy * sig;
+  const float qm1 = a * rp - 1.0f;
+  return pred > eps_f ? f2(fv.x * qm1, fv.y * qm1) : f2(a - fv.x, -fv.y);
+}
+
 // One sweep of one tile. The support pixels are handled in strips of
 // kGatherStrip vertically consecutive pixels (the tile's strip list), one
 // strip per thread: the gather writes each pixel's patch x pupil at its
@@ -971,6 +988,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       const double gain = rec_double(r, kRecSrc + 2);
       const float gain_f = (float)gain;
       const float inv_gain = (float)(1.0 / gain);
+      const float sg = (float)sqrt(gain), sig = (float)sqrt(1.0 / gain);
       // 1. patch x pupil over the support pixels, written ifftshifted
       gather_strips<N>(rspec, rpup, sp, T.rw, tid, slist, nstrip, my_strip, lds);
       __syncthreads();
@@ -1009,13 +1027,7 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         for (int j = 0; j < PP; ++j) {
           const int lp = (row0 + j * RSTEP) * LD + col;
           const float2 fv = lds[lp];
-          const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
-          const float r = fsqrt(fmaxf(gain_f * pred, 0.0f)) - fsqrt(fmaxf(y[j], 0.0f));  // point_loss, b = 0
-          lsum_f += r * r;
-          const float target = fmaxf(y[j] * inv_gain, 0.0f);
-          const bool big = pred > eps_f;
-          const float q = fsqrt(big ? target * frcp(pred) : target);
-          lds[lp] = big ? f2(fv.x * (q - 1.0f), fv.y * (q - 1.0f)) : f2(q - fv.x, -fv.y);
+          lds[lp] = replace_plain(fv, y[j], sg, sig, eps_f, lsum_f);
         }
         lcnt_i = PP;
       } else {
@@ -1283,6 +1295,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       const double gain = rec_double(r, kRecSrc + 2);
       const float gain_f = (float)gain;
       const float inv_gain = (float)(1.0 / gain);
+      const float sg = (float)sqrt(gain), sig = (float)sqrt(1.0 / gain);
       // the next LED's frame index (during A), then its record words (during
       // B), written to the other slot after B
       uint32_t next_f = 0, next_w = 0;
@@ -1345,13 +1358,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           for (int j = 0; j < PP; ++j) {
             const int lp = (row0 + j * RSTEP) * LDC + col;
             const float2 fv = lds[lp];
-            const float pred = fmaf(fv.x, fv.x, fv.y * fv.y);
-            const float rr = fsqrt(fmaxf(gain_f * pred, 0.0f)) - fsqrt(fmaxf(y[j], 0.0f));
-            lsum_f += rr * rr;
-            const float target = fmaxf(y[j] * inv_gain, 0.0f);
-            const bool big = pred > eps_f;
-            const float qv = fsqrt(big ? target * frcp(pred) : target);
-            lds[lp] = big ? f2(fv.x * (qv - 1.0f), fv.y * (qv - 1.0f)) : f2(qv - fv.x, -fv.y);
+            lds[lp] = replace_plain(fv, y[j], sg, sig, eps_f, lsum_f);
           }
           lsum += (double)lsum_f;
           lcnt += PP;
