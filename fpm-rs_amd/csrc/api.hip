@@ -820,8 +820,7 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       b.mspan.allocate(momentum_span_capacity(*m) * sizeof(uint32_t));
     if (precision == FPMHIP_MIXED_F32) {
       b.extent.allocate(m->nh * sizeof(uint32_t));
-      if (m->nh == m->nw && m->nh <= kFusedLdsMaxN)
-        b.dest.allocate((2 + (m->nh + 1) * (m->nw + 1) + (m->nh / kGatherStrip + 1) * m->nw) * sizeof(uint32_t));
+      if (m->nh == m->nw && m->nh <= kFusedMaxN) b.dest.allocate(dest_table_words(m->nh) * sizeof(uint32_t));
       launch_pupil_extent(*s);
       s->extent_from_model = true;
     }

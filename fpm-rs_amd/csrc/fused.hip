@@ -660,87 +660,6 @@ __device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, con
   }
 }
 
-// Adjoint insert of one patch update U (crop.rs:188-256) as a gather over
-// destination rows [a0, a1) of the (n+1)^2 window: each destination pixel
-// sums its <= 4 weighted taps of U (fetched by u(row, col) in centred patch
-// coordinates) and is read-modified-written in FP64 only when that sum is
-// non-zero. U is zero outside the pupil's support extents `ext`, so pixels
-// beyond them are skipped without reading U.
-template <int N, int THREADS, class MEM = PlainMem, int UNROLL = 4, class GetU>
-__device__ __forceinline__ void insert_window(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
-                                              uint32_t rw, double scale, int tid, int a0, int a1, const uint32_t* ext,
-                                              GetU u) {
-  const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
-  // destination columns that can receive a non-zero update: the union of the
-  // support extents of tap rows a - 1 and a, widened by one for a column tap
-  auto span = [&](int a, int& lo, int& hi) {
-    lo = N;
-    hi = 0;
-    if (a < N) {
-      lo = (int)(ext[a] & 0xFFFFu);
-      hi = (int)(ext[a] >> 16);
-    }
-    if (fr && a >= 1) {
-      lo = min(lo, (int)(ext[a - 1] & 0xFFFFu));
-      hi = max(hi, (int)(ext[a - 1] >> 16));
-    }
-    if (fc) hi += 1;
-  };
-  const int ww = N + (fc ? 1 : 0);
-  const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
-              wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
-  const int total = (a1 - a0) * ww;
-  for (int base = tid; base < total; base += UNROLL * THREADS) {
-    float2 add[UNROLL];
-    uint32_t off[UNROLL];
-    bool hit[UNROLL];
-#pragma unroll
-    for (int k = 0; k < UNROLL; ++k) {
-      const int qd = base + k * THREADS;
-      const int a = a0 + qd / ww, b = qd % ww;
-      off[k] = (sp.ar.lower + a) * rw + sp.ac.lower + b;
-      float ax = 0.0f, ay = 0.0f;
-      auto tap = [&](int sr_, int sc_, float wt) {
-        if (sr_ >= 0 && sr_ < N && sc_ >= 0 && sc_ < N) {
-          const float2 v = u(sr_, sc_);
-          ax = fmaf(v.x, wt, ax);
-          ay = fmaf(v.y, wt, ay);
-        }
-      };
-      int lo, hi;
-      span(a, lo, hi);
-      if (qd < total && b >= lo && b < hi) {
-        if (fr && fc) {
-          tap(a - 1, b - 1, wuu);
-          tap(a - 1, b, wul);
-          tap(a, b - 1, wlu);
-          tap(a, b, wll);
-        } else if (fr) {
-          tap(a - 1, b, wul);
-          tap(a, b, wll);
-        } else if (fc) {
-          tap(a, b - 1, wlu);
-          tap(a, b, wll);
-        } else {
-          tap(a, b, wll);
-        }
-      }
-      add[k] = f2(ax, ay);
-      hit[k] = qd < total && (ax != 0.0f || ay != 0.0f);
-    }
-    // branch-free: every read of the group is in flight at once
-    constexpr int AUX = kAux<MEM>;
-    const Rsrc rs = make_rsrc(spec, spec_elems * 16u);
-    double2 old[UNROLL];
-#pragma unroll
-    for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2<AUX>(rs, hit[k] ? off[k] * 16u : kSkip);
-#pragma unroll
-    for (int k = 0; k < UNROLL; ++k)
-      bst_d2<AUX>(rs, hit[k] ? off[k] * 16u : kSkip,
-                  make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
-  }
-}
-
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
 // strips `sl` (count ns; kGatherStrip vertically consecutive pixels of one
 // column, centred patch coordinates, with a mask of the pixels in the
@@ -813,10 +732,11 @@ __device__ __forceinline__ void insert_list_first(const uint32_t* dl, uint32_t n
   }
 }
 
-template <int N, class GetU>
+template <int N, class MEM = PlainMem, class GetU>
 __device__ __forceinline__ void insert_list(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
                                             uint32_t rw, double scale, int tid, const uint32_t* dl, uint32_t nd,
                                             uint32_t (&en)[kInsertGroup], GetU u) {
+  constexpr int AUX = kAux<MEM>;
   const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
   const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
               wul = (float)(sp.ar.wu * sp.ac.wl), wuu = (float)(sp.ar.wu * sp.ac.wu);
@@ -864,10 +784,11 @@ __device__ __forceinline__ void insert_list(double2* __restrict__ spec, uint32_t
     }
     double2 old[UNROLL];
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2(rs, off[k]);
+    for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2<AUX>(rs, off[k]);
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k)
-      bst_d2(rs, off[k], make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
+      bst_d2<AUX>(rs, off[k],
+                  make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
   }
 }
 
@@ -1454,12 +1375,20 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
             }
           }
         } else {
+          // the chunk's destination rows a0 .. a1 - 1 of the tile's list
           const int a1 = a0 + CH + ((q == NCH - 1 && sp.ar.upper != sp.ar.lower) ? 1 : 0);
           const float2* above = COOP ? T.edge + (q - 1) * N : prev;
-          insert_window<N, kThreads, MEM>(spec, T.rh * T.rw, sp, T.rw, scale, opaque_tid(), a0, a1, ext, [&](int sr_, int sc_) {
-            return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)]
-                             : (COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF]);
-          });
+          const uint32_t* rows = T.dest + dest_row_starts_offset(N);
+          const uint32_t d0 = __builtin_amdgcn_readfirstlane(gptr(rows)[a0]);
+          const uint32_t d1 = __builtin_amdgcn_readfirstlane(gptr(rows)[a1]);
+          uint32_t dfirst[kInsertGroup];
+          const int itid = opaque_tid();
+          insert_list_first(T.dest + 2 + d0, d1 - d0, itid, dfirst);
+          insert_list<N, MEM>(spec, T.rh * T.rw, sp, T.rw, scale, itid, T.dest + 2 + d0, d1 - d0, dfirst,
+                              [&](int sr_, int sc_) {
+                                return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)]
+                                                 : (COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF]);
+                              });
         }
         __syncthreads();
         if (!COOP && q + 1 < NCH) {

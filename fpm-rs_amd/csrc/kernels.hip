@@ -839,7 +839,7 @@ __global__ void k_pupil_extent(const float2* __restrict__ pupil, uint32_t h, uin
 // of its extent (the gather reads nothing elsewhere). Layout: dest[0] and
 // dest[1] the two counts, then the destinations a | b << 16 and the support
 // pixels r | c << 16, both row-major.
-constexpr uint32_t kDestMaxRows = 129;
+constexpr uint32_t kDestMaxRows = 257;
 __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, uint32_t* __restrict__ dest) {
   __shared__ uint32_t lo_s[kDestMaxRows], len_s[kDestMaxRows], start_s[kDestMaxRows + 1];
   __shared__ uint32_t top_s[kDestMaxRows], cnt_s[kDestMaxRows], cstart_s[kDestMaxRows];
@@ -890,7 +890,9 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
   }
   __syncthreads();
   uint32_t* dl = dest + 2;
-  uint32_t* sl = dest + 2 + rows * rows;
+  uint32_t* sl = dest + dest_strips_offset(n);
+  uint32_t* rs = dest + dest_row_starts_offset(n);
+  for (uint32_t a = threadIdx.x; a <= rows; a += blockDim.x) rs[a] = start_s[a];
   for (uint32_t a = 0; a < rows; ++a)
     for (uint32_t j = threadIdx.x; j < len_s[a]; j += blockDim.x) dl[start_s[a] + j] = a | ((lo_s[a] + j) << 16);
   for (uint32_t k = 0; k < bands; ++k) {

@@ -38,6 +38,12 @@ Config make_config(const fpmhip_algorithm& a);
 
 // Rows per strip of the fused sweep's gather list.
 constexpr uint32_t kGatherStrip = 5;
+// Layout of SessionBuffers::dest for an n x n tile, in uint32 words: header
+// (2), insert destinations ((n+1)^2 slots), support strips ((n / kGatherStrip
+// + 1) n slots), then the first destination index of each window row (n + 2).
+__host__ __device__ inline uint64_t dest_strips_offset(uint32_t n) { return 2 + (uint64_t)(n + 1) * (n + 1); }
+__host__ __device__ inline uint64_t dest_row_starts_offset(uint32_t n) { return dest_strips_offset(n) + (uint64_t)(n / kGatherStrip + 1) * n; }
+__host__ __device__ inline uint64_t dest_table_words(uint32_t n) { return dest_row_starts_offset(n) + n + 2; }
 
 struct SessionBuffers {
   DeviceBuffer spec;      // double2[M] centred object spectrum (FP64 accumulation)
@@ -56,12 +62,12 @@ struct SessionBuffers {
   DeviceBuffer avg, aw;   // double[N] (init)
   DeviceBuffer group;     // cooperative-group state of the fused sweep (large MIXED_F32 tiles)
   DeviceBuffer extent;    // uint32[n_h] per-row support of the complex64 pupil (MIXED_F32)
-  DeviceBuffer dest;      // uint32[2 + (n+1)^2 + (n / kGatherStrip + 1) n] live-pixel lists of the fused
-                          // sweep (MIXED_F32 tiles up to the LDS-resident size): [0] destinations of
-                          // the adjoint insert, [1] support strips; then the destinations a | b << 16
-                          // and the strips r | c << 8 | mask << 16 (kGatherStrip rows r.. of column c,
-                          // bit j: pixel r + j is in the support), band of rows by band, columns in
-                          // order within a band
+  DeviceBuffer dest;      // uint32[dest_table_words(n)] live-pixel lists of the fused sweep (square
+                          // MIXED_F32 tiles): [0] destinations of the adjoint insert, [1] support
+                          // strips; then the destinations a | b << 16 (row-major), the strips
+                          // r | c << 8 | mask << 16 (kGatherStrip rows r.. of column c, bit j: pixel
+                          // r + j is in the support; band of rows by band, columns in order within a
+                          // band) and the index of each window row's first destination
   DeviceBuffer mspan;     // uint32 momentum chunks of the fused sweep (MIXED_F32 mPIE): [0] count, then
                           // start | (length - 1) << 25, up to 64 object pixels of one row each, covering
                           // every pixel an LED update can change
