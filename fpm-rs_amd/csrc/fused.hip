@@ -586,8 +586,10 @@ __device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, c
 // extents the arithmetic is gather_strip's.
 template <int STRIP, int AUX = 0, class Put>
 __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
-                                               int c, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du, double wll,
-                                               double wlu, double wul, double wuu, Put put) {
+                                               int rows, int c, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du,
+                                               double wll, double wlu, double wul, double wuu, Put put) {
+  // rows j >= rows of the strip are treated as empty (their extents are not read)
+  auto ex = [&](int j) { return j < rows ? ext[j] : 0u; };
   // sub-blocks of SUB rows: SUB pupil values and SUB (+1) tap rows in flight;
   // tap row t is the lower row of pixel t and the upper row of pixel t - 1,
   // so each sub-block carries its last tap row into the next.
@@ -600,7 +602,7 @@ __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t p
     double2 lo[SUB + 1], hi[SUB + 1];
 #pragma unroll
     for (int k = 0; k < SUB; ++k)
-      P[k] = bld_f2(rpup, in_extent(ext[j0 + k], c) ? (pofs + (j0 + k) * pstride) * 8u : kSkip);
+      P[k] = bld_f2(rpup, in_extent(ex(j0 + k), c) ? (pofs + (j0 + k) * pstride) * 8u : kSkip);
 #pragma unroll
     for (int k = 0; k <= SUB; ++k) {
       const int t = j0 + k;
@@ -609,7 +611,7 @@ __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t p
         hi[0] = chi;
         continue;
       }
-      const bool need = (t < STRIP && in_extent(ext[t], c)) || (du != 0 && t > 0 && in_extent(ext[t - 1], c));
+      const bool need = (t < STRIP && in_extent(ex(t), c)) || (du != 0 && t > 0 && in_extent(ex(t - 1), c));
       const uint32_t o = off + t * rw;
       const uint32_t ol = need ? o * 16u : kSkip;
       lo[k] = bld_d2<AUX>(rspec, ol);
@@ -632,10 +634,10 @@ __device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t p
 // copy_strip through buffer loads, without branches (see gather_strip_b).
 template <int STRIP, int AUX = 0, class Put>
 __device__ __forceinline__ void copy_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
-                                             int c, uint32_t off, uint32_t rw, Put put) {
+                                             int rows, int c, uint32_t off, uint32_t rw, Put put) {
 #pragma unroll
   for (int j = 0; j < STRIP; ++j) {
-    const bool in = in_extent(ext[j], c);
+    const bool in = j < rows && in_extent(ext[j], c);
     const float2 P = bld_f2(rpup, in ? (pofs + j * pstride) * 8u : kSkip);
     const double2 v = bld_d2<AUX>(rspec, in ? (off + j * rw) * 16u : kSkip);
     put(j, fmul(f2((float)v.x, (float)v.y), P));
@@ -1072,6 +1074,61 @@ __device__ __forceinline__ void copy_out(const float2* lds, float2* __restrict__
   }
 }
 
+// Rows 0..nk-1 of an LDS row chunk (pitch LD, rows of N) to / from rows
+// row_of(j) of a row-major global array with N columns: every load of the
+// thread is issued before its first store.
+template <int CH, int N, int LD, class MEM, class RowOf>
+__device__ __forceinline__ void rows_out(const float2* lds, float2* __restrict__ g, int nk, int tid, RowOf row_of) {
+  constexpr int PER = CH * N / kThreads;
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = i < nk * N ? lds[(i / N) * LD + i % N] : f2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    if (i < nk * N) MEM::st(g + row_of(i / N) * N + i % N, v[j]);
+  }
+}
+
+template <int CH, int N, int LD, class MEM, class RowOf>
+__device__ __forceinline__ void rows_in(float2* lds, const float2* __restrict__ g, int nk, int tid, RowOf row_of) {
+  constexpr int PER = CH * N / kThreads;
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = i < nk * N ? MEM::ld(g + row_of(i / N) * N + i % N) : f2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    if (i < nk * N) lds[(i / N) * LD + i % N] = v[j];
+  }
+}
+
+// copy_in of a column block whose rows with live(row) false read as zero
+// (without loading them).
+template <int TOTAL, int W, int LD, class MEM, class Live>
+__device__ __forceinline__ void copy_in_live(float2* lds, const float2* __restrict__ g, int gstride, int tid,
+                                             Live live) {
+  constexpr int PER = TOTAL / kThreads;
+  static_assert(TOTAL % kThreads == 0, "copy shape");
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = live(i / W) ? MEM::ld(g + (i / W) * gstride + i % W) : f2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    lds[(i / W) * LD + i % W] = v[j];
+  }
+}
+
 // Group barrier of a cooperative tile (members > 1). Each wave first waits
 // for its own device-coherent stores to complete (s_waitcnt vmcnt(0); a
 // completed device-scope store is visible to device-scope loads on every
@@ -1174,24 +1231,26 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
   uint64_t counter = *T.counter;
-  // Rows of the transforms that carry data: the centred rows rlo..rhi with
-  // pupil support, i.e. the field rows (rlo + N/2 + i) mod N, i < nl (the
-  // patch x pupil is zero elsewhere and U is read only there).
-  LiveRows live;
-  {
-    int rlo = N, rhi = -1;
-    for (int r = 0; r < N; ++r) {
-      const uint32_t e = ext[r];
-      if ((e & 0xFFFFu) < (e >> 16)) {
-        rlo = rlo < r ? rlo : r;
-        rhi = r;
-      }
+  // The centred rows rlo..rhi carry pupil support: the patch x pupil is zero
+  // on the other rows and U is read only on these. Passes A and C split
+  // them evenly over the chunks (chunk k: centred rows rlo + k crows, at most
+  // crows <= CH of them), so no member spends its pass on rows without data;
+  // pass B keeps whole column blocks and reads the other rows as zero.
+  int rlo = N, rhi = -1;
+  for (int r = 0; r < N; ++r) {
+    const uint32_t e = ext[r];
+    if ((e & 0xFFFFu) < (e >> 16)) {
+      rlo = rlo < r ? rlo : r;
+      rhi = r;
     }
-    const int nl = rhi >= rlo ? rhi - rlo + 1 : 0;
-    live.nl = __builtin_amdgcn_readfirstlane(nl);
-    live.l0 = __builtin_amdgcn_readfirstlane((rlo + HALF) & (N - 1));
-    live.rcp = live.nl ? 1.0f / (float)live.nl : 0.0f;
   }
+  rlo = __builtin_amdgcn_readfirstlane(rlo);
+  rhi = __builtin_amdgcn_readfirstlane(rhi);
+  const int nl = rhi >= rlo ? rhi - rlo + 1 : 0;
+  const int crows = (nl + NCH - 1) / NCH;  // <= CH
+  const int clast = nl ? (nl - 1) / crows : -1;  // last chunk with rows
+  auto chunk_rows = [&](int k) { return k <= clast ? min(crows, nl - k * crows) : 0; };
+  auto live_rows = [&](int nk) { return LiveRows{nk, 0, nk ? 1.0f / (float)nk : 0.0f}; };
   if (threadIdx.x < kRecWords) rec[threadIdx.x] = rec_word(T, frame_at(T, 0), threadIdx.x);
   __syncthreads();
   uint32_t slot = 0;
@@ -1221,28 +1280,34 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       // B), written to the other slot after B
       uint32_t next_f = 0, next_w = 0;
       if (fetch) next_f = frame_at(T, npos);
-      // A. shifted row r' holds centred row r' ^ HALF (ifftshift for even N)
+      // A. chunk k: centred rows R0 .. R0 + nk - 1 in LDS rows 0 .. nk - 1 (with
+      //    the ifftshifted columns), inverse row transforms, out to the natural
+      //    rows (R0 + j) ^ HALF of T.field (ifftshift for even N)
       for (int k = first; k < NCH; k += step) {
         tid = opaque_tid();
+        const int nk = chunk_rows(k), R0 = rlo + k * crows;
         {
           const int c = tid % N, s0 = (tid / N) * STRIP;
-          const int rc = ((k * CH) ^ HALF) + s0;  // first centred row of the strip
+          const int rc = R0 + s0;  // first centred row of the strip
           const int cs = c ^ HALF;
           auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
-          if (sp.zero) {
-            copy_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, c,
-                                     ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw, put);
-          } else {
-            gather_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, c,
-                                       (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw, sp.ac.upper - sp.ac.lower,
-                                       (sp.ar.upper - sp.ar.lower) * T.rw, sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu,
-                                       sp.ar.wu * sp.ac.wl, sp.ar.wu * sp.ac.wu, put);
+          if (s0 < nk) {
+            if (sp.zero) {
+              copy_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, nk - s0, c,
+                                       ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw, put);
+            } else {
+              gather_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, nk - s0, c,
+                                         (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
+                                         sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw,
+                                         sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl,
+                                         sp.ar.wu * sp.ac.wu, put);
+            }
           }
         }
         __syncthreads();
-        stage<N, CH, P0, 1, LDR, 1, true>(lds, twi, tid);
-        stage<N, CH, P1, P0, LDR, 1, true>(lds, twi, tid);
-        copy_out<CH * N, N, LDR, MEM>(lds, fld + k * CH * N, N, opaque_tid());
+        stage<N, CH, P0, 1, LDR, 1, true, true>(lds, twi, tid, live_rows(nk));
+        stage<N, CH, P1, P0, LDR, 1, true, true>(lds, twi, tid, live_rows(nk));
+        rows_out<CH, N, LDR, MEM>(lds, fld, nk, opaque_tid(), [&](int j) { return (R0 + j) ^ HALF; });
         __syncthreads();
       }
       if (COOP && !group_barrier(T, passed, bflag)) return;
@@ -1254,7 +1319,10 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       for (int jc = first; jc < NCH; jc += step) {
         tid = opaque_tid();
         const int c0 = jc * CH;
-        copy_in<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid());
+        copy_in_live<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid(), [&](int nr) {
+          const int cr = nr ^ HALF;  // centred row of natural row nr
+          return cr >= rlo && cr <= rhi;
+        });
         __syncthreads();
         stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
         stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
@@ -1338,34 +1406,39 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         }
         __syncthreads();
       }
-      // C. rows in centred order: centred row q*CH + lr is natural row (q*CH + lr) ^ HALF
+      // C. chunk q: centred rows R0 .. R0 + nk - 1 (natural rows (R0 + j) ^ HALF),
+      //    forward row transforms, U, then the insert of destination rows
+      //    R0 .. R0 + nk - 1 (the last chunk with rows also takes the row below
+      //    its last, for a row fraction); the row above comes from the
+      //    previous chunk (zero above the first)
       const double scale = fw * T.object_step;
       for (int q = first; q < NCH; q += step) {
         tid = opaque_tid();
-        const int nr0 = (q * CH) ^ HALF;
-        copy_in<CH * N, N, LDR, MEM>(lds, fld + nr0 * N, N, opaque_tid());
+        const int nk = chunk_rows(q), R0 = rlo + q * crows;
+        rows_in<CH, N, LDR, MEM>(lds, fld, nk, opaque_tid(), [&](int j) { return (R0 + j) ^ HALF; });
         __syncthreads();
-        stage<N, CH, P0, 1, LDR, 1, false>(lds, twf, tid);
-        stage<N, CH, P1, P0, LDR, 1, false>(lds, twf, tid);
-        for (int i = tid; i < CH * N; i += kThreads) {
+        stage<N, CH, P0, 1, LDR, 1, false, true>(lds, twf, tid, live_rows(nk));
+        stage<N, CH, P1, P0, LDR, 1, false, true>(lds, twf, tid, live_rows(nk));
+        for (int i = tid; i < nk * N; i += kThreads) {
           const int lr = i / N, kc = i % N;
-          const bool in = in_extent(ext[q * CH + lr], kc ^ HALF);  // U is zero outside the support
-          const float2 P = bld_f2(rpup, in ? (uint32_t)((q * CH + lr) * N + (kc ^ HALF)) * 8u : kSkip);
+          const bool in = in_extent(ext[R0 + lr], kc ^ HALF);  // U is zero outside the support
+          const float2 P = bld_f2(rpup, in ? (uint32_t)((R0 + lr) * N + (kc ^ HALF)) * 8u : kSkip);
           const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
           const float2 u = fmulc(P, lds[lr * LDR + kc]);
           lds[lr * LDR + kc] = in ? f2(u.x * kk, u.y * kk) : f2(0.0f, 0.0f);
         }
         __syncthreads();
-        const int a0 = q * CH;
+        const int a0 = R0;
         if (COOP) {
           // publish this chunk's last row of U for the next chunk's insert
           if (q + 1 < NCH)
-            for (int i = opaque_tid(); i < N; i += kThreads) MEM::st(T.edge + q * N + i, lds[(CH - 1) * LDR + i]);
+            for (int i = opaque_tid(); i < N; i += kThreads)
+              MEM::st(T.edge + q * N + i, nk > 0 ? lds[(nk - 1) * LDR + i] : f2(0.0f, 0.0f));
           if (!group_barrier(T, passed, bflag)) return;
         }
-        // destination rows of this chunk (and row N after the last chunk)
+        // destination rows of this chunk
         if (sp.zero) {
-          for (int i = opaque_tid(); i < CH * N; i += kThreads) {
+          for (int i = opaque_tid(); i < nk * N; i += kThreads) {
             const int lr = i / N, c = i % N;
             const float2 u = lds[lr * LDR + (c ^ HALF)];
             if (u.x != 0.0f || u.y != 0.0f) {
@@ -1376,23 +1449,25 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           }
         } else {
           // the chunk's destination rows a0 .. a1 - 1 of the tile's list
-          const int a1 = a0 + CH + ((q == NCH - 1 && sp.ar.upper != sp.ar.lower) ? 1 : 0);
+          const int a1 = a0 + nk + ((q == clast && sp.ar.upper != sp.ar.lower) ? 1 : 0);
           const float2* above = COOP ? T.edge + (q - 1) * N : prev;
           const uint32_t* rows = T.dest + dest_row_starts_offset(N);
-          const uint32_t d0 = __builtin_amdgcn_readfirstlane(gptr(rows)[a0]);
-          const uint32_t d1 = __builtin_amdgcn_readfirstlane(gptr(rows)[a1]);
+          const uint32_t d0 = nk > 0 ? __builtin_amdgcn_readfirstlane(gptr(rows)[a0]) : 0u;
+          const uint32_t d1 = nk > 0 ? __builtin_amdgcn_readfirstlane(gptr(rows)[a1]) : 0u;
           uint32_t dfirst[kInsertGroup];
           const int itid = opaque_tid();
           insert_list_first(T.dest + 2 + d0, d1 - d0, itid, dfirst);
           insert_list<N, MEM>(spec, T.rh * T.rw, sp, T.rw, scale, itid, T.dest + 2 + d0, d1 - d0, dfirst,
                               [&](int sr_, int sc_) {
-                                return sr_ >= a0 ? lds[(sr_ - a0) * LDR + (sc_ ^ HALF)]
-                                                 : (COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF]);
+                                if (sr_ >= a0 + nk) return f2(0.0f, 0.0f);  // below the support rows
+                                if (sr_ >= a0) return lds[(sr_ - a0) * LDR + (sc_ ^ HALF)];
+                                if (q == 0) return f2(0.0f, 0.0f);  // above the support rows
+                                return COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF];
                               });
         }
         __syncthreads();
         if (!COOP && q + 1 < NCH) {
-          for (int i = tid; i < N; i += kThreads) prev[i] = lds[(CH - 1) * LDR + i];
+          for (int i = tid; i < N; i += kThreads) prev[i] = nk > 0 ? lds[(nk - 1) * LDR + i] : f2(0.0f, 0.0f);
           __syncthreads();
         }
       }
