@@ -540,133 +540,12 @@ __device__ __forceinline__ bool in_extent(uint32_t e, int c) {
   return c >= (int)(e & 0xFFFFu) && c < (int)(e >> 16);
 }
 
-// Patch x pupil for STRIP vertically consecutive pixels of one patch column
-// (bilinear extract, crop.rs:129-167), handed to put(j, value). `off` is the
-// window offset of the first pixel's lower-left tap. Where the pupil is zero
-// the product is exactly zero, so the spectrum taps are not read at all;
-// vertically adjacent pixels share one row of taps. Columns outside a row's
-// support extent `ext[j]` (row of pixel j) read nothing, not even the
-// pupil. No read leaves the (n+1)^2 window.
-template <int STRIP, class MEM = PlainMem, class Put>
-__device__ __forceinline__ void gather_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
-                                             int pstride, const uint32_t* ext, int c, uint32_t off, uint32_t rw,
-                                             uint32_t dc, uint32_t du, double wll, double wlu, double wul, double wuu,
-                                             Put put) {
-  double2 a = make_double2(0.0, 0.0), b = a;
-  bool have = false;
-#pragma unroll
-  for (int j = 0; j < STRIP; ++j) {
-    const float2 P = in_extent(ext[j], c) ? pcol[j * pstride] : f2(0.0f, 0.0f);
-    float2 out = f2(0.0f, 0.0f);
-    if (P.x != 0.0f || P.y != 0.0f) {
-      if (!have) {
-        a = MEM::ld(spec + off);
-        b = MEM::ld(spec + off + dc);
-      }
-      const double2 c2 = du ? MEM::ld(spec + off + du) : a, d = du ? MEM::ld(spec + off + du + dc) : b;
-      const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
-                                     fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
-      out = fmul(f2((float)v.x, (float)v.y), P);
-      a = c2;
-      b = d;
-      have = du != 0;
-    } else {
-      have = false;
-    }
-    put(j, out);
-    off += rw;
-  }
-}
-
-// gather_strip through buffer loads, without branches: pixels outside their
-// row's support extent load their pupil value and spectrum taps with offset
-// kSkip (zero, no memory access), so the strip's pupil loads and its STRIP+1
-// rows of taps are all independent of each other and of loaded values.
-// `pofs` is the pupil element index of the strip's first pixel. Inside the
-// extents the arithmetic is gather_strip's.
-template <int STRIP, int AUX = 0, class Put>
-__device__ __forceinline__ void gather_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
-                                               int rows, int c, uint32_t off, uint32_t rw, uint32_t dc, uint32_t du,
-                                               double wll, double wlu, double wul, double wuu, Put put) {
-  // rows j >= rows of the strip are treated as empty (their extents are not read)
-  auto ex = [&](int j) { return j < rows ? ext[j] : 0u; };
-  // sub-blocks of SUB rows: SUB pupil values and SUB (+1) tap rows in flight;
-  // tap row t is the lower row of pixel t and the upper row of pixel t - 1,
-  // so each sub-block carries its last tap row into the next.
-  constexpr int SUB = STRIP < 4 ? STRIP : 4;
-  static_assert(STRIP % SUB == 0, "strip");
-  double2 clo = make_double2(0.0, 0.0), chi = clo;  // carried tap row
-#pragma unroll
-  for (int j0 = 0; j0 < STRIP; j0 += SUB) {
-    float2 P[SUB];
-    double2 lo[SUB + 1], hi[SUB + 1];
-#pragma unroll
-    for (int k = 0; k < SUB; ++k)
-      P[k] = bld_f2(rpup, in_extent(ex(j0 + k), c) ? (pofs + (j0 + k) * pstride) * 8u : kSkip);
-#pragma unroll
-    for (int k = 0; k <= SUB; ++k) {
-      const int t = j0 + k;
-      if (k == 0 && j0 > 0) {
-        lo[0] = clo;
-        hi[0] = chi;
-        continue;
-      }
-      const bool need = (t < STRIP && in_extent(ex(t), c)) || (du != 0 && t > 0 && in_extent(ex(t - 1), c));
-      const uint32_t o = off + t * rw;
-      const uint32_t ol = need ? o * 16u : kSkip;
-      lo[k] = bld_d2<AUX>(rspec, ol);
-      hi[k] = bld_d2<AUX>(rspec, dc != 0 ? ol + 16u : kSkip);
-    }
-    clo = lo[SUB];
-    chi = hi[SUB];
-    // An axis without a fraction has an upper weight of exactly zero, so its
-    // (skipped, zero) upper taps add nothing, as the lower taps repeated do.
-#pragma unroll
-    for (int k = 0; k < SUB; ++k) {
-      const double2 a = lo[k], b = hi[k], c2 = lo[k + 1], d = hi[k + 1];
-      const double2 v = make_double2(fma(d.x, wuu, fma(c2.x, wul, fma(b.x, wlu, a.x * wll))),
-                                     fma(d.y, wuu, fma(c2.y, wul, fma(b.y, wlu, a.y * wll))));
-      put(j0 + k, fmul(f2((float)v.x, (float)v.y), P[k]));  // P and the taps are zero off the extent
-    }
-  }
-}
-
-// copy_strip through buffer loads, without branches (see gather_strip_b).
-template <int STRIP, int AUX = 0, class Put>
-__device__ __forceinline__ void copy_strip_b(Rsrc rspec, Rsrc rpup, uint32_t pofs, int pstride, const uint32_t* ext,
-                                             int rows, int c, uint32_t off, uint32_t rw, Put put) {
-#pragma unroll
-  for (int j = 0; j < STRIP; ++j) {
-    const bool in = j < rows && in_extent(ext[j], c);
-    const float2 P = bld_f2(rpup, in ? (pofs + j * pstride) * 8u : kSkip);
-    const double2 v = bld_d2<AUX>(rspec, in ? (off + j * rw) * 16u : kSkip);
-    put(j, fmul(f2((float)v.x, (float)v.y), P));
-  }
-}
-
-// Integer-offset crop: the patch is a plain copy of the window (crop.rs:79-102).
-template <int STRIP, class MEM = PlainMem, class Put>
-__device__ __forceinline__ void copy_strip(const double2* __restrict__ spec, const float2* __restrict__ pcol,
-                                           int pstride, const uint32_t* ext, int c, uint32_t off, uint32_t rw,
-                                           Put put) {
-#pragma unroll
-  for (int j = 0; j < STRIP; ++j) {
-    const float2 P = in_extent(ext[j], c) ? pcol[j * pstride] : f2(0.0f, 0.0f);
-    float2 out = f2(0.0f, 0.0f);
-    if (P.x != 0.0f || P.y != 0.0f) {
-      const double2 v = MEM::ld(spec + off);
-      out = fmul(f2((float)v.x, (float)v.y), P);
-    }
-    put(j, out);
-    off += rw;
-  }
-}
-
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
 // strips `sl` (count ns; kGatherStrip vertically consecutive pixels of one
 // column, centred patch coordinates, with a mask of the pixels in the
-// support; `en` is entry tid, loaded by the caller), written to the N x N LDS field at the ifftshifted position; the
-// rest of the field is zero (written first, then a barrier). Consecutive
+// support; `en` is entry tid, loaded by the caller), written to the N x N
+// LDS field at the ifftshifted position; the rest of the field is zero
+// (written first, then a barrier). Consecutive
 // pixels of a strip share a row of taps: S + 1 rows of two spectrum taps and
 // S pupil values per strip, all independent loads issued together. An axis
 // without a fraction has an upper weight of exactly zero and its upper taps
@@ -709,6 +588,52 @@ __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcPa
       const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
                                      fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
       if (mask >> j & 1u) lds[((r0 + j + HALF) % N) * LD + cc] = fmul(f2((float)v.x, (float)v.y), P[j]);
+    }
+  }
+}
+
+// gather_strips for one row chunk of a large tile: the strips of list
+// entries [s0, s1) (the bands covering centred rows R0 .. R0 + nk - 1),
+// restricted to those rows, written to an LDS row chunk (pitch LD; chunk row
+// r - R0, ifftshifted column); the chunk's rows are zeroed first (then a
+// barrier). AUX: cache policy of the spectrum reads.
+template <int N, int LD, int AUX>
+__device__ __forceinline__ void gather_strips_rows(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
+                                                   const uint32_t* sl, uint32_t s0, uint32_t s1, int R0, int nk,
+                                                   float2* lds) {
+  constexpr int HALF = N / 2, S = (int)kGatherStrip;
+  for (int i = tid; i < nk * N; i += kThreads) lds[(i / N) * LD + i % N] = f2(0.0f, 0.0f);
+  __syncthreads();
+  const Rsrc rl = make_rsrc(sl, s1 * 4u);
+  const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
+  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+               wuu = sp.ar.wu * sp.ac.wu;
+  const double2 zero = make_double2(0.0, 0.0);
+  for (uint32_t i = s0 + (uint32_t)tid; i < s1; i += kThreads) {
+    const uint32_t e = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
+    const int r0 = (int)(e & 0xFFu), c = (int)((e >> 8) & 0xFFu);
+    uint32_t mask = e >> 16;
+#pragma unroll
+    for (int j = 0; j < S; ++j)  // rows of other chunks
+      if (r0 + j < R0 || r0 + j >= R0 + nk) mask &= ~(1u << j);
+    const uint32_t o = (sp.ar.lower + r0) * rw + sp.ac.lower + c;
+    double2 t0[S + 1], t1[S + 1];
+    float2 P[S];
+#pragma unroll
+    for (int j = 0; j <= S; ++j) {
+      const bool need = (j < S && (mask >> j & 1u)) || (fr && j > 0 && (mask >> (j - 1) & 1u));
+      t0[j] = bld_d2<AUX>(rspec, need ? (o + j * rw) * 16u : kSkip);
+      t1[j] = bld_d2<AUX>(rspec, need && fc ? (o + j * rw + 1) * 16u : kSkip);
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) P[j] = bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+    const int cs = c ^ HALF;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
+      const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
+                                     fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
+      if (mask >> j & 1u) lds[(r0 + j - R0) * LD + cs] = fmul(f2((float)v.x, (float)v.y), P[j]);
     }
   }
 }
@@ -1167,16 +1092,19 @@ __device__ __forceinline__ bool group_barrier(const TileDesc& T, uint64_t& passe
 
 // Tiles whose N x N complex64 field does not fit in LDS (N = 256: 512 KiB).
 // The field lives in the tile's global scratch buffer T.field (it stays in
-// L2 / the last-level cache) and passes through LDS in chunks of CH lines:
+// L2 / the last-level cache) and passes through LDS in chunks of at most CH
+// lines:
 //
-//   A  rows, in shifted order: patch x pupil written ifftshifted, inverse
-//      row transforms, back to T.field
-//   B  columns: inverse column transforms, amplitude replacement kept as the
-//      field change psi' - psi, forward column transforms, back to T.field
-//   C  rows, in centred order: forward row transforms, fftshift by index,
-//      U = conj(P) diff / N / den in LDS, then the adjoint insert of the
-//      destination rows this chunk completes (the row above the chunk is the
-//      previous chunk's last row of U)
+//   A  rows with pupil support, split evenly over the chunks: patch x pupil
+//      (support strips of the tile's list) written ifftshifted, inverse row
+//      transforms, out to their natural rows of T.field
+//   B  columns (the rows without support read as zero): inverse column
+//      transforms, amplitude replacement kept as the field change psi' - psi,
+//      forward column transforms, back to T.field
+//   C  the rows of A, in centred order: forward row transforms, fftshift by
+//      index, U = conj(P) diff / N / den in LDS, then the adjoint insert of
+//      the destination rows this chunk completes (the row above the chunk is
+//      the previous chunk's last row of U)
 //
 // One workgroup runs the chunks in turn (MEM = PlainMem), or a cooperative
 // group of N/CH workgroups takes one chunk each (MEM = CoherentMem): group
@@ -1213,10 +1141,8 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   constexpr int LDR = N + 1;   // row chunk: CH rows of N (+1 pad)
   constexpr int LDC = CH + 1;  // column chunk: N rows of CH (+1 pad)
   constexpr int P0 = DefaultPlan<N>::P0, P1 = N / P0;
-  constexpr int SPLIT = kThreads / N;  // row strips per chunk in phase A
-  constexpr int STRIP = CH / SPLIT;    // rows per strip
   constexpr bool COOP = MEM::kCoherent;
-  static_assert((N & (N - 1)) == 0 && HALF % CH == 0 && CH % SPLIT == 0, "big tile shape");
+  static_assert((N & (N - 1)) == 0 && HALF % CH == 0, "big tile shape");
   // a cooperative group has one member per chunk; a lone workgroup runs them all
   const int first = COOP ? (int)T.member : 0, step = COOP ? NCH : 1;
   const bool lead = !COOP || T.member == 0;
@@ -1287,22 +1213,13 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         tid = opaque_tid();
         const int nk = chunk_rows(k), R0 = rlo + k * crows;
         {
-          const int c = tid % N, s0 = (tid / N) * STRIP;
-          const int rc = R0 + s0;  // first centred row of the strip
-          const int cs = c ^ HALF;
-          auto put = [&](int j, float2 v) { lds[(s0 + j) * LDR + cs] = v; };
-          if (s0 < nk) {
-            if (sp.zero) {
-              copy_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, nk - s0, c,
-                                       ((uint32_t)sp.sr + rc) * T.rw + (uint32_t)sp.sc + c, T.rw, put);
-            } else {
-              gather_strip_b<STRIP, AUX>(rspec, rpup, rc * N + c, N, ext + rc, nk - s0, c,
-                                         (sp.ar.lower + rc) * T.rw + sp.ac.lower + c, T.rw,
-                                         sp.ac.upper - sp.ac.lower, (sp.ar.upper - sp.ar.lower) * T.rw,
-                                         sp.ar.wl * sp.ac.wl, sp.ar.wl * sp.ac.wu, sp.ar.wu * sp.ac.wl,
-                                         sp.ar.wu * sp.ac.wu, put);
-            }
-          }
+          // the support strips of the bands that cover the chunk's rows
+          const uint32_t* bands = T.dest + dest_band_starts_offset(N);
+          const uint32_t st0 = nk > 0 ? __builtin_amdgcn_readfirstlane(gptr(bands)[R0 / (int)kGatherStrip]) : 0u;
+          const uint32_t st1 =
+              nk > 0 ? __builtin_amdgcn_readfirstlane(gptr(bands)[(R0 + nk - 1) / (int)kGatherStrip + 1]) : 0u;
+          gather_strips_rows<N, LDR, AUX>(rspec, rpup, sp, T.rw, tid, T.dest + dest_strips_offset(N), st0, st1, R0,
+                                          nk, lds);
         }
         __syncthreads();
         stage<N, CH, P0, 1, LDR, 1, true, true>(lds, twi, tid, live_rows(nk));

@@ -884,6 +884,7 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
       cstart_s[k] = sacc;
       sacc += cnt_s[k];
     }
+    cstart_s[bands] = sacc;
     start_s[rows] = acc;
     dest[0] = acc;
     dest[1] = sacc;
@@ -893,6 +894,8 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
   uint32_t* sl = dest + dest_strips_offset(n);
   uint32_t* rs = dest + dest_row_starts_offset(n);
   for (uint32_t a = threadIdx.x; a <= rows; a += blockDim.x) rs[a] = start_s[a];
+  uint32_t* bs = dest + dest_band_starts_offset(n);
+  for (uint32_t k = threadIdx.x; k <= bands; k += blockDim.x) bs[k] = cstart_s[k];
   for (uint32_t a = 0; a < rows; ++a)
     for (uint32_t j = threadIdx.x; j < len_s[a]; j += blockDim.x) dl[start_s[a] + j] = a | ((lo_s[a] + j) << 16);
   for (uint32_t k = 0; k < bands; ++k) {

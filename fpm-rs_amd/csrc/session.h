@@ -40,10 +40,17 @@ Config make_config(const fpmhip_algorithm& a);
 constexpr uint32_t kGatherStrip = 5;
 // Layout of SessionBuffers::dest for an n x n tile, in uint32 words: header
 // (2), insert destinations ((n+1)^2 slots), support strips ((n / kGatherStrip
-// + 1) n slots), then the first destination index of each window row (n + 2).
+// + 1) n slots), the first destination index of each window row (n + 2),
+// then the first strip index of each band of kGatherStrip rows (n /
+// kGatherStrip + 2).
 __host__ __device__ inline uint64_t dest_strips_offset(uint32_t n) { return 2 + (uint64_t)(n + 1) * (n + 1); }
-__host__ __device__ inline uint64_t dest_row_starts_offset(uint32_t n) { return dest_strips_offset(n) + (uint64_t)(n / kGatherStrip + 1) * n; }
-__host__ __device__ inline uint64_t dest_table_words(uint32_t n) { return dest_row_starts_offset(n) + n + 2; }
+__host__ __device__ inline uint64_t dest_row_starts_offset(uint32_t n) {
+  return dest_strips_offset(n) + (uint64_t)(n / kGatherStrip + 1) * n;
+}
+__host__ __device__ inline uint64_t dest_band_starts_offset(uint32_t n) { return dest_row_starts_offset(n) + n + 2; }
+__host__ __device__ inline uint64_t dest_table_words(uint32_t n) {
+  return dest_band_starts_offset(n) + n / kGatherStrip + 2;
+}
 
 struct SessionBuffers {
   DeviceBuffer spec;      // double2[M] centred object spectrum (FP64 accumulation)
