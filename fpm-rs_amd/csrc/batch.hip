@@ -433,7 +433,7 @@ void check_tile_status(int32_t status) {
 
 void validate_schedule(const fpmhip_batch_schedule& s) {
   if (s.group_members != 1 && s.group_members != kGroupMembers)
-    fail(FPMHIP_INVALID_PARAMETER, "group_members must be 1 or 4", "group_members");
+    fail(FPMHIP_INVALID_PARAMETER, "group_members must be 1 or 8", "group_members");
   if (s.largest_first > 1) fail(FPMHIP_INVALID_PARAMETER, "largest_first must be 0 or 1", "largest_first");
   if (s.sweeps_per_launch > 1000)
     fail(FPMHIP_INVALID_PARAMETER, "sweeps_per_launch must be in 0..1000", "sweeps_per_launch");

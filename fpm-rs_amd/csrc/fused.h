@@ -36,7 +36,7 @@ struct TileDesc {
   // share one tile; this descriptor is workgroup `member` of the group
   uint64_t* group;        // [0] barrier arrivals, [1] barriers completed before this launch, [2] stall flag
   double2* partial;       // [members] per-member objective sums (loss, count)
-  float2* edge;           // [N / 64][N] last row of U of each 64-row chunk
+  float2* edge;           // [N / kBigChunk][N] last row of U of each chunk
   uint32_t member, members;
   uint32_t plan;          // first-stage radix of the LDS line transforms (0: default)
   uint32_t trace_len;
@@ -75,8 +75,10 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
 // First-stage radix choices of the fused sweep for an n x n tile; the first
 // entry is the default. Returns the number of choices (0: size unsupported).
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]);
-// Workgroups of a cooperative group for a 256^2 tile (one 64-line chunk each).
-constexpr uint32_t kGroupMembers = 4;
+// Lines per chunk of a 256^2 tile's field, and the workgroups of its
+// cooperative group (one chunk each).
+constexpr uint32_t kBigChunk = 32;
+constexpr uint32_t kGroupMembers = kFusedMaxN / kBigChunk;
 // Bytes of a session's cooperative-group state for an n x n tile.
 size_t group_state_bytes(uint32_t n);
 // Largest grid whose workgroups are all resident at once (one per CU at the

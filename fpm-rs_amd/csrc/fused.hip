@@ -18,6 +18,8 @@
 // spectrum, momentum, denominators and objective reductions in FP64.
 // Workgroup-internal reductions use a fixed order, so results are
 // deterministic run to run. No atomics anywhere.
+#include <algorithm>
+
 #include "fused.h"
 
 namespace fpmhip {
@@ -1117,7 +1119,7 @@ template <int N>
 struct BigPlan;
 template <>
 struct BigPlan<256> {
-  static constexpr int CH = 64;  // lines per chunk
+  static constexpr int CH = (int)kBigChunk;  // lines per chunk
 };
 
 template <int N>
@@ -1456,10 +1458,18 @@ __global__ void __launch_bounds__(kThreads)
 
 }  // namespace
 
+// LDS of one tile of n x n pixels (the layout of k_sweep).
+static size_t sweep_lds_bytes_one(uint32_t n) {
+  return ((size_t)sweep_field_elems(n) + (n > kFusedLdsMaxN ? n : 0) + 2 * (size_t)n) * sizeof(float2) +
+         2 * (kThreads / 64) * sizeof(double) + 16 + (size_t)n * sizeof(uint32_t) + 2 * kRecPitch * sizeof(uint32_t);
+}
+
+// A launch's LDS: the largest layout of any tile size up to nmax (a 256^2
+// tile's chunk can need less than a 128^2 tile's whole field).
 size_t sweep_lds_bytes(uint32_t nmax) {
-  return ((size_t)sweep_field_elems(nmax) + (nmax > kFusedLdsMaxN ? nmax : 0) + 2 * (size_t)nmax) * sizeof(float2) +
-         2 * (kThreads / 64) * sizeof(double) + 16 + (size_t)nmax * sizeof(uint32_t) +
-         2 * kRecPitch * sizeof(uint32_t);
+  size_t bytes = 0;
+  for (uint32_t n = 32; n <= nmax && n <= kFusedMaxN; n *= 2) bytes = std::max(bytes, sweep_lds_bytes_one(n));
+  return bytes;
 }
 
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
@@ -1482,8 +1492,8 @@ uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
 }
 
 size_t group_state_bytes(uint32_t n) {
-  // [0, 24) counters, [64, 64 + 16 kGroupMembers) partial sums, then n/64 edge rows
-  return 64 + 16 * (size_t)kGroupMembers + (size_t)(n / 64) * n * sizeof(float2);
+  // [0, 24) counters, [64, 64 + 16 kGroupMembers) partial sums, then n / kBigChunk edge rows
+  return 64 + 16 * (size_t)kGroupMembers + (size_t)(n / kBigChunk) * n * sizeof(float2);
 }
 
 void prepare_sweep(uint32_t nmax) {

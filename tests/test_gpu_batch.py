@@ -149,14 +149,14 @@ def test_batch_256_tile_identical_alone_and_in_mix(ctx):
 
 
 def test_cooperative_group_matches_single_workgroup(ctx):
-    """A 256^2 tile split over a cooperative group of 4 workgroups gives the
+    """A 256^2 tile split over a cooperative group of 8 workgroups gives the
     same spectrum, bit for bit, as one workgroup running all chunks (the
     per-pixel arithmetic and insert order are the same); the objective sums
     differ only by summation order."""
     alg = Mpie(iterations=3, momentum_interval=20, momentum_friction=0.5, momentum_feedback=0.5)
     tiles = [tile(256, 9, 91), tile(128, 7, 92), tile(256, 7, 93)]
     results = {}
-    for g in (4, 1):
+    for g in (8, 1):
         ss = sessions_for(tiles, alg)
         b = runtime.Batch(ss)
         b.set_schedule(group_members=g)
@@ -164,9 +164,9 @@ def test_cooperative_group_matches_single_workgroup(ctx):
         tr = b.run(3)
         results[g] = ([s.spectrum() for s in ss], tr)
         say("group", g, "sweep ms", b.timing()[0])
-    for a, c in zip(results[4][0], results[1][0]):
+    for a, c in zip(results[8][0], results[1][0]):
         assert np.array_equal(a, c)
-    assert np.allclose(results[4][1], results[1][1], rtol=1e-12, atol=0)
+    assert np.allclose(results[8][1], results[1][1], rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize("radix", [[4, 4, 8, 0], [8, 16, 16, 0]])
@@ -307,7 +307,7 @@ def test_autotuner_keeps_a_valid_schedule(ctx):
     assert rep["candidates"] == 7
     assert rep["best_ms"] <= rep["first_ms"]
     sch = rep["schedule"]
-    assert sch["group_members"] in (1, 4) and sch["largest_first"] in (0, 1)
+    assert sch["group_members"] in (1, 8) and sch["largest_first"] in (0, 1)
     b.run(2)
     for (model, meas), s in zip(tiles, ss):
         ref = oracle.run(model, meas, alg.to_c(), 2)
