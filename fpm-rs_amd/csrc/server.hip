@@ -115,9 +115,10 @@ uint64_t job_bytes(const fpmhip_model_desc& d, uint64_t iterations) {
   return 2 * M * 16 + 2 * M * 8 + 12 * N * 16 + F * N * 2 + N * 25 + F * 64 + 16 + (iterations + F) * 8;
 }
 
-// Speed-up of a cooperative group over one workgroup on a 256^2 tile
-// (measured: about 2.8x), used only to balance per-tick sweep quotas.
-constexpr double kGroupSpeedup = 2.8;
+// Speed-up of a cooperative group (kGroupMembers workgroups) over one
+// workgroup on a 256^2 tile (measured: about 5x), used only to balance
+// per-tick sweep quotas.
+constexpr double kGroupSpeedup = 5.0;
 
 // Work of one sweep of a job's tile: LED updates x pixels per update.
 double sweep_work(const Job& j) {
