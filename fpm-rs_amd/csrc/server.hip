@@ -84,7 +84,8 @@ struct Job {
 }  // namespace
 
 struct fpmhip_server {
-  fpmhip_context own;  // private stream: the worker never shares the caller's stream
+  fpmhip_context own;    // private stream: the worker never shares the caller's stream
+  fpmhip_context stage;  // second private stream: admissions, while a tick runs on `own`
   Context* ctx = nullptr;
   fpmhip_server_config cfg{};
   std::mutex mu;
@@ -102,6 +103,8 @@ struct fpmhip_server {
 
   void run();
   bool admit(Job& j);
+  // Admits jobs (admit), marks them resident or failed, and empties the list.
+  void admit_all(std::vector<Job*>& to_admit);
   void finish(Job& j);
 };
 
@@ -129,9 +132,9 @@ double sweep_work(const Job& j) {
 }  // namespace
 
 bool fpmhip_server::admit(Job& j) {
-  int rc = fpmhip_model_create(&own, &j.desc, &j.model);
+  int rc = fpmhip_model_create(&stage, &j.desc, &j.model);
   if (!rc)
-    rc = fpmhip_measurements_create(&own, j.model, j.counts.empty() ? nullptr : j.counts.data(),
+    rc = fpmhip_measurements_create(&stage, j.model, j.counts.empty() ? nullptr : j.counts.data(),
                                     j.frames.empty() ? nullptr : j.frames.data(),
                                     j.weights.empty() ? nullptr : j.weights.data(), nullptr, &j.meas);
   if (!rc) rc = fpmhip_session_create(j.model, j.meas, FPMHIP_MIXED_F32, &j.alg, &j.sess);
@@ -148,7 +151,8 @@ bool fpmhip_server::admit(Job& j) {
     return false;
   }
   j.scratch.allocate(16 + (j.iterations + j.sess->model->F) * sizeof(double));
-  HIP_CHECK(hipMemsetAsync(j.scratch.ptr, 0, j.scratch.bytes, ctx->stream));
+  HIP_CHECK(hipMemsetAsync(j.scratch.ptr, 0, j.scratch.bytes, stage.stream));
+  HIP_CHECK(hipStreamSynchronize(stage.stream));
   // the counter starts from the session's (zero after initialisation)
   return true;
 }
@@ -181,6 +185,33 @@ void fpmhip_server::finish(Job& j) {
   j.release_device();
 }
 
+void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
+  for (Job* j : to_admit) {
+    bool ok = false;
+    try {
+      ok = admit(*j);
+    } catch (const Failure& f) {
+      j->error_code = f.code;
+      j->error = fpmhip_last_error();
+      j->release_device();
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (ok) {
+      j->state = kResident;
+      j->t_admit = now_s();
+      resident.push_back(j);
+      resident_bytes += j->bytes;
+      stats.admitted++;
+    } else {
+      j->state = kFailed;
+      j->t_done = now_s();
+      stats.failed++;
+      cv_done.notify_all();
+    }
+  }
+  to_admit.clear();
+}
+
 void fpmhip_server::run() {
   (void)hipSetDevice(ctx->device);
   double busy_since = -1.0;
@@ -203,35 +234,13 @@ void fpmhip_server::run() {
         ++slots;
       }
     }
-    for (Job* j : to_admit) {
-      bool ok = false;
-      try {
-        ok = admit(*j);
-      } catch (const Failure& f) {
-        j->error_code = f.code;
-        j->error = fpmhip_last_error();
-        j->release_device();
-      }
-      std::lock_guard<std::mutex> lock(mu);
-      if (ok) {
-        j->state = kResident;
-        j->t_admit = now_s();
-        resident.push_back(j);
-        resident_bytes += j->bytes;
-        stats.admitted++;
-      } else {
-        j->state = kFailed;
-        j->t_done = now_s();
-        stats.failed++;
-        cv_done.notify_all();
-      }
-    }
     std::vector<Job*> active;
     {
       std::lock_guard<std::mutex> lock(mu);
       active = resident;
     }
     if (active.empty()) {
+      admit_all(to_admit);
       if (busy_since >= 0.0) {
         std::lock_guard<std::mutex> lock(mu);
         stats.busy_seconds += now_s() - busy_since;
@@ -279,8 +288,12 @@ void fpmhip_server::run() {
       HIP_CHECK(hipMemcpyAsync(tiles.ptr, launch.data(), launch.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));
       launch_sweep(tiles.as<TileDesc>(), (uint32_t)launch.size(), tw.as<float2>(), nmax, cooperative, ctx->stream);
+      // the queued jobs are admitted (uploads and initialisation on the
+      // second stream) while the tick runs
+      admit_all(to_admit);
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (const Failure& f) {
+      admit_all(to_admit);  // no-op when they were admitted before the failure
       std::lock_guard<std::mutex> lock(mu);
       for (Job* j : active) {
         j->error_code = f.code;
@@ -289,9 +302,9 @@ void fpmhip_server::run() {
         j->state = kFailed;
         j->t_done = now_s();
         stats.failed++;
+        resident.erase(std::find(resident.begin(), resident.end(), j));
+        resident_bytes -= j->bytes;
       }
-      resident.clear();
-      resident_bytes = 0;
       cv_done.notify_all();
       continue;
     }
@@ -343,6 +356,9 @@ int fpmhip_server_create(fpmhip_context* c, const fpmhip_server_config* config, 
     s->own.device = c->device;
     s->own.props = c->props;
     HIP_CHECK(hipStreamCreateWithFlags(&s->own.stream, hipStreamNonBlocking));
+    s->stage.device = c->device;
+    s->stage.props = c->props;
+    HIP_CHECK(hipStreamCreateWithFlags(&s->stage.stream, hipStreamNonBlocking));
     s->ctx = &s->own;
     s->cfg = *config;
     make_sweep_twiddles(s->tw, s->own.stream);
@@ -363,9 +379,10 @@ int fpmhip_server_destroy(fpmhip_server* s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     for (auto& kv : s->jobs) kv.second->release_device();
-    hipStream_t st = s->own.stream;
+    hipStream_t st = s->own.stream, st2 = s->stage.stream;
     delete s;
     if (st) (void)hipStreamDestroy(st);
+    if (st2) (void)hipStreamDestroy(st2);
   });
 }
 
