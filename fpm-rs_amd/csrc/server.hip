@@ -8,6 +8,8 @@
 // pixels), so that a small tile is not held to the pace of the largest one
 // resident. A job that reaches its iteration count is finished (result
 // transform, download) and leaves; the queue is re-examined between ticks.
+// Admissions and the downloads of finished jobs run on a second stream while
+// the next tick's sweep runs on the first.
 // Tiles are independent workgroups and sweeps are sequential within a tile,
 // so a job's result does not depend on which other jobs share its ticks.
 #include <algorithm>
@@ -58,6 +60,7 @@ struct Job {
   fpmhip_algorithm alg{};
   uint64_t iterations = 0, done_iterations = 0;
   uint64_t bytes = 0;
+  bool finishing = false;  // iterations done, results not yet downloaded
   // device objects while resident
   fpmhip_model* model = nullptr;
   fpmhip_measurements* meas = nullptr;
@@ -70,6 +73,15 @@ struct Job {
   double t_submit = 0, t_admit = 0, t_done = 0;
   std::vector<double> object, spectrum, trace;
 
+  // Moves the device objects into `to` (released later, by its own
+  // release_device), leaving this job without device state.
+  void hand_over(Job& to) {
+    std::swap(sess, to.sess);
+    std::swap(meas, to.meas);
+    std::swap(model, to.model);
+    std::swap(scratch, to.scratch);
+    to.bytes = bytes;
+  }
   void release_device() {
     if (sess) fpmhip_session_destroy(sess);
     if (meas) fpmhip_measurements_destroy(meas);
@@ -106,6 +118,10 @@ struct fpmhip_server {
   // Admits jobs (admit), marks them resident or failed, and empties the list.
   void admit_all(std::vector<Job*>& to_admit);
   void finish(Job& j);
+  // Finishes jobs (finish) and marks them done or failed; with `held`, their
+  // device objects are handed over to it instead of released (a free could
+  // wait for the running tick) and stay counted in resident_bytes.
+  void finish_all(std::vector<Job*>& done, std::vector<Job>* held);
 };
 
 namespace {
@@ -167,10 +183,10 @@ void fpmhip_server::finish(Job& j) {
   if (!rc) rc = fpmhip_session_get_object(&s, j.object.data());
   if (!rc) {
     HIP_CHECK(hipMemcpyAsync(j.trace.data(), j.scratch.as<char>() + 16, j.iterations * sizeof(double),
-                             hipMemcpyDeviceToHost, ctx->stream));
+                             hipMemcpyDeviceToHost, stage.stream));
     int32_t status = 0;
-    HIP_CHECK(hipMemcpyAsync(&status, j.scratch.as<char>() + 12, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(&status, j.scratch.as<char>() + 12, sizeof(status), hipMemcpyDeviceToHost, stage.stream));
+    HIP_CHECK(hipStreamSynchronize(stage.stream));
     if (status == 2) {
       rc = FPMHIP_DEVICE_ERROR;
       j.error = "a cooperative tile group stalled at a group barrier";
@@ -182,7 +198,34 @@ void fpmhip_server::finish(Job& j) {
     j.error = fpmhip_last_error();
   }
   j.error_code = rc;
-  j.release_device();
+}
+
+void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) {
+  for (Job* j : done) {
+    try {
+      finish(*j);
+    } catch (const Failure& f) {
+      j->error_code = f.code;
+      j->error = fpmhip_last_error();
+    }
+    if (held) {
+      held->emplace_back();
+      j->hand_over(held->back());
+    } else {
+      j->release_device();
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    j->t_done = now_s();
+    j->state = j->error_code ? kFailed : kDone;
+    if (j->error_code)
+      stats.failed++;
+    else
+      stats.completed++;
+    resident.erase(std::find(resident.begin(), resident.end(), j));
+    if (!held) resident_bytes -= j->bytes;
+    cv_done.notify_all();
+  }
+  done.clear();
 }
 
 void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
@@ -215,6 +258,7 @@ void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
 void fpmhip_server::run() {
   (void)hipSetDevice(ctx->device);
   double busy_since = -1.0;
+  std::vector<Job*> finished;  // jobs whose last tick has run
   for (;;) {
     std::vector<Job*> to_admit;
     {
@@ -237,9 +281,11 @@ void fpmhip_server::run() {
     std::vector<Job*> active;
     {
       std::lock_guard<std::mutex> lock(mu);
-      active = resident;
+      for (Job* j : resident)
+        if (!j->finishing) active.push_back(j);
     }
     if (active.empty()) {
+      finish_all(finished, nullptr);
       admit_all(to_admit);
       if (busy_since >= 0.0) {
         std::lock_guard<std::mutex> lock(mu);
@@ -251,6 +297,17 @@ void fpmhip_server::run() {
     if (busy_since < 0.0) busy_since = now_s();
     // one tick: every resident tile advances by its quota of sweeps
     std::vector<uint32_t> quota(active.size());
+    std::vector<Job> held;
+    auto release_held = [&] {
+      uint64_t freed = 0;
+      for (Job& h : held) {
+        h.release_device();
+        freed += h.bytes;
+      }
+      held.clear();
+      std::lock_guard<std::mutex> lock(mu);
+      resident_bytes -= freed;
+    };
     try {
       std::vector<TileDesc> desc(active.size());
       uint32_t nmax = 0;
@@ -288,12 +345,18 @@ void fpmhip_server::run() {
       HIP_CHECK(hipMemcpyAsync(tiles.ptr, launch.data(), launch.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));
       launch_sweep(tiles.as<TileDesc>(), (uint32_t)launch.size(), tw.as<float2>(), nmax, cooperative, ctx->stream);
-      // the queued jobs are admitted (uploads and initialisation on the
-      // second stream) while the tick runs
+      // while the tick runs: the results of the jobs finished by the last
+      // tick are downloaded, and the queued jobs are admitted (uploads and
+      // initialisation), on the second stream
+      finish_all(finished, &held);
       admit_all(to_admit);
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      release_held();
     } catch (const Failure& f) {
-      admit_all(to_admit);  // no-op when they were admitted before the failure
+      // no-ops for the lists handled before the failure
+      release_held();
+      finish_all(finished, nullptr);
+      admit_all(to_admit);
       std::lock_guard<std::mutex> lock(mu);
       for (Job* j : active) {
         j->error_code = f.code;
@@ -308,34 +371,17 @@ void fpmhip_server::run() {
       cv_done.notify_all();
       continue;
     }
-    std::vector<Job*> finished;
     {
       std::lock_guard<std::mutex> lock(mu);
       stats.ticks++;
       for (size_t i = 0; i < active.size(); ++i) {
         Job* j = active[i];
         j->done_iterations += quota[i];
-        if (j->done_iterations >= j->iterations) finished.push_back(j);
+        if (j->done_iterations >= j->iterations) {
+          j->finishing = true;
+          finished.push_back(j);
+        }
       }
-    }
-    for (Job* j : finished) {
-      try {
-        finish(*j);
-      } catch (const Failure& f) {
-        j->error_code = f.code;
-        j->error = fpmhip_last_error();
-        j->release_device();
-      }
-      std::lock_guard<std::mutex> lock(mu);
-      j->t_done = now_s();
-      j->state = j->error_code ? kFailed : kDone;
-      if (j->error_code)
-        stats.failed++;
-      else
-        stats.completed++;
-      resident.erase(std::find(resident.begin(), resident.end(), j));
-      resident_bytes -= j->bytes;
-      cv_done.notify_all();
     }
   }
   // drop resident device state on shutdown
