@@ -60,6 +60,7 @@ struct Job {
   fpmhip_algorithm alg{};
   uint64_t iterations = 0, done_iterations = 0;
   uint64_t bytes = 0;
+  uint32_t wgs = 1;        // workgroups of its tile in a tick (a cooperative group for large tiles)
   bool finishing = false;  // iterations done, results not yet downloaded
   // device objects while resident
   fpmhip_model* model = nullptr;
@@ -106,6 +107,8 @@ struct fpmhip_server {
   std::map<uint64_t, std::unique_ptr<Job>> jobs;
   std::vector<Job*> resident;
   uint64_t resident_bytes = 0;
+  uint64_t resident_wgs = 0;  // workgroups of the resident, unfinished tiles
+  uint32_t wg_limit = 0;      // workgroups of k_sweep resident at once on the device
   uint64_t next_id = 1;
   bool stop = false;
   fpmhip_server_statistics stats{};
@@ -223,6 +226,7 @@ void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) 
       stats.completed++;
     resident.erase(std::find(resident.begin(), resident.end(), j));
     if (!held) resident_bytes -= j->bytes;
+    resident_wgs -= j->wgs;
     cv_done.notify_all();
   }
   done.clear();
@@ -244,6 +248,7 @@ void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
       j->t_admit = now_s();
       resident.push_back(j);
       resident_bytes += j->bytes;
+      resident_wgs += j->wgs;
       stats.admitted++;
     } else {
       j->state = kFailed;
@@ -265,16 +270,19 @@ void fpmhip_server::run() {
       std::unique_lock<std::mutex> lock(mu);
       cv_work.wait(lock, [&] { return stop || !queue.empty() || !resident.empty(); });
       if (stop) break;
-      // admission by slots and HBM footprint (a job larger than the whole
-      // budget is still admitted when nothing else is resident)
-      uint64_t bytes = resident_bytes;
+      // admission by slots, HBM footprint and workgroups: the resident
+      // tiles, large ones as cooperative groups, fit on the device at once,
+      // so that no large tile falls back to a single workgroup (a job over
+      // the HBM or workgroup budget is still admitted when nothing else is)
+      uint64_t bytes = resident_bytes, wgs = resident_wgs;
       size_t slots = resident.size();
       while (!queue.empty() && slots < cfg.max_tiles) {
         Job* j = jobs[queue.front()].get();
-        if (slots > 0 && bytes + j->bytes > cfg.hbm_budget_bytes) break;
+        if (slots > 0 && (bytes + j->bytes > cfg.hbm_budget_bytes || wgs + j->wgs > wg_limit)) break;
         queue.pop_front();
         to_admit.push_back(j);
         bytes += j->bytes;
+        wgs += j->wgs;
         ++slots;
       }
     }
@@ -367,6 +375,7 @@ void fpmhip_server::run() {
         stats.failed++;
         resident.erase(std::find(resident.begin(), resident.end(), j));
         resident_bytes -= j->bytes;
+        resident_wgs -= j->wgs;
       }
       cv_done.notify_all();
       continue;
@@ -407,6 +416,7 @@ int fpmhip_server_create(fpmhip_context* c, const fpmhip_server_config* config, 
     HIP_CHECK(hipStreamCreateWithFlags(&s->stage.stream, hipStreamNonBlocking));
     s->ctx = &s->own;
     s->cfg = *config;
+    s->wg_limit = std::max<uint32_t>(sweep_coresident_limit(kFusedMaxN), kGroupMembers);
     make_sweep_twiddles(s->tw, s->own.stream);
     s->tiles.allocate(config->max_tiles * sizeof(TileDesc));
     fpmhip_server* raw = s.get();
@@ -473,6 +483,7 @@ int fpmhip_server_submit(fpmhip_server* s, const fpmhip_model_desc* d, const uin
     j->alg = *alg;
     j->iterations = alg->iterations;
     j->bytes = job_bytes(*d, alg->iterations);
+    j->wgs = d->image_h > kFusedLdsMaxN ? kGroupMembers : 1;
     j->t_submit = now_s();
     {
       std::lock_guard<std::mutex> lock(s->mu);

@@ -364,7 +364,9 @@ int fpmhip_batch_tune(fpmhip_batch* batch, uint32_t trial_sweeps, fpmhip_batch_t
  * alone or among others. Jobs must satisfy the batched form's constraints
  * (MIXED_F32, square 32/64/128/256 images, no multiplexing, not EPRY). */
 typedef struct fpmhip_server_config {
-  uint64_t max_tiles;        /* resident tiles at most (workgroups per tick) */
+  uint64_t max_tiles;        /* resident tiles at most; admission also keeps the resident tiles'
+                                workgroups (8 per 256^2 tile, a cooperative group) within what
+                                the device holds at once */
   uint64_t hbm_budget_bytes; /* admission limit for resident job state */
 } fpmhip_server_config;
 
