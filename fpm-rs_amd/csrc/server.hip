@@ -141,6 +141,10 @@ uint64_t job_bytes(const fpmhip_model_desc& d, uint64_t iterations) {
 // workgroup on a 256^2 tile (measured: about 5x), used only to balance
 // per-tick sweep quotas.
 constexpr double kGroupSpeedup = 5.0;
+// A tick lasts about this many sweeps of its heaviest tile: every other tile
+// runs floor(kTickSweeps x heaviest work / its work) sweeps, so the rounding
+// leaves a tile idle for less than 1 / kTickSweeps of the tick.
+constexpr double kTickSweeps = 2.0;
 
 // Work of one sweep of a job's tile: LED updates x pixels per update.
 double sweep_work(const Job& j) {
@@ -342,7 +346,7 @@ void fpmhip_server::run() {
         if (t.members > 1 && t.member == 0) work[index[t.spec]] /= kGroupSpeedup;
       const double wmax = *std::max_element(work.begin(), work.end());
       for (size_t i = 0; i < active.size(); ++i) {
-        const double share = std::max(std::floor(wmax / work[i]), 1.0);
+        const double share = std::max(std::floor(kTickSweeps * wmax / work[i]), 1.0);
         quota[i] = (uint32_t)std::min<double>(share, (double)(active[i]->iterations - active[i]->done_iterations));
       }
       for (TileDesc& t : launch) t.sweeps = quota[index[t.spec]];
