@@ -1440,10 +1440,10 @@ __global__ void __launch_bounds__(kThreads)
         sweep_tile<64, 8>(T, lds, twf, twi, red, ext, rec);
       break;
     case 128:
-      if (T.plan == 8)
-        sweep_tile<128, 8>(T, lds, twf, twi, red, ext, rec);
-      else
+      if (T.plan == 16)
         sweep_tile<128, 16>(T, lds, twf, twi, red, ext, rec);
+      else
+        sweep_tile<128, 8>(T, lds, twf, twi, red, ext, rec);
       break;
     case 256:
       if (T.members > 1)
@@ -1481,7 +1481,7 @@ uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
       out[0] = 8, out[1] = 4, out[2] = 16;
       return 3;
     case 128:
-      out[0] = 16, out[1] = 8;
+      out[0] = 8, out[1] = 16;
       return 2;
     case 256:
       out[0] = 16;

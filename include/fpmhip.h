@@ -331,7 +331,7 @@ typedef struct fpmhip_batch_schedule {
                              chunks; used only when the whole grid can be resident at once) */
   uint32_t largest_first; /* 1: launch the largest tiles first; 0: session order */
   uint32_t radix[4];      /* first-stage FFT radix for 32, 64, 128, 256 pixel tiles (0: default;
-                             32: 8|4, 64: 8|4|16, 128: 16|8, 256: 16) */
+                             32: 8|4, 64: 8|4|16, 128: 8|16, 256: 16) */
   uint32_t sweeps_per_launch; /* sweeps one launch runs (1..1000; 0: default, 10): fewer launches
                                  and host synchronisations per run */
 } fpmhip_batch_schedule;
