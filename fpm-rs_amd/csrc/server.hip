@@ -97,8 +97,10 @@ struct Job {
 }  // namespace
 
 struct fpmhip_server {
-  fpmhip_context own;    // private stream: the worker never shares the caller's stream
-  fpmhip_context stage;  // second private stream: admissions, while a tick runs on `own`
+  static constexpr int kAdmitThreads = 4;
+  fpmhip_context own;                   // private stream of the ticks: never the caller's stream
+  fpmhip_context stage[kAdmitThreads];  // streams of the admission threads, while a tick runs
+  fpmhip_context drain;                 // stream of the result downloads, while a tick runs
   Context* ctx = nullptr;
   fpmhip_server_config cfg{};
   std::mutex mu;
@@ -117,8 +119,10 @@ struct fpmhip_server {
   uint32_t prepared_nmax = 0;
 
   void run();
-  bool admit(Job& j);
-  // Admits jobs (admit), marks them resident or failed, and empties the list.
+  bool admit(Job& j, fpmhip_context& c);
+  // Admits one job on `c` (admit) and marks it resident or failed.
+  void admit_one(Job& j, fpmhip_context& c);
+  // Admits jobs on up to kAdmitThreads threads and empties the list.
   void admit_all(std::vector<Job*>& to_admit);
   void finish(Job& j);
   // Finishes jobs (finish) and marks them done or failed; with `held`, their
@@ -154,10 +158,10 @@ double sweep_work(const Job& j) {
 
 }  // namespace
 
-bool fpmhip_server::admit(Job& j) {
-  int rc = fpmhip_model_create(&stage, &j.desc, &j.model);
+bool fpmhip_server::admit(Job& j, fpmhip_context& c) {
+  int rc = fpmhip_model_create(&c, &j.desc, &j.model);
   if (!rc)
-    rc = fpmhip_measurements_create(&stage, j.model, j.counts.empty() ? nullptr : j.counts.data(),
+    rc = fpmhip_measurements_create(&c, j.model, j.counts.empty() ? nullptr : j.counts.data(),
                                     j.frames.empty() ? nullptr : j.frames.data(),
                                     j.weights.empty() ? nullptr : j.weights.data(), nullptr, &j.meas);
   if (!rc) rc = fpmhip_session_create(j.model, j.meas, FPMHIP_MIXED_F32, &j.alg, &j.sess);
@@ -174,14 +178,15 @@ bool fpmhip_server::admit(Job& j) {
     return false;
   }
   j.scratch.allocate(16 + (j.iterations + j.sess->model->F) * sizeof(double));
-  HIP_CHECK(hipMemsetAsync(j.scratch.ptr, 0, j.scratch.bytes, stage.stream));
-  HIP_CHECK(hipStreamSynchronize(stage.stream));
+  HIP_CHECK(hipMemsetAsync(j.scratch.ptr, 0, j.scratch.bytes, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
   // the counter starts from the session's (zero after initialisation)
   return true;
 }
 
 void fpmhip_server::finish(Job& j) {
   fpmhip_session& s = *j.sess;
+  s.ctx = &drain;  // the result transform and downloads run on the drain stream
   const uint64_t M = s.model->M();
   j.object.resize(2 * M);
   j.spectrum.resize(2 * M);
@@ -190,10 +195,10 @@ void fpmhip_server::finish(Job& j) {
   if (!rc) rc = fpmhip_session_get_object(&s, j.object.data());
   if (!rc) {
     HIP_CHECK(hipMemcpyAsync(j.trace.data(), j.scratch.as<char>() + 16, j.iterations * sizeof(double),
-                             hipMemcpyDeviceToHost, stage.stream));
+                             hipMemcpyDeviceToHost, drain.stream));
     int32_t status = 0;
-    HIP_CHECK(hipMemcpyAsync(&status, j.scratch.as<char>() + 12, sizeof(status), hipMemcpyDeviceToHost, stage.stream));
-    HIP_CHECK(hipStreamSynchronize(stage.stream));
+    HIP_CHECK(hipMemcpyAsync(&status, j.scratch.as<char>() + 12, sizeof(status), hipMemcpyDeviceToHost, drain.stream));
+    HIP_CHECK(hipStreamSynchronize(drain.stream));
     if (status == 2) {
       rc = FPMHIP_DEVICE_ERROR;
       j.error = "a cooperative tile group stalled at a group barrier";
@@ -214,6 +219,9 @@ void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) 
     } catch (const Failure& f) {
       j->error_code = f.code;
       j->error = fpmhip_last_error();
+    } catch (...) {
+      j->error_code = FPMHIP_DEVICE_ERROR;
+      j->error = "result download failed";
     }
     if (held) {
       held->emplace_back();
@@ -236,31 +244,47 @@ void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) 
   done.clear();
 }
 
-void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
-  for (Job* j : to_admit) {
-    bool ok = false;
-    try {
-      ok = admit(*j);
-    } catch (const Failure& f) {
-      j->error_code = f.code;
-      j->error = fpmhip_last_error();
-      j->release_device();
-    }
-    std::lock_guard<std::mutex> lock(mu);
-    if (ok) {
-      j->state = kResident;
-      j->t_admit = now_s();
-      resident.push_back(j);
-      resident_bytes += j->bytes;
-      resident_wgs += j->wgs;
-      stats.admitted++;
-    } else {
-      j->state = kFailed;
-      j->t_done = now_s();
-      stats.failed++;
-      cv_done.notify_all();
-    }
+void fpmhip_server::admit_one(Job& j, fpmhip_context& c) {
+  bool ok = false;
+  try {
+    ok = admit(j, c);
+  } catch (const Failure& f) {
+    j.error_code = f.code;
+    j.error = fpmhip_last_error();
+    j.release_device();
+  } catch (...) {
+    j.error_code = FPMHIP_DEVICE_ERROR;
+    j.error = "admission failed";
+    j.release_device();
   }
+  std::lock_guard<std::mutex> lock(mu);
+  if (ok) {
+    j.state = kResident;
+    j.t_admit = now_s();
+    resident.push_back(&j);
+    resident_bytes += j.bytes;
+    resident_wgs += j.wgs;
+    stats.admitted++;
+  } else {
+    j.state = kFailed;
+    j.t_done = now_s();
+    stats.failed++;
+    cv_done.notify_all();
+  }
+}
+
+void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
+  // jobs are independent: thread k admits jobs k, k + t, ... on stage[k]
+  const size_t n = to_admit.size();
+  const size_t t = std::min<size_t>(n, kAdmitThreads);
+  auto work = [&](size_t k) {
+    (void)hipSetDevice(ctx->device);
+    for (size_t i = k; i < n; i += t) admit_one(*to_admit[i], stage[k]);
+  };
+  std::vector<std::thread> pool;
+  for (size_t k = 1; k < t; ++k) pool.emplace_back(work, k);
+  if (n) work(0);
+  for (std::thread& th : pool) th.join();
   to_admit.clear();
 }
 
@@ -360,8 +384,19 @@ void fpmhip_server::run() {
       // while the tick runs: the results of the jobs finished by the last
       // tick are downloaded, and the queued jobs are admitted (uploads and
       // initialisation), on the second stream
-      finish_all(finished, &held);
-      admit_all(to_admit);
+      std::thread drainer;
+      if (!finished.empty())
+        drainer = std::thread([&] {
+          (void)hipSetDevice(ctx->device);
+          finish_all(finished, &held);
+        });
+      try {
+        admit_all(to_admit);
+      } catch (...) {
+        if (drainer.joinable()) drainer.join();
+        throw;
+      }
+      if (drainer.joinable()) drainer.join();
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
       release_held();
     } catch (const Failure& f) {
@@ -415,9 +450,13 @@ int fpmhip_server_create(fpmhip_context* c, const fpmhip_server_config* config, 
     s->own.device = c->device;
     s->own.props = c->props;
     HIP_CHECK(hipStreamCreateWithFlags(&s->own.stream, hipStreamNonBlocking));
-    s->stage.device = c->device;
-    s->stage.props = c->props;
-    HIP_CHECK(hipStreamCreateWithFlags(&s->stage.stream, hipStreamNonBlocking));
+    std::vector<fpmhip_context*> aux = {&s->drain};
+    for (fpmhip_context& x : s->stage) aux.push_back(&x);
+    for (fpmhip_context* x : aux) {
+      x->device = c->device;
+      x->props = c->props;
+      HIP_CHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+    }
     s->ctx = &s->own;
     s->cfg = *config;
     s->wg_limit = std::max<uint32_t>(sweep_coresident_limit(kFusedMaxN), kGroupMembers);
@@ -439,10 +478,11 @@ int fpmhip_server_destroy(fpmhip_server* s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     for (auto& kv : s->jobs) kv.second->release_device();
-    hipStream_t st = s->own.stream, st2 = s->stage.stream;
+    std::vector<hipStream_t> streams = {s->own.stream, s->drain.stream};
+    for (const fpmhip_context& x : s->stage) streams.push_back(x.stream);
     delete s;
-    if (st) (void)hipStreamDestroy(st);
-    if (st2) (void)hipStreamDestroy(st2);
+    for (hipStream_t st : streams)
+      if (st) (void)hipStreamDestroy(st);
   });
 }
 
