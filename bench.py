@@ -299,7 +299,7 @@ def main():
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--sustained-rate", type=float, default=40.0,
-                    help="second serving leg below the GPU's capacity (the first leg's rate overloads it)")
+                    help="second serving leg below the GPU's capacity (the first leg's rate overloads it; 0: none)")
     ap.add_argument("--no-extras", action="store_true", help="batched mode without the plain and serving legs")
     ap.add_argument("--tiles", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -313,7 +313,7 @@ def main():
         sys.exit(2)
     if args.mode == "serving":
         res = serving(args.requests, args.rate, 30)
-        sus = serving(args.requests, args.sustained_rate, 30)
+        sus = serving(args.requests, args.sustained_rate, 30) if args.sustained_rate > 0 else None
         print(json.dumps({"metric": "serving p99 latency, FP32 mPIE 30 sweeps", "value": res["p99_ms"],
                           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
                           "config": {"workload": "Poisson arrivals", **res}, "sustained": sus}), flush=True)
