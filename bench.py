@@ -11,6 +11,7 @@ interval 30, friction/feedback 0.5 (fpm-rs's 0.9/0.9 diverges on 225 LEDs).
 Prints one JSON line (rank 0). Only the cpu_baseline leg uses the oracle.
 """
 import argparse
+import copy
 import json
 import os
 import sys
@@ -61,19 +62,66 @@ def workload(n=128, k=15, seed=7, peak=1e4):
     return model, MeasurementStack(counts=counts)
 
 
+def workload_tiles(count, n=128, k=15, seed=7, peak=1e4):
+    """`count` distinct FOV tiles of the workload: one compiled geometry, and
+    per tile its own object (seed + i), its own Poisson counts and its own
+    frame gains (photons per pixel of its own peak)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    base = runtime.compile_model(syn.base_optics(), syn.planar_array(k), (n, n))
+    ctx = runtime.default_context()
+    dm = runtime.DeviceModel(ctx, base)
+
+    def counts_of(args):
+        i, intensity = args
+        counts, ppp = syn.poisson_counts(intensity, peak, seed + i)
+        model = copy.deepcopy(base)
+        model.with_frame_gains(np.full(model.frame_count, ppp))
+        return model, MeasurementStack(counts=counts)
+
+    try:
+        # numpy's generators release the GIL: objects and counts are drawn on
+        # host threads, the ideal intensities come from the device forward stack
+        with ThreadPoolExecutor(host_threads()) as pool:
+            specs = pool.map(lambda i: syn.object_spectrum(syn.random_field_object(base.reconstruction_shape, seed + i)),
+                             range(count))
+            stacks = ((i, runtime.forward_stack(base, sp, context=ctx, dmodel=dm)) for i, sp in enumerate(specs))
+            return list(pool.map(counts_of, stacks))
+    finally:
+        dm.close()
+
+
 def algorithm(iterations):
     return Mpie(iterations=iterations, momentum_friction=0.5, momentum_feedback=0.5)
 
 
-def tile_bytes(model, iterations, interval, s_m, s_o, s_p):
-    """Algorithmic HBM bytes of one tile (per-LED window traffic, momentum
-    events, init read and the R^2 init/result passes)."""
+def tile_bytes(model, iterations, interval, s_m, s_o, s_p, s_v):
+    """Algorithmic HBM bytes of one tile: per LED the frame, the object
+    window read and written and the pupil; per momentum event the object
+    read and written (s_o each) and the velocity and anchor read and written
+    at their stored size s_v; the init read of every frame and the R^2
+    init/result passes."""
     N = model.image_len
     M = model.reconstruction_shape[0] * model.reconstruction_shape[1]
     F = model.frame_count
     b_led = N * (s_m + 2 * s_o + s_p)
     events = (iterations * F) // interval
-    return iterations * F * b_led + events * 6 * M * s_o + F * N * s_m + 4 * M * s_o
+    return iterations * F * b_led + events * M * (2 * s_o + 4 * s_v) + F * N * s_m + 4 * M * s_o
+
+
+def sweep_bytes(model, iterations, interval, s_m, s_o, s_p, s_v):
+    """tile_bytes of one sweep (the per-LED and momentum terms only)."""
+    M = model.reconstruction_shape[0] * model.reconstruction_shape[1]
+    fixed = model.frame_count * model.image_len * s_m + 4 * M * s_o
+    return (tile_bytes(model, iterations, interval, s_m, s_o, s_p, s_v) - fixed) / iterations
+
+
+def close_all(*objs):
+    """Destroys device handles in the order given (dependents first)."""
+    for o in objs:
+        for x in (o if isinstance(o, (list, tuple)) else [o]):
+            if x is not None:
+                x.close()
 
 
 def run_plain(model, meas, iterations, steps, warmup, precision):
@@ -85,47 +133,58 @@ def run_plain(model, meas, iterations, steps, warmup, precision):
 
     def one():
         s = runtime.Session(prob, alg, precision=precision, context=ctx, dmodel=dm, dmeas=dme)
-        s.initialize()
-        s.run(iterations)
-        return s.object()
+        try:
+            s.initialize()
+            s.run(iterations)
+            return s.object()
+        finally:
+            s.close()
 
-    for i in range(warmup):
-        one()
-        log(f"warmup tile {i + 1}/{warmup} done")
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        one()
-        log(f"tile {i + 1}/{steps} done")
-    ctx.synchronize()
-    return time.perf_counter() - t0
+    try:
+        for i in range(warmup):
+            one()
+            log(f"warmup tile {i + 1}/{warmup} done")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            one()
+            log(f"tile {i + 1}/{steps} done")
+        ctx.synchronize()
+        return time.perf_counter() - t0
+    finally:
+        close_all(dme, dm)
 
 
-def run_batched(model, meas, iterations, steps, warmup, tiles):
-    """`tiles` copies of the workload tile (same model and counts, separate
-    state) advanced together; one step = one batch of tiles."""
+def run_batched(tiles, iterations, steps, warmup):
+    """The workload tiles (each its own model, counts and state) advanced
+    together; one step = one batch of tiles."""
     ctx = runtime.default_context()
-    dm = runtime.DeviceModel(ctx, model)
-    dme = runtime.DeviceMeasurements(ctx, dm, meas)
-    prob = ReconstructionProblem(meas, model)
     alg = algorithm(iterations)
-    sessions = [runtime.Session(prob, alg, precision="f32", context=ctx, dmodel=dm, dmeas=dme) for _ in range(tiles)]
-    batch = runtime.Batch(sessions, context=ctx)
-    for i in range(warmup):
-        batch.run(iterations)
-        log(f"warmup batch {i + 1}/{warmup} done")
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    sweep_ms = []
-    for i in range(steps):
-        batch.run(iterations)
-        batch.objects_device()  # every tile's result object, complete in HBM
-        sweep_ms.append(batch.timing()[0])
-        log(f"batch {i + 1}/{steps} done")
-    ctx.synchronize()
-    secs = time.perf_counter() - t0
-    per_launch = batch.schedule()["sweeps_per_launch"] or 10
-    return secs, float(np.mean(sweep_ms)), min(per_launch, iterations), download_ms(batch)
+    dms, dmes, sessions, batch = [], [], [], None
+    try:
+        for model, meas in tiles:
+            dms.append(runtime.DeviceModel(ctx, model))
+            dmes.append(runtime.DeviceMeasurements(ctx, dms[-1], meas))
+            sessions.append(runtime.Session(ReconstructionProblem(meas, model), alg, precision="f32", context=ctx,
+                                            dmodel=dms[-1], dmeas=dmes[-1]))
+        batch = runtime.Batch(sessions, context=ctx)
+        for i in range(warmup):
+            batch.run(iterations)
+            log(f"warmup batch {i + 1}/{warmup} done")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sweep_ms = []
+        for i in range(steps):
+            batch.run(iterations)
+            batch.objects_device()  # every tile's result object, complete in HBM
+            sweep_ms.append(batch.timing()[0])
+            log(f"batch {i + 1}/{steps} done")
+        ctx.synchronize()
+        secs = time.perf_counter() - t0
+        per_launch = batch.schedule()["sweeps_per_launch"] or 10
+        return secs, float(np.mean(sweep_ms)), min(per_launch, iterations), download_ms(batch)
+    finally:
+        close_all(batch, sessions, dmes, dms)
 
 
 def download_ms(batch):
@@ -173,30 +232,31 @@ def serving(requests, rate, iterations, seed=5, max_tiles=256):
         reqs.append((t, [problem(n, k, wavelengths[c]) for c in sorted(ch)]))
     alg = Mpie(iterations=iterations, momentum_friction=0.5, momentum_feedback=0.5)
     server = runtime.Server(max_tiles=max_tiles)
-    # one warm-up job so that first-use costs stay out of the measurement
-    w = server.submit(reqs[0][1][0], alg)
-    server.wait(w, 120)
-    server.result(w)
-    log(f"serving: {requests} requests, {sum(len(r[1]) for r in reqs)} tiles, rate {rate}/s")
-    ids = []
-    t0 = time.perf_counter()
-    for (ta, probs) in reqs:
-        delay = t0 + ta - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        ids.append([server.submit(p, alg) for p in probs])
-    lat = []
-    last = 0.0
-    for group in ids:
-        worst = 0.0
-        for jid in group:
-            server.wait(jid, 600)
-            _, _, _, l, _ = server.result(jid)
-            worst = max(worst, l)
-        lat.append(worst)
-    stats = server.stats()
-    wall = time.perf_counter() - t0
-    server.close()
+    try:
+        # one warm-up job so that first-use costs stay out of the measurement
+        w = server.submit(reqs[0][1][0], alg)
+        server.wait(w, 120)
+        server.result(w)
+        log(f"serving: {requests} requests, {sum(len(r[1]) for r in reqs)} tiles, rate {rate}/s")
+        ids = []
+        t0 = time.perf_counter()
+        for (ta, probs) in reqs:
+            delay = t0 + ta - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append([server.submit(p, alg) for p in probs])
+        lat = []
+        for group in ids:
+            worst = 0.0
+            for jid in group:
+                server.wait(jid, 600)
+                _, _, _, l, _ = server.result(jid)
+                worst = max(worst, l)
+            lat.append(worst)
+        stats = server.stats()
+        wall = time.perf_counter() - t0
+    finally:
+        server.close()
     tiles = sum(len(g) for g in ids)
     lat = np.array(lat) * 1000.0
     return {"requests": requests, "tiles": tiles, "rate_per_s": rate, "iterations": iterations,
@@ -205,23 +265,33 @@ def serving(requests, rate, iterations, seed=5, max_tiles=256):
             "sizes": "64^2,128^2,256^2", "channels": "1-3 of 450/532/630 nm", "leds": "9x9..15x15"}
 
 
-def uneven_tiles(seed=11):
+def uneven_tiles(seed=11, only=None):
     """96 FOV tiles of uneven size: 32 x 64^2, 48 x 128^2, 16 x 256^2 images
     (-> 168^2, 336^2, 672^2 'smooth'), 13x13 LEDs, each tile with its own
     object (seed) and its own Zernike Z4-Z15 pupil with RMS ~ U(0, 0.8] rad
     (simulated with it and reconstructed with it); uint16 Poisson counts at
-    a 4e3 photon peak."""
+    a 4e3 photon peak. Tiles 0-15 are 256^2, 16-63 128^2, 64-95 64^2.
+    `only`: indices of the tiles to build (the others' random draws are
+    still taken, so a tile is the same whichever subset is built)."""
     rng = np.random.default_rng(seed)
     out = []
+    compiled = {}
+    index = 0
     for n, count in ((256, 16), (128, 48), (64, 32)):
         for _ in range(count):
-            model = runtime.compile_model(syn.base_optics(), syn.planar_array(13), (n, n))
-            model.pupil = syn.zernike_pupil(model, float(rng.uniform(1e-3, 0.8)), int(rng.integers(1 << 30)))
-            obj = syn.random_field_object(model.reconstruction_shape, int(rng.integers(1 << 30)))
-            I = runtime.forward_stack(model, syn.object_spectrum(obj))
-            counts, ppp = syn.poisson_counts(I, 4e3, int(rng.integers(1 << 30)))
-            model.with_frame_gains(np.full(model.frame_count, ppp))
-            out.append(ReconstructionProblem(MeasurementStack(counts=counts), model))
+            rms = float(rng.uniform(1e-3, 0.8))
+            pupil_seed, obj_seed, count_seed = (int(rng.integers(1 << 30)) for _ in range(3))
+            if only is None or index in only:
+                if n not in compiled:
+                    compiled[n] = runtime.compile_model(syn.base_optics(), syn.planar_array(13), (n, n))
+                model = copy.deepcopy(compiled[n])
+                model.pupil = syn.zernike_pupil(model, rms, pupil_seed)
+                obj = syn.random_field_object(model.reconstruction_shape, obj_seed)
+                I = runtime.forward_stack(model, syn.object_spectrum(obj))
+                counts, ppp = syn.poisson_counts(I, 4e3, count_seed)
+                model.with_frame_gains(np.full(model.frame_count, ppp))
+                out.append(ReconstructionProblem(MeasurementStack(counts=counts), model))
+            index += 1
     return out
 
 
@@ -233,25 +303,27 @@ def run_uneven(iterations=40, steps=1, warmup=1):
     probs = uneven_tiles()
     ctx = runtime.default_context()
     alg = algorithm(iterations)
-    sessions = [runtime.Session(p, alg, precision="f32", context=ctx) for p in probs]
-    batch = runtime.Batch(sessions, context=ctx)
-    for i in range(warmup):
-        batch.run(iterations)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    sweep_ms = []
-    for i in range(steps):
-        batch.run(iterations)
-        batch.objects_device()  # every tile's result object, complete in HBM
-        sweep_ms.append(batch.timing()[0])
-        log(f"uneven batch {i + 1}/{steps} done")
-    ctx.synchronize()
-    secs = time.perf_counter() - t0
-    dl = download_ms(batch)
-    total = sum(tile_bytes(p.model, iterations, 30, 2, 16, 8) for p in probs)
-    per_sweep = sum((tile_bytes(p.model, iterations, 30, 2, 16, 8) - p.model.frame_count * p.model.image_len * 2
-                     - 4 * p.model.reconstruction_shape[0] * p.model.reconstruction_shape[1] * 16) / iterations
-                    for p in probs)
+    sessions, batch = [], None
+    try:
+        sessions = [runtime.Session(p, alg, precision="f32", context=ctx) for p in probs]
+        batch = runtime.Batch(sessions, context=ctx)
+        for i in range(warmup):
+            batch.run(iterations)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sweep_ms = []
+        for i in range(steps):
+            batch.run(iterations)
+            batch.objects_device()  # every tile's result object, complete in HBM
+            sweep_ms.append(batch.timing()[0])
+            log(f"uneven batch {i + 1}/{steps} done")
+        ctx.synchronize()
+        secs = time.perf_counter() - t0
+        dl = download_ms(batch)
+    finally:
+        close_all(batch, sessions, [s.dmeas for s in sessions], [s.dmodel for s in sessions])
+    total = sum(tile_bytes(p.model, iterations, 30, 2, 16, 8, 8) for p in probs)
+    per_sweep = sum(sweep_bytes(p.model, iterations, 30, 2, 16, 8, 8) for p in probs)
     ms = float(np.mean(sweep_ms))
     return {"tiles": len(probs), "sizes": "32x64^2, 48x128^2, 16x256^2", "leds": "13x13", "iterations": iterations,
             "tiles_per_s": steps * len(probs) / secs, "ms_per_batch": 1000.0 * secs / steps,
@@ -261,20 +333,49 @@ def run_uneven(iterations=40, steps=1, warmup=1):
             "batch_algorithmic_GB": total / 1e9}
 
 
-def cpu_baseline(model, meas, iterations, sample_iterations, threads):
+def host_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    at most 16 (the host share that comes with one GPU of the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def fft_speed_vs_pocketfft(n=128, reps=200):
+    """Time of the oracle's 2-D FFT (this project's Stockham code, which the
+    CPU baseline uses) over numpy's pocketfft (single thread) on one n x n
+    complex128 forward transform: > 1 means the oracle's FFT is slower."""
+    import oracle
+
+    x = (np.random.default_rng(0).standard_normal((n, n)) + 1j * np.random.default_rng(1).standard_normal((n, n)))
+    oracle.fft2(x)
+    np.fft.fft2(x)
+    t = time.perf_counter()
+    for _ in range(reps):
+        oracle.fft2(x)
+    t_or = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        np.fft.fft2(x) / x.size
+    t_np = (time.perf_counter() - t) / reps
+    return {"n": n, "oracle_us": 1e6 * t_or, "pocketfft_us": 1e6 * t_np, "ratio": t_or / t_np}
+
+
+def cpu_baseline(tiles, iterations, threads):
     """The oracle (scalar C++ FP64 restatement of the reference's serial
-    per-tile path) on `threads` host threads, one tile per thread, each
-    running `sample_iterations` of the tile's sweeps; scaled to full tiles."""
+    per-tile path) on `threads` host threads, one whole tile (all
+    `iterations` sweeps) per thread, tiles distinct."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    alg = algorithm(sample_iterations).to_c()
-    times = [0.0] * threads
+    isa = oracle.use_widest_isa()
+    alg = algorithm(iterations).to_c()
 
     def work(i):
-        t = time.perf_counter()
-        oracle.run(model, meas, alg, sample_iterations)
-        times[i] = time.perf_counter() - t
+        model, meas = tiles[i % len(tiles)]
+        oracle.run(model, meas, alg, iterations)
 
     th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
     t0 = time.perf_counter()
@@ -283,10 +384,11 @@ def cpu_baseline(model, meas, iterations, sample_iterations, threads):
     for t in th:
         t.join()
     wall = time.perf_counter() - t0
-    per_tile = wall * iterations / sample_iterations
-    return {"value": threads / per_tile, "unit": "tiles/s", "cores": threads, "kind": "port",
-            "sample": f"{threads} tiles x {sample_iterations} of {iterations} mPIE sweeps on {threads} threads "
-                      f"({wall:.1f} s wall), scaled linearly to full tiles"}
+    fft = fft_speed_vs_pocketfft()
+    return {"value": threads / wall, "unit": "tiles/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} distinct workload tiles, all {iterations} mPIE sweeps each, one tile per thread on "
+                      f"{threads} threads ({wall:.1f} s wall); oracle built -O3 -march=x86-64-{isa}",
+            "fft_vs_pocketfft": fft}
 
 
 def main():
@@ -303,7 +405,6 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="batched mode without the plain and serving legs")
     ap.add_argument("--tiles", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample-iterations", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -325,27 +426,29 @@ def main():
                           "config": {"workload": "96 uneven tiles", **res}}), flush=True)
         return
     log("building workload")
-    model, meas = workload()
-    log("workload ready", model.reconstruction_shape)
     precision = "c128" if args.mode == "plain" else "f32"
+    ntiles = args.tiles if args.mode == "batched" else 1
+    t_build = time.perf_counter()
+    tiles = workload_tiles(max(ntiles, 1 if args.no_cpu else host_threads()))
+    model, meas = tiles[0]
+    log(f"workload ready: {len(tiles)} tiles {model.reconstruction_shape} in {time.perf_counter() - t_build:.1f} s")
     s_o = 16
     s_p = 16 if precision == "c128" else 8
-    B = tile_bytes(model, args.iterations, 30, 2, s_o, s_p)
+    s_v = 16 if precision == "c128" else 8  # velocity / anchor storage
+    B = tile_bytes(model, args.iterations, 30, 2, s_o, s_p, s_v)
     traffic = None
     if args.mode == "batched":
-        secs, sweep_ms, per_launch, dl = run_batched(model, meas, args.iterations, args.steps, args.warmup,
-                                                     args.tiles)
+        secs, sweep_ms, per_launch, dl = run_batched(tiles[:ntiles], args.iterations, args.steps, args.warmup)
         if args.tiles == 256:
             traffic = measured_traffic()
         tiles_per_s = args.steps * args.tiles / secs
         # dominant kernel: the fused sweep; one dispatch runs `per_launch`
         # sweeps of every tile, and bytes and time are both per sweep here
-        sweep_bytes = (B - model.frame_count * model.image_len * 2
-                       - 4 * model.reconstruction_shape[0] * model.reconstruction_shape[1] * s_o) / args.iterations
-        achieved = sweep_bytes * args.tiles / (sweep_ms * 1e-3) / 1e9
+        achieved = sweep_bytes(model, args.iterations, 30, 2, s_o, s_p, s_v) * args.tiles / (sweep_ms * 1e-3) / 1e9
         note = (f"fused sweep kernel k_sweep, {per_launch} sweeps of every tile per dispatch: algorithmic bytes "
-                "of one sweep of every tile / mean per-sweep time (HIP events around each dispatch / its sweeps); "
-                "traffic is per sweep too")
+                "of one sweep of every tile (object window c128 read+write, pupil c64, uint16 frame per LED; "
+                "object c128 and velocity/anchor c64 read+write per momentum event) / mean per-sweep time "
+                "(HIP events around each dispatch / its sweeps); traffic is counter-measured HBM bytes per sweep")
     else:
         secs = run_plain(model, meas, args.iterations, args.steps, args.warmup, precision)
         tiles_per_s = args.steps / secs
@@ -364,11 +467,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if precision == "c128" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "225 LEDs 128^2 -> 375^2 mPIE 50 sweeps, uint16 Poisson 1e4", "path": args.mode,
-                   "tiles_per_batch": args.tiles if args.mode == "batched" else 1},
+        "config": {"workload": "225 LEDs 128^2 -> 375^2 mPIE 50 sweeps, uint16 Poisson 1e4, distinct tiles",
+                   "path": args.mode, "tiles_per_batch": ntiles},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "note": note},
     }
+    if traffic:
+        line["roofline"]["traffic_GBs"] = traffic / (sweep_ms * 1e-3) / 1e9
+        line["roofline"]["traffic_frac"] = line["roofline"]["traffic_GBs"] / HBM_PEAK_GBS
     if args.mode == "batched":
         # results left in HBM count in `value`; the same batch with its
         # results downloaded to host memory over PCIe:
@@ -383,15 +489,19 @@ def main():
         line["uneven_batch"] = run_uneven(40, 1, 1)
         log("serving leg")
         line["serving"] = serving(args.requests, args.rate, 30)
-        log("serving leg at a sustainable rate")
-        line["serving_sustained"] = serving(args.requests, args.sustained_rate, 30)
+        if args.sustained_rate > 0:
+            log("serving leg at a sustainable rate")
+            line["serving_sustained"] = serving(args.requests, args.sustained_rate, 30)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_threads()
         log(f"cpu baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(model, meas, args.iterations, args.cpu_sample_iterations, threads)
+        line["cpu_baseline"] = cpu_baseline(tiles, args.iterations, threads)
         line["speedup_vs_cpu"] = tiles_per_s / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    finally:
+        runtime.close_all()

@@ -8,6 +8,7 @@
 #include <numeric>
 
 #include "fused.h"
+#include "gauge.h"
 #include "session.h"
 
 using namespace fpmhip;
@@ -41,128 +42,6 @@ void set_device(const Context* c) { HIP_CHECK(hipSetDevice(c->device)); }
 
 inline bool off_zero(double r, double c) { return std::fabs(r) <= 1e-12 && std::fabs(c) <= 1e-12; }
 
-// ---------------------------------------------------------------------------
-// host FP64 restatement of canonicalize_object_pupil (src/algorithms/gauge.rs:20-282)
-// over downloaded state; complex values interleaved.
-struct Cd {
-  double re, im;
-};
-inline Cd cm(Cd a, Cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-inline Cd cj(Cd a) { return {a.re, -a.im}; }
-inline double cabs(Cd a) { return std::hypot(a.re, a.im); }
-inline double carg(Cd a) { return std::atan2(a.im, a.re); }
-inline Cd cpol(double r, double t) { return {r * std::cos(t), r * std::sin(t)}; }
-
-double phase_slope(const Model& m, const Cd* P, const Cd* Pref, bool row, double tol) {
-  const uint64_t h = m.nh, w = m.nw;
-  Cd res{0, 0}, strongest{0, 0};
-  double total = 0.0, sw = 0.0;
-  const uint64_t rl = row ? h - 1 : h, cl = row ? w : w - 1;
-  for (uint64_t r = 0; r < rl; ++r)
-    for (uint64_t c = 0; c < cl; ++c) {
-      const uint64_t cur = r * w + c, nxt = row ? (r + 1) * w + c : cur + 1;
-      if (!m.support[cur] || !m.support[nxt]) continue;
-      const Cd contribution = cm(cm(P[nxt], cj(P[cur])), cj(cm(Pref[nxt], cj(Pref[cur]))));
-      const double weight = cabs(contribution);
-      if (!std::isfinite(weight)) continue;
-      res = {res.re + contribution.re, res.im + contribution.im};
-      total += weight;
-      if (weight > sw) {
-        strongest = contribution;
-        sw = weight;
-      }
-    }
-  if (total == 0.0) return 0.0;
-  if (cabs(res) > tol * total) return carg(res);
-  return sw > 0.0 ? carg(strongest) : 0.0;
-}
-
-void canonicalize_host(const Model& m, std::vector<Cd>& spec, std::vector<Cd>& P) {
-  const Cd* Pref = reinterpret_cast<const Cd*>(m.pupil.data());
-  if (m.support_count == 0) fail(FPMHIP_INVALID_MODEL, "compiled reference pupil support is empty");
-  const double tol = 64.0 * std::numeric_limits<double>::epsilon() * (double)m.support_count;
-  bool row_exact = true, col_exact = true;
-  for (uint64_t s = 0; s < m.S; ++s) {
-    row_exact = row_exact && std::fabs(m.off_r[s]) <= 1e-12;
-    col_exact = col_exact && std::fabs(m.off_c[s]) <= 1e-12;
-  }
-  const double rs = row_exact ? phase_slope(m, P.data(), Pref, true, tol) : 0.0;
-  const double cs = col_exact ? phase_slope(m, P.data(), Pref, false, tol) : 0.0;
-  if (std::fabs(rs) > tol || std::fabs(cs) > tol) {
-    for (uint64_t r = 0; r < m.nh; ++r)
-      for (uint64_t c = 0; c < m.nw; ++c) {
-        const double y = (double)r - (double)(m.nh / 2), x = (double)c - (double)(m.nw / 2);
-        P[r * m.nw + c] = cm(P[r * m.nw + c], cpol(1.0, -(rs * y + cs * x)));
-      }
-    for (uint64_t r = 0; r < m.rh; ++r)
-      for (uint64_t c = 0; c < m.rw; ++c) {
-        const double y = (double)r - (double)(m.rh / 2), x = (double)c - (double)(m.rw / 2);
-        spec[r * m.rw + c] = cm(spec[r * m.rw + c], cpol(1.0, rs * y + cs * x));
-      }
-  }
-  double eref = 0.0, ecur = 0.0;
-  for (uint64_t p = 0; p < m.N(); ++p)
-    if (m.support[p]) {
-      eref += Pref[p].re * Pref[p].re + Pref[p].im * Pref[p].im;
-      ecur += P[p].re * P[p].re + P[p].im * P[p].im;
-    }
-  if (!std::isfinite(eref) || eref <= 0.0)
-    fail(FPMHIP_INVALID_MODEL, "compiled reference pupil has invalid supported energy");
-  if (!std::isfinite(ecur) || ecur <= 0.0)
-    fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization requires positive finite recovered pupil energy");
-  const double computed = std::sqrt(eref / ecur);
-  if (!std::isfinite(computed) || computed <= 0.0)
-    fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization produced an invalid pupil scale");
-  const double scale = std::fabs(computed - 1.0) <= tol ? 1.0 : computed;
-  Cd ov{0, 0}, st{0, 0};
-  double ow = 0.0, sw = 0.0;
-  for (uint64_t p = 0; p < m.N(); ++p) {
-    if (!m.support[p]) continue;
-    const Cd contribution = cm(cj(Pref[p]), P[p]);
-    const double weight = cabs(contribution);
-    ov = {ov.re + contribution.re, ov.im + contribution.im};
-    ow += weight;
-    if (weight > sw) {
-      st = contribution;
-      sw = weight;
-    }
-  }
-  double phase;
-  if (ow > 0.0 && cabs(ov) > tol * ow)
-    phase = carg(ov);
-  else if (sw > 0.0)
-    phase = carg(st);
-  else
-    fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization could not anchor recovered pupil phase");
-  if (!std::isfinite(phase)) fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization produced a non-finite pupil phase");
-  if (std::fabs(phase) <= tol) phase = 0.0;
-  const Cd pf = cpol(scale, -phase);
-  const double n2 = pf.re * pf.re + pf.im * pf.im;
-  const Cd of{(1.0 * pf.re + 0.0 * pf.im) / n2, (0.0 * pf.re - 1.0 * pf.im) / n2};
-  if (!(pf.re == 1.0 && pf.im == 0.0)) {
-    for (auto& v : P) v = cm(v, pf);
-    for (auto& v : spec) v = cm(v, of);
-  }
-  Cd strongest{0, 0};
-  double smag = 0.0;
-  for (const Cd& v : spec) {
-    if (!std::isfinite(v.re) || !std::isfinite(v.im))
-      fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization requires a finite object spectrum");
-    const double mag = cabs(v);
-    if (mag > smag) {
-      strongest = v;
-      smag = mag;
-    }
-  }
-  if (smag == 0.0) return;
-  const Cd dc = spec[(m.rh / 2) * m.rw + m.rw / 2];
-  const double oph = carg(cabs(dc) > tol * smag ? dc : strongest);
-  if (!std::isfinite(oph)) fail(FPMHIP_NUMERICAL, "object/pupil gauge normalization produced a non-finite object phase");
-  if (std::fabs(oph) <= tol) return;
-  const Cd corr = cpol(1.0, -oph);
-  for (auto& v : spec) v = cm(v, corr);
-}
-
 // ---- session helpers ----------------------------------------------------------------
 
 void download_pupil64(fpmhip_session& s, double* out) {
@@ -192,15 +71,40 @@ void refresh_max_pupil(fpmhip_session& s) {
   launch_max_abs2(s, s.b.pupil.ptr, s.precision == FPMHIP_PLAIN_C128, s.model->N(), 0);
 }
 
+// canonicalize_state (runner.rs:122, 220) of an EPRY session: the device
+// gauge (canonicalize_object_pupil, gauge.rs:20-296) on the session stream,
+// which also refreshes max |P|^2; its failure code lands in flag[2].
 void canonicalize_session(fpmhip_session& s) {
+  if (s.model->support_count == 0) fail(FPMHIP_INVALID_MODEL, "compiled reference pupil support is empty");
+  launch_gauge(s.b.gauge.as<GaugeTile>(), 1, s.ctx->stream);
+}
+
+// The gauge descriptor of a session that canonicalizes.
+void prepare_gauge(fpmhip_session& s) {
   const Model& m = *s.model;
-  std::vector<Cd> spec(m.M()), P(m.N());
-  HIP_CHECK(hipMemcpyAsync(spec.data(), s.b.spec.ptr, m.M() * sizeof(double2), hipMemcpyDeviceToHost, s.ctx->stream));
-  download_pupil64(s, reinterpret_cast<double*>(P.data()));
-  canonicalize_host(m, spec, P);
-  HIP_CHECK(hipMemcpyAsync(s.b.spec.ptr, spec.data(), m.M() * sizeof(double2), hipMemcpyHostToDevice, s.ctx->stream));
-  upload_pupil64(s, reinterpret_cast<const double*>(P.data()));
-  refresh_max_pupil(s);
+  GaugeTile t{};
+  t.spec = s.b.spec.as<double2>();
+  t.pupil = s.b.pupil.ptr;
+  t.ref = m.d_pupil64.as<double2>();
+  t.support = m.d_support.as<uint8_t>();
+  t.maxp_out = s.b.scal.as<double>();
+  t.status = s.b.flag.as<int32_t>() + 2;
+  t.tol = 64.0 * std::numeric_limits<double>::epsilon() * (double)m.support_count;
+  t.nh = m.nh;
+  t.nw = m.nw;
+  t.rh = m.rh;
+  t.rw = m.rw;
+  t.pupil_f32 = s.precision == FPMHIP_MIXED_F32 ? 1 : 0;
+  bool row_exact = true, col_exact = true;
+  for (uint64_t k = 0; k < m.S; ++k) {
+    row_exact = row_exact && std::fabs(m.off_r[k]) <= 1e-12;
+    col_exact = col_exact && std::fabs(m.off_c[k]) <= 1e-12;
+  }
+  t.row_exact = row_exact ? 1 : 0;
+  t.col_exact = col_exact ? 1 : 0;
+  s.b.gauge.allocate(sizeof(GaugeTile));
+  HIP_CHECK(hipMemcpyAsync(s.b.gauge.ptr, &t, sizeof(t), hipMemcpyHostToDevice, s.ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));  // `t` is on the host stack
 }
 
 void check_flags(fpmhip_session& s) {
@@ -209,6 +113,12 @@ void check_flags(fpmhip_session& s) {
   HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
   if (flags[0]) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
   if (flags[1]) fail(FPMHIP_INVALID_MEASUREMENTS, "no positive-weight, unmasked measurements are available");
+  if (flags[2]) {
+    int code = FPMHIP_NUMERICAL;
+    const char* msg = gauge_failure(flags[2], &code);
+    fail(code, msg ? msg : "object/pupil gauge normalization failed");
+  }
+  if (flags[3]) fail(FPMHIP_INVALID_MODEL, "state frame has invalid gain");
 }
 
 void init_aux(fpmhip_session& s) {
@@ -299,76 +209,57 @@ void load_calibration(fpmhip_session& s) {
                              s.ctx->stream));
 }
 
-// Sum of the calibration partials of the last launch_calibration_sums, in
-// block order.
-std::pair<double, double> calibration_totals(fpmhip_session& s) {
-  std::vector<double> part((size_t)s.nblk * 2);
-  HIP_CHECK(hipMemcpyAsync(part.data(), s.b.calib.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           s.ctx->stream));
-  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
-  double a = 0.0, b = 0.0;
-  for (uint32_t k = 0; k < s.nblk; ++k) {
-    a += part[2 * k];
-    b += part[2 * k + 1];
-  }
-  return {a, b};
-}
-
-double clamp_nan(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
-
 // update_gain then update_background for one frame (common.rs:144-168,
-// 390-489), from the frame's intrinsic prediction; `gain` is the frame's
-// gain in and out.
-void calibrate(fpmhip_session& s, uint64_t f, uint32_t K, const double* mode_w_dev, double& gain) {
+// 390-489), on the device from the frame's intrinsic prediction: the state's
+// gain of the frame and its background are updated in place, with no wait
+// on the host. Whether the background step applies (the frame has unmasked
+// pixels) is known on the host; the first step makes the background
+// per-frame (update_background expands a shared or absent background).
+void calibrate(fpmhip_session& s, uint64_t f, uint32_t K, const double* mode_w_dev) {
   const Model& m = *s.model;
-  if (s.cfg.gain_update) {
-    launch_calibration_sums(s, f, K, mode_w_dev, gain, 0);
-    const auto [num, den] = calibration_totals(s);
-    if (den > s.cfg.epsilon) {
-      const double est = clamp_nan(num / den, s.cfg.gain_min, s.cfg.gain_max);
-      gain = clamp_nan((1.0 - s.cfg.gain_step) * gain + s.cfg.gain_step * est, s.cfg.gain_min, s.cfg.gain_max);
-      if (!s.has_gain) {
-        s.has_gain = true;
-        s.gain.assign(m.F, 1.0);
-      }
-      s.gain[f] = gain;
-      HIP_CHECK(hipMemcpyAsync(s.b.gains.as<double>() + f, &s.gain[f], sizeof(double), hipMemcpyHostToDevice,
-                               s.ctx->stream));
-    }
-  }
+  const Measurements& me = *s.meas;
+  if (s.cfg.gain_update) launch_calibration(s, f, K, mode_w_dev, 0);
   if (s.cfg.bg_update) {
-    launch_calibration_sums(s, f, K, mode_w_dev, gain, 1);
-    const auto [rsum, rcount] = calibration_totals(s);
-    if (rcount > 0.0) {
-      const double correction = s.cfg.bg_step * rsum / rcount;
+    bool any = me.masks.empty();
+    for (uint64_t p = 0; !any && p < m.N(); ++p) any = me.masks[f * m.N() + p] != 0;
+    if (any) {
+      launch_calibration(s, f, K, mode_w_dev, 1);
       const uint64_t total = m.N() * m.F;
       if (s.bg_len != total) {
         launch_background_expand(s);
         s.bg_len = total;
       }
-      launch_background_shift(s, f, correction, s.cfg.bg_min, s.cfg.bg_max);
+      launch_background_shift(s, f, s.cfg.bg_min, s.cfg.bg_max);
     }
   }
 }
 
-// One iteration of projection_update over `order` (common.rs:57-306), frame
-// by frame on the session stream; objectives reduced on the host afterwards.
-void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_frame, double* objective) {
+// The host mirror of the state's frame gains after a sweep that recovered
+// them on the device (the stream has been synchronised).
+void download_recovered_gains(fpmhip_session& s) {
+  const Model& m = *s.model;
+  double present = 0.0;
+  HIP_CHECK(hipMemcpy(&present, s.b.scal.as<double>() + kScalGainsPresent, sizeof(double), hipMemcpyDeviceToHost));
+  if (present == 0.0 && !s.has_gain) return;
+  s.has_gain = true;
+  s.gain.resize(m.F);
+  HIP_CHECK(hipMemcpy(s.gain.data(), s.b.gains.ptr, m.F * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// The launches of one iteration of projection_update over `order`
+// (common.rs:57-306), frame by frame on the session stream (checked by
+// sweep_checks beforehand, so nothing here fails mid-way). Host-side state
+// they advance: the mPIE counter and the background length.
+void issue_sweep(fpmhip_session& s, const uint64_t* order, uint64_t n) {
   const Model& m = *s.model;
   const Measurements& me = *s.meas;
   const uint64_t N = m.N();
   launch_flag_reset(s);
   if (!s.cfg.has_pupil_step) refresh_max_pupil(s);
-  std::vector<double> fw(n, 0.0);
-  std::vector<bool> ran(n, false);
   for (uint64_t i = 0; i < n; ++i) {
     const uint64_t f = order[i];
-    if (f >= m.F) fail(FPMHIP_FRAME_OUT_OF_RANGE, "frame index out of range");
     const double w = me.weight[f];
-    fw[i] = w;
     if (w == 0.0) continue;
-    const double gain = s.has_gain ? s.gain[f] : 1.0;
-    if (!std::isfinite(gain) || gain <= 0.0) fail(FPMHIP_INVALID_MODEL, "state frame has invalid gain");
     uint64_t e0 = f, e1 = f + 1;
     if (m.mux) {
       e0 = m.row_ptr[f];
@@ -386,11 +277,8 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
       fft2_f64(*m.fft_low, s.b.field.as<double2>(), K, true, s.ctx->stream);
     else
       fft2_f32(*m.fft_low, s.b.field.as<float2>(), K, true, s.ctx->stream);
-    double frame_gain = gain;
-    if (s.cfg.gain_update || s.cfg.bg_update)
-      calibrate(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, frame_gain);
-    launch_project(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, sw_sum, frame_gain, i);
-    ran[i] = true;
+    if (s.cfg.gain_update || s.cfg.bg_update) calibrate(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr);
+    launch_project(s, f, K, m.mux ? m.d_mux_w.as<double>() + e0 : nullptr, sw_sum, i);
     for (uint32_t k = 0; k < K; ++k) {
       const uint64_t src = m.mux ? m.msrc[e0 + k] : f;
       const double swk = m.mux ? m.mw[e0 + k] : 1.0;
@@ -410,22 +298,100 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
     if (s.cfg.momentum) {  // apply_object_momentum (common.rs:308-353)
       s.counter += 1;
       if (s.counter >= s.cfg.interval) {
-        if (s.counter != s.cfg.interval)
-          fail(FPMHIP_INVALID_MODEL, "mPIE frame counter exceeds the configured momentum interval");
         launch_momentum(s);
         s.counter = 0;
       }
     }
   }
+  if (s.cfg.gauge) canonicalize_session(s);
+}
+
+// The checks of a sweep that the reference makes while it runs, made up
+// front from host state: frame range, the state's gains (unless recovered
+// on the device, which checks them), the mPIE counter.
+void sweep_checks(const fpmhip_session& s, const uint64_t* order, uint64_t n) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  uint64_t counter = s.counter;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t f = order[i];
+    if (f >= m.F) fail(FPMHIP_FRAME_OUT_OF_RANGE, "frame index out of range");
+    if (me.weight[f] == 0.0) continue;
+    if (!s.cfg.gain_update) {
+      const double gain = s.has_gain ? s.gain[f] : 1.0;
+      if (!std::isfinite(gain) || gain <= 0.0) fail(FPMHIP_INVALID_MODEL, "state frame has invalid gain");
+    }
+    if (s.cfg.momentum) {
+      counter += 1;
+      if (counter >= s.cfg.interval) {
+        if (counter != s.cfg.interval)
+          fail(FPMHIP_INVALID_MODEL, "mPIE frame counter exceeds the configured momentum interval");
+        counter = 0;
+      }
+    }
+  }
+}
+
+// One iteration (projection_update over `order`, then canonicalize_state),
+// as one replay of a HIP graph of its launches: a sweep's launches depend on
+// the order, the mPIE counter and the background length at its start, and
+// the graph of each such start is captured once per session and replayed
+// after (the session keeps at most kSweepGraphs of them). Objectives are
+// reduced on the host afterwards, in order.
+void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_frame, double* objective) {
+  const Model& m = *s.model;
+  const Measurements& me = *s.meas;
+  sweep_checks(s, order, n);
+  SweepGraph* g = nullptr;
+  for (SweepGraph& c : s.graphs)
+    if (c.counter0 == s.counter && c.bg0 == s.bg_len && c.order.size() == n &&
+        std::equal(c.order.begin(), c.order.end(), order))
+      g = &c;
+  if (!g) {
+    if (s.graphs.size() >= kSweepGraphs) s.graphs.erase(s.graphs.begin());
+    s.graphs.emplace_back();
+    g = &s.graphs.back();
+    g->order.assign(order, order + n);
+    g->counter0 = s.counter;
+    g->bg0 = s.bg_len;
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(s.ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      issue_sweep(s, order, n);
+    } catch (...) {
+      (void)hipStreamEndCapture(s.ctx->stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      s.counter = g->counter0;
+      s.bg_len = g->bg0;
+      s.graphs.pop_back();
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(s.ctx->stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    g->counter1 = s.counter;
+    g->bg1 = s.bg_len;
+    s.counter = g->counter0;
+    s.bg_len = g->bg0;
+    if (e != hipSuccess) {
+      s.graphs.pop_back();
+      HIP_CHECK(e);
+    }
+  }
+  HIP_CHECK(hipGraphLaunch(g->exec, s.ctx->stream));
+  s.counter = g->counter1;
+  s.bg_len = g->bg1;
   std::vector<double> partials(n * s.nblk * 2);
   if (n)
     HIP_CHECK(hipMemcpyAsync(partials.data(), s.b.partials.ptr, partials.size() * sizeof(double),
                              hipMemcpyDeviceToHost, s.ctx->stream));
   check_flags(s);
+  if (s.cfg.gain_update) download_recovered_gains(s);
   double osum = 0.0, wsum = 0.0;
   for (uint64_t i = 0; i < n; ++i) {
+    const double fw = me.weight[order[i]];
     double loss = 0.0;
-    if (ran[i]) {
+    if (fw != 0.0) {
       double ls = 0.0, lc = 0.0;
       for (uint32_t b = 0; b < s.nblk; ++b) {
         ls += partials[(i * s.nblk + b) * 2];
@@ -435,10 +401,10 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
       loss = ls / lc;
     }
     if (per_frame) per_frame[i] = loss;
-    osum += fw[i] * loss;  // StepSummary::push_frame (metrics.rs:25-31)
-    wsum += fw[i];
+    osum += fw * loss;  // StepSummary::push_frame (metrics.rs:25-31)
+    wsum += fw;
   }
-  if (s.cfg.gauge) canonicalize_session(s);
+  (void)m;
   if (objective) *objective = wsum > 0.0 ? osum / wsum : std::numeric_limits<double>::quiet_NaN();
 }
 
@@ -517,9 +483,8 @@ void session_initialize_launch(fpmhip_session& s) {
 }
 
 void session_initialize_check(fpmhip_session& s) {
-  check_flags(s);  // waits for the launches
   if (s.cfg.gauge) canonicalize_session(s);
-  HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+  check_flags(s);  // waits for the launches
   s.initialized = true;
 }
 
@@ -546,7 +511,11 @@ int fpmhip_context_destroy(fpmhip_context* c) {
   return guarded([&] {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream) {
+      // work queued on the stream finishes before the stream goes away
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamDestroy(c->stream);
+    }
     delete c;
   });
 }
@@ -826,6 +795,7 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     }
     upload_momentum_spans(*s, m->pupil.data());
     launch_flag_reset(*s);
+    if (s->cfg.gauge) prepare_gauge(*s);
     HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
     *out = s.release();
   });
@@ -905,8 +875,9 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
       }
     }
     refresh_max_pupil(*s);
+    launch_flag_reset(*s);
     if (s->cfg.gauge) canonicalize_session(*s);
-    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    check_flags(*s);
     s->initialized = true;
   });
 }

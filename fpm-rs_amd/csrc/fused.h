@@ -12,7 +12,7 @@ struct TileDesc {
   double2* spec;          // [R] FP64 centred object spectrum
   float2* vel;            // [R] mPIE velocity (or null), complex64 storage
   float2* anc;            // [R] mPIE anchor (or null), complex64 storage
-  const float2* pupil;    // [N] complex64 pupil
+  float2* pupil;          // [N] complex64 pupil (updated in place by EPRY pupil recovery)
   const uint32_t* extent; // [n] per-row support of the pupil: columns [lo, hi) as lo | hi << 16
   const uint32_t* dest;   // live-pixel lists, tiles <= 128^2: [0] insert destinations, [1] support
                           // strips, then (n+1)^2 slots of destinations a | b << 16 and the strips
@@ -44,7 +44,10 @@ struct TileDesc {
   uint32_t n, rh, rw, F;
   uint32_t interval;
   int32_t is_u16, denom, momentum, loss;
+  int32_t recover_pupil;  // EPRY pupil recovery (common.rs:271-300; tiles up to kFusedLdsMaxN)
+  int32_t pad_;
   double object_step, alpha, eps, max_p, friction, feedback;
+  double pupil_step;
 };
 
 // Offset (in float2 units) of the forward/inverse twiddle tables of length n

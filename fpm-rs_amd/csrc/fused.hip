@@ -286,6 +286,20 @@ __device__ __forceinline__ double2 wg_sum2(double a, double b, double* red) {
   return make_double2(sa, sb);  // valid in thread 0
 }
 
+// Fixed-order workgroup maximum of one double per thread; valid in every
+// thread (red holds kThreads / 64 doubles).
+__device__ __forceinline__ double wg_max(double a, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) a = fmax(a, __shfl_xor(a, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < kThreads / 64; ++w) r = fmax(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
 // Object-update denominator of common.rs:247-258 as a + b |P|^2 with FP64
 // constants a, b rounded once: Local: (eps, 1); Global: (max + eps, 0);
 // Rpie: (alpha max + eps, 1 - alpha).
@@ -540,6 +554,69 @@ __device__ __forceinline__ void finish_sweep(const TileDesc& T, double obj_sum, 
 // Whether column c lies in a row's support extent (lo | hi << 16).
 __device__ __forceinline__ bool in_extent(uint32_t e, int c) {
   return c >= (int)(e & 0xFFFFu) && c < (int)(e >> 16);
+}
+
+// max |patch|^2 over the whole n x n bilinear patch of a source (the EPRY
+// pupil-step denominator, common.rs:271-277), in FP64: thread t takes column
+// t % N and N^2 / kThreads consecutive rows from row (t / N) N^2 / kThreads,
+// tap rows shared between consecutive pixels, in groups of G rows.
+template <int N>
+__device__ __forceinline__ double patch_max(Rsrc rspec, const SrcParam& sp, uint32_t rw, int tid) {
+  constexpr int PP = N * N / kThreads, G = PP < 4 ? PP : 4;
+  static_assert(PP >= 1 && PP % G == 0, "patch rows per thread");
+  const int c = tid % N, r0 = (tid / N) * PP;
+  const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
+  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+               wuu = sp.ar.wu * sp.ac.wu;
+  const double2 zero = make_double2(0.0, 0.0);
+  double m = 0.0;
+#pragma unroll
+  for (int g = 0; g < PP; g += G) {
+    const uint32_t o = (sp.ar.lower + (uint32_t)(r0 + g)) * rw + sp.ac.lower + (uint32_t)c;
+    double2 t0[G + 1], t1[G + 1];
+#pragma unroll
+    for (int j = 0; j <= G; ++j) {
+      const bool need = j < G || fr;
+      t0[j] = bld_d2(rspec, need ? (o + j * rw) * 16u : kSkip);
+      t1[j] = bld_d2(rspec, need && fc ? (o + j * rw + 1) * 16u : kSkip);
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
+      const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
+                                     fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
+      m = fmax(m, v.x * v.x + v.y * v.y);
+    }
+  }
+  return m;
+}
+
+// The bilinear patch values (crop.rs:129-167, FP64 taps, rounded to FP32) of
+// the support pixels of one strip entry `e` (the arithmetic of
+// gather_strips), for the EPRY pupil gradient.
+__device__ __forceinline__ void strip_patch(Rsrc rspec, const SrcParam& sp, uint32_t rw, uint32_t e,
+                                            float2 (&out)[kGatherStrip]) {
+  constexpr int S = (int)kGatherStrip;
+  const uint32_t r0 = e & 0xFFu, c = (e >> 8) & 0xFFu, mask = e >> 16;
+  const uint32_t o = (sp.ar.lower + r0) * rw + sp.ac.lower + c;
+  const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
+  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
+               wuu = sp.ar.wu * sp.ac.wu;
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 t0[S + 1], t1[S + 1];
+#pragma unroll
+  for (int j = 0; j <= S; ++j) {
+    const bool need = (j < S && (mask >> j & 1u)) || (fr && j > 0 && (mask >> (j - 1) & 1u));
+    t0[j] = bld_d2(rspec, need ? (o + j * rw) * 16u : kSkip);
+    t1[j] = bld_d2(rspec, need && fc ? (o + j * rw + 1) * 16u : kSkip);
+  }
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
+    const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
+                                   fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
+    out[j] = f2((float)v.x, (float)v.y);
+  }
 }
 
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
@@ -1056,14 +1133,25 @@ __device__ __forceinline__ void copy_in_live(float2* lds, const float2* __restri
   }
 }
 
-// Group barrier of a cooperative tile (members > 1). Each wave first waits
-// for its own device-coherent stores to complete (s_waitcnt vmcnt(0); a
-// completed device-scope store is visible to device-scope loads on every
-// CU), then one thread per workgroup arrives on the group's counter and
-// spins, bounded, until every member has arrived. `passed` counts the
-// group's barriers since its state was last reset. Returns false when the
-// group stalled (the spin limit expired here or in another member): the
-// caller abandons the sweep, so every workgroup still exits.
+// Group barrier of a cooperative tile (members > 1). Every access of data
+// shared between the members (the field, chunk edges, partial sums, the
+// spectrum and momentum arrays of the tile) is device-coherent: relaxed
+// agent-scope atomics or buffer accesses with the sc1 policy bit, which the
+// hardware serves from the device-coherent level. Each wave first waits for
+// its own stores to complete (s_waitcnt vmcnt(0): a completed device-scope
+// store is visible to device-scope loads on every CU), then the workgroup
+// synchronises (a workgroup-scope fence: the compiler keeps every access of
+// the workgroup on its side of it), one thread arrives on the group's
+// counter and spins, bounded, until every member has arrived, and a second
+// workgroup barrier holds the other threads' accesses until then. The
+// arrival and the spin are relaxed, not release / acquire: at agent scope
+// those compile to a write-back and an invalidation of the whole L2 of the
+// XCD, which the coherent accesses do not need and which cost the uneven
+// batch (16 cooperative groups beside 80 single-workgroup tiles) 28 % of
+// its sweep time (8.53 vs 6.66 ms per sweep). `passed` counts the group's
+// barriers since its state was last reset. Returns false when the group
+// stalled (the spin limit expired here or in another member): the caller
+// abandons the sweep, so every workgroup still exits.
 constexpr int kWaitVmcnt0 = 0x0F70;  // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15)
 constexpr uint32_t kGroupSpinLimit = 1u << 20;
 

@@ -96,15 +96,20 @@ __global__ void k_extract(const double2* __restrict__ spec, uint32_t rw, const S
 
 // predicted intensity, objective terms and amplitude ratio (common.rs:106-196);
 // the first mode's difference field (:219-228) is written in place.
+// The frame's gain is read from the state's device gains (EPRY recovery may
+// have just updated it); an invalid gain raises flag[3] (common.rs:139-143).
 template <class V>
 __global__ void __launch_bounds__(kBlock) k_project(V* __restrict__ fields, uint32_t K, const double* __restrict__ mode_w,
                                                     double sw_sum, uint32_t N, const void* __restrict__ frames,
                                                     int is_u16, uint64_t frame, const uint8_t* __restrict__ masks,
-                                                    double gain, const double* __restrict__ bg, uint64_t bg_len,
-                                                    double eps, int loss, double* __restrict__ pred_out,
-                                                    V* __restrict__ proj_out, double* __restrict__ partials) {
+                                                    const double* __restrict__ gain_p, const double* __restrict__ bg,
+                                                    uint64_t bg_len, double eps, int loss, double* __restrict__ pred_out,
+                                                    V* __restrict__ proj_out, double* __restrict__ partials,
+                                                    int* __restrict__ flag) {
   __shared__ double sh[kBlock];
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const double gain = *gain_p;
+  if (p == 0 && !(isfinite(gain) && gain > 0.0)) flag[3] = 1;
   double lsum = 0.0, lcnt = 0.0;
   if (p < N) {
     double pred = 0.0;
@@ -148,10 +153,11 @@ template <class V>
 __global__ void __launch_bounds__(kBlock)
     k_calibration_sums(const V* __restrict__ fields, uint32_t K, const double* __restrict__ mode_w, uint32_t N,
                        const void* __restrict__ frames, int is_u16, uint64_t frame, const uint8_t* __restrict__ masks,
-                       const double* __restrict__ bg, uint64_t bg_len, double gain, int mode,
+                       const double* __restrict__ bg, uint64_t bg_len, const double* __restrict__ gain_p, int mode,
                        double* __restrict__ partials) {
   __shared__ double sh[kBlock];
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const double gain = *gain_p;
   double a = 0.0, b = 0.0;
   if (p < N && !(masks && masks[frame * N + p] == 0)) {
     double pred = 0.0;
@@ -177,9 +183,36 @@ __global__ void __launch_bounds__(kBlock)
 // clamp as f64::clamp: NaN stays NaN
 __device__ inline double clamp_nan(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-__global__ void k_background_shift(double* __restrict__ bg, uint64_t N, double correction, double lo, double hi) {
+// The calibration step of one frame from the block partials of
+// k_calibration_sums, summed in block order by one thread (common.rs:390-489):
+// mode 0 (update_gain): when sum pred^2 > eps, gain <- clamp((1 - step) gain +
+// step clamp(sum pred max(y - bg, 0) / sum pred^2)), written to the state's
+// gains, and present[0] = 1 (the state now holds frame gains); mode 1
+// (update_background): out[0] = step * residual sum / unmasked pixel count.
+__global__ void k_calibration_apply(const double* __restrict__ partials, uint32_t nblk, int mode,
+                                    double* __restrict__ gain, double eps, double step, double lo, double hi,
+                                    double* __restrict__ out, double* __restrict__ present) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (uint32_t k = 0; k < nblk; ++k) {
+    a += partials[2 * k];
+    b += partials[2 * k + 1];
+  }
+  if (mode == 0) {
+    if (b > eps) {
+      const double est = clamp_nan(a / b, lo, hi);
+      *gain = clamp_nan((1.0 - step) * *gain + step * est, lo, hi);
+      *present = 1.0;
+    }
+  } else {
+    *out = step * a / b;  // launched only for frames with unmasked pixels (b > 0)
+  }
+}
+
+__global__ void k_background_shift(double* __restrict__ bg, uint64_t N, const double* __restrict__ correction,
+                                   double lo, double hi) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < N) bg[p] = clamp_nan(bg[p] + correction, lo, hi);
+  if (p < N) bg[p] = clamp_nan(bg[p] + *correction, lo, hi);
 }
 
 // in place: entries [0, N) hold the shared background (or nothing)
@@ -329,8 +362,10 @@ __global__ void k_momentum(double2* __restrict__ obj, W* __restrict__ vel, W* __
   anc[i] = from_d2<W>(no);
 }
 
-__global__ void k_flag_reset(int* flag) {
+// Clears the session's flags and its per-sweep calibration marker.
+__global__ void k_flag_reset(int* flag, double* gains_present) {
   if (threadIdx.x < 4) flag[threadIdx.x] = 0;
+  if (threadIdx.x == 0) *gains_present = 0.0;
 }
 
 // weighted mean measured amplitude per low-resolution pixel (state.rs:288-313)
@@ -630,33 +665,39 @@ void launch_extract(const fpmhip_session& s, uint64_t source, uint32_t mode) {
 }
 
 void launch_project(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev, double sw_sum,
-                    double gain, uint64_t slot) {
+                    uint64_t slot) {
   const Model& m = *s.model;
   const Measurements& me = *s.meas;
   const uint64_t N = m.N();
   DISPATCH_V(s, hipLaunchKernelGGL(k_project<V>, dim3(s.nblk), dim3(kBlock), 0, s.ctx->stream, s.b.field.as<V>(),
                                    K, mode_w_dev, sw_sum, (uint32_t)N, me.d_frames.ptr, me.is_u16 ? 1 : 0, frame,
-                                   me.d_masks.as<uint8_t>(), gain, s.b.bg.as<double>(), s.bg_len, s.cfg.epsilon,
-                                   s.cfg.loss, s.b.pred.as<double>(),
-                                   s.b.proj.as<V>(), s.b.partials.as<double>() + slot * s.nblk * 2));
+                                   me.d_masks.as<uint8_t>(), s.b.gains.as<double>() + frame, s.b.bg.as<double>(),
+                                   s.bg_len, s.cfg.epsilon, s.cfg.loss, s.b.pred.as<double>(), s.b.proj.as<V>(),
+                                   s.b.partials.as<double>() + slot * s.nblk * 2, s.b.flag.as<int>()));
   HIP_CHECK(hipGetLastError());
 }
 
-void launch_calibration_sums(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
-                             double gain, int mode) {
+void launch_calibration(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev, int mode) {
   const Model& m = *s.model;
   const Measurements& me = *s.meas;
+  double* gain = s.b.gains.as<double>() + frame;
   DISPATCH_V(s, hipLaunchKernelGGL(k_calibration_sums<V>, dim3(s.nblk), dim3(kBlock), 0, s.ctx->stream,
                                    s.b.field.as<V>(), K, mode_w_dev, (uint32_t)m.N(), me.d_frames.ptr,
                                    me.is_u16 ? 1 : 0, frame, me.d_masks.as<uint8_t>(), s.b.bg.as<double>(),
                                    s.bg_len, gain, mode, s.b.calib.as<double>()));
   HIP_CHECK(hipGetLastError());
+  const Config& c = s.cfg;
+  double* scal = s.b.scal.as<double>();
+  hipLaunchKernelGGL(k_calibration_apply, dim3(1), dim3(64), 0, s.ctx->stream, s.b.calib.as<double>(), s.nblk, mode,
+                     gain, c.epsilon, mode == 0 ? c.gain_step : c.bg_step, mode == 0 ? c.gain_min : c.bg_min,
+                     mode == 0 ? c.gain_max : c.bg_max, scal + kScalBackgroundStep, scal + kScalGainsPresent);
+  HIP_CHECK(hipGetLastError());
 }
 
-void launch_background_shift(const fpmhip_session& s, uint64_t frame, double correction, double lo, double hi) {
+void launch_background_shift(const fpmhip_session& s, uint64_t frame, double lo, double hi) {
   const uint64_t N = s.model->N();
   hipLaunchKernelGGL(k_background_shift, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
-                     s.b.bg.as<double>() + frame * N, N, correction, lo, hi);
+                     s.b.bg.as<double>() + frame * N, N, s.b.scal.as<double>() + kScalBackgroundStep, lo, hi);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -716,7 +757,8 @@ void launch_momentum(const fpmhip_session& s) {
 }
 
 void launch_flag_reset(const fpmhip_session& s) {
-  hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(64), 0, s.ctx->stream, s.b.flag.as<int>());
+  hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(64), 0, s.ctx->stream, s.b.flag.as<int>(),
+                     s.b.scal.as<double>() + kScalGainsPresent);
   HIP_CHECK(hipGetLastError());
 }
 

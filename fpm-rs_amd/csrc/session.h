@@ -36,6 +36,10 @@ struct Config {
 };
 Config make_config(const fpmhip_algorithm& a);
 
+// Slots of SessionBuffers::scal written by the calibration updates.
+constexpr int kScalBackgroundStep = 4;
+constexpr int kScalGainsPresent = 5;
+
 // Rows per strip of the fused sweep's gather list.
 constexpr uint32_t kGatherStrip = 5;
 // Layout of SessionBuffers::dest for an n x n tile, in uint32 words: header
@@ -62,7 +66,8 @@ struct SessionBuffers {
   DeviceBuffer proj;      // V[N]
   DeviceBuffer U;         // V[N]
   DeviceBuffer grad;      // V[N]
-  DeviceBuffer scal;      // double[8]: [0] max|P|^2, [1] max|patch|^2, [2] initial fallback amplitude
+  DeviceBuffer scal;      // double[8]: [0] max|P|^2, [1] max|patch|^2, [2] initial fallback amplitude,
+                          // [kScalBackgroundStep], [kScalGainsPresent] (EPRY calibration)
   DeviceBuffer partials;  // double[F * nblk * 2]
   DeviceBuffer flag;      // int[4]
   DeviceBuffer work_m;    // double2[M] (init / result scratch)
@@ -81,8 +86,37 @@ struct SessionBuffers {
   DeviceBuffer bg;        // double[F * N] background of the reconstruction state (bg_len used)
   DeviceBuffer gains;     // double[F] frame gains of the state (1 where the model has none)
   DeviceBuffer calib;     // double[nblk * 2] block partial sums of the calibration updates
+  DeviceBuffer gauge;     // GaugeTile of the EPRY gauge (sessions that canonicalize)
 };
 
+}  // namespace fpmhip
+
+namespace fpmhip {
+// A captured per-stage sweep: its frame order and the mPIE counter and
+// background length before and after it.
+struct SweepGraph {
+  std::vector<uint64_t> order;
+  uint64_t counter0 = 0, bg0 = 0, counter1 = 0, bg1 = 0;
+  hipGraphExec_t exec = nullptr;
+  SweepGraph() = default;
+  SweepGraph(const SweepGraph&) = delete;
+  SweepGraph& operator=(const SweepGraph&) = delete;
+  SweepGraph(SweepGraph&& o) noexcept { *this = std::move(o); }
+  SweepGraph& operator=(SweepGraph&& o) noexcept {
+    if (this != &o) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      order = std::move(o.order);
+      counter0 = o.counter0, bg0 = o.bg0, counter1 = o.counter1, bg1 = o.bg1;
+      exec = o.exec;
+      o.exec = nullptr;
+    }
+    return *this;
+  }
+  ~SweepGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+constexpr size_t kSweepGraphs = 64;
 }  // namespace fpmhip
 
 struct fpmhip_session {
@@ -101,6 +135,7 @@ struct fpmhip_session {
   uint64_t bg_len = 0;  // background of the state: 0 (none), N (shared) or F * N (per frame)
   bool extent_from_model = false;  // b.extent / b.dest describe the model's pupil
   fpmhip::SessionBuffers b;
+  std::vector<fpmhip::SweepGraph> graphs;  // captured per-stage sweeps (api.hip)
   size_t vbytes() const { return precision == FPMHIP_PLAIN_C128 ? sizeof(double2) : sizeof(float2); }
 };
 
@@ -109,15 +144,19 @@ namespace fpmhip {
 // ---- launchers (kernels.hip) ------------------------------------------------
 // All take the precision of the session (double2 for PLAIN_C128, float2 for MIXED_F32).
 void launch_extract(const fpmhip_session& s, uint64_t source, uint32_t mode);
+// Projection of `frame` with its gain from the state's device gains.
 void launch_project(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
-                    double sw_sum, double gain, uint64_t slot);
-// Block partial sums over the unmasked pixels of `frame` from the current
-// fields: mode 0 (update_gain): sum pred * max(y - bg, 0), sum pred^2;
-// mode 1 (update_background): sum (y - gain * pred - bg), pixel count.
-void launch_calibration_sums(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev,
-                             double gain, int mode);
-// Background state of the session: bg[frame * N + p] = clamp(bg + correction, lo, hi).
-void launch_background_shift(const fpmhip_session& s, uint64_t frame, double correction, double lo, double hi);
+                    double sw_sum, uint64_t slot);
+// One calibration update of `frame` on the device, from the current fields
+// over its unmasked pixels: mode 0 (update_gain) sums pred * max(y - bg, 0)
+// and pred^2 and updates the state's gain of the frame (scal[kScalGainsPresent]
+// = 1 once any gain was updated); mode 1 (update_background) sums the residual
+// y - gain pred - bg and the pixel count into scal[kScalBackgroundStep]
+// (launch only for frames with unmasked pixels).
+void launch_calibration(const fpmhip_session& s, uint64_t frame, uint32_t K, const double* mode_w_dev, int mode);
+// Background state of the session: bg[frame * N + p] = clamp(bg +
+// scal[kScalBackgroundStep], lo, hi).
+void launch_background_shift(const fpmhip_session& s, uint64_t frame, double lo, double hi);
 // Expands a shared (length N) or absent background state to per-frame F * N.
 void launch_background_expand(const fpmhip_session& s);
 void launch_difference_frame(const fpmhip_session& s, uint32_t mode, uint64_t frame);

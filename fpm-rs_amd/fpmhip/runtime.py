@@ -6,8 +6,10 @@ There is no CPU fallback: a missing library or device raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as ct
 import os
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -46,6 +48,36 @@ class FpmError(RuntimeError):
 
 
 _lib = None
+
+# Ordered teardown. Every live handle is registered here; at interpreter exit
+# (or on close_all()) they are destroyed dependents first: servers, batches,
+# sessions, measurements, models, then contexts, while the HIP runtime is
+# still up. After that, __del__ does nothing, so that no destroy call runs in
+# an arbitrary order during the interpreter's own finalization.
+_LEVELS = ("server", "batch", "session", "measurements", "model", "context")
+_live = {k: weakref.WeakSet() for k in _LEVELS}
+_closed = False
+
+
+def _register(level, obj):
+    _live[level].add(obj)
+
+
+def close_all():
+    """Destroys every live handle of this process, dependents first (also
+    run at exit). Handles closed this way are unusable afterwards."""
+    global _closed, _default_context
+    for level in _LEVELS:
+        for obj in list(_live[level]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _default_context = None
+    _closed = True
+
+
+atexit.register(close_all)
 
 
 def lib():
@@ -116,6 +148,7 @@ class Context:
     def __init__(self, device=0):
         self.handle = ct.c_void_p()
         check(lib().fpmhip_context_create(ct.c_int32(device), ct.byref(self.handle)))
+        _register("context", self)
 
     def close(self):
         if self.handle:
@@ -123,10 +156,11 @@ class Context:
             self.handle = ct.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def synchronize(self):
         check(lib().fpmhip_context_synchronize(self.handle))
@@ -155,13 +189,19 @@ class DeviceModel:
         self.handle = ct.c_void_p()
         d = model.desc()
         check(lib().fpmhip_model_create(ctx.handle, ct.byref(d), ct.byref(self.handle)))
+        _register("model", self)
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_model_destroy(self.handle)
+            self.handle = ct.c_void_p()
 
     def __del__(self):
-        try:
-            if self.handle:
-                lib().fpmhip_model_destroy(self.handle)
-        except Exception:
-            pass
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 class DeviceMeasurements:
@@ -177,13 +217,19 @@ class DeviceMeasurements:
             ctx.handle, dmodel.handle, None if counts is None else abi.ptr(counts, ct.c_uint16),
             None if frames is None else _cd(frames), None if w is None else _cd(w),
             None if m is None else abi.ptr(m, ct.c_uint8), ct.byref(self.handle)))
+        _register("measurements", self)
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_measurements_destroy(self.handle)
+            self.handle = ct.c_void_p()
 
     def __del__(self):
-        try:
-            if self.handle:
-                lib().fpmhip_measurements_destroy(self.handle)
-        except Exception:
-            pass
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 def forward_stack(model: ImagePlaneModel, spectrum, pupil=None, context: Optional[Context] = None,
@@ -216,13 +262,19 @@ class Session:
         self.R = tuple(m.reconstruction_shape)
         self.N = tuple(m.image_shape)
         self.F = m.frame_count
+        _register("session", self)
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_session_destroy(self.handle)
+            self.handle = ct.c_void_p()
 
     def __del__(self):
-        try:
-            if self.handle:
-                lib().fpmhip_session_destroy(self.handle)
-        except Exception:
-            pass
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def initialize(self):
         check(lib().fpmhip_session_initialize(self.handle))
@@ -435,13 +487,19 @@ class Batch:
         self.handle = ct.c_void_p()
         check(lib().fpmhip_batch_create(self.ctx.handle, arr, ct.c_uint64(len(self.sessions)),
                                         ct.byref(self.handle)))
+        _register("batch", self)
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_batch_destroy(self.handle)
+            self.handle = ct.c_void_p()
 
     def __del__(self):
-        try:
-            if self.handle:
-                lib().fpmhip_batch_destroy(self.handle)
-        except Exception:
-            pass
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def run(self, iterations):
         """Initialise every tile and run `iterations` sweeps; returns the
@@ -539,17 +597,20 @@ class Server:
         self.handle = ct.c_void_p()
         check(lib().fpmhip_server_create(self.ctx.handle, ct.byref(cfg), ct.byref(self.handle)))
         self._shapes = {}
+        _register("server", self)
 
     def close(self):
+        """Stops the worker thread (joined) and frees the server's device state."""
         if self.handle:
             lib().fpmhip_server_destroy(self.handle)
             self.handle = ct.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def submit(self, problem: ReconstructionProblem, algorithm) -> int:
         """Queue one tile job; returns its id."""

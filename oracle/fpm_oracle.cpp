@@ -34,6 +34,14 @@
 //                   5.4e-5 (:992)
 //   gauge           energy / overlap / DC invariants (reconstruction_tests.rs:2674),
 //                   prediction invariance (src/algorithms/gauge.rs:436,465)
+//   point_loss      the reference's scalar values (tests/diagnostics_tests.rs:34-58)
+//                   and the Huber / clamping branches by closed form
+//   schedules       BrightfieldFirst / SpiralOut orders (tests/schedule_tests.rs:15-22),
+//                   SnrWeighted with a masked outlier (:37-48)
+//   EPRY pupil      pupil phase error falls from the defocus + astigmatism
+//                   start (tests/reconstruction_tests.rs:2923)
+//   aberrations     DC phase = spherical, axis values of astigmatism / coma,
+//                   apodization, defocus quadratic in |k| with its sign
 //   forward stack   constant object (forward_model_tests.rs:76), intensity =
 //                   |IFFT|^2 written with numpy
 #include <algorithm>
@@ -1277,6 +1285,13 @@ typedef struct orc_outputs {
 
 const char* orc_last_error(void) { return g_error.c_str(); }
 const char* orc_last_error_name(void) { return g_error_name.c_str(); }
+
+// point_loss (src/algorithms/objective.rs:47-71) of n value pairs.
+int orc_point_loss(const double* predicted, const double* measured, uint64_t n, int32_t loss, double* out) {
+  return guarded([&] {
+    for (uint64_t i = 0; i < n; ++i) out[i] = point_loss(predicted[i], measured[i], loss);
+  });
+}
 
 int orc_fft2(double* values, uint64_t h, uint64_t w, int32_t direction) {
   return guarded([&] {

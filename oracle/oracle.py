@@ -63,6 +63,31 @@ class OrcOutputs(ct.Structure):
 _lib = None
 
 
+def host_isa_level():
+    """'v4' when this host's CPU has the AVX-512 subsets of x86-64-v4, else 'v3'."""
+    try:
+        flags = set()
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+        return "v4" if {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags else "v3"
+    except OSError:
+        return "v3"
+
+
+def use_widest_isa():
+    """Loads the x86-64-v4 build of the oracle when the host supports it
+    (before the first call into the oracle); returns the level in use."""
+    global LIB_PATH
+    level = host_isa_level()
+    v4 = os.path.join(HERE, "libfpm_oracle_v4.so")
+    if _lib is None and level == "v4" and os.path.exists(v4):
+        LIB_PATH = v4
+    return "v4" if LIB_PATH == v4 else "v3"
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -82,6 +107,15 @@ def _check(code):
 
 def _cd(a):
     return a.ctypes.data_as(abi.P_dbl)
+
+
+def point_loss(predicted, measured, loss):
+    """point_loss (objective.rs:47-71) element-wise; loss is an abi LOSS code."""
+    pr = np.ascontiguousarray(predicted, dtype=np.float64).ravel()
+    me = np.ascontiguousarray(measured, dtype=np.float64).ravel()
+    out = np.zeros_like(pr)
+    _check(lib().orc_point_loss(_cd(pr), _cd(me), ct.c_uint64(pr.size), ct.c_int32(loss), _cd(out)))
+    return out
 
 
 def fft2(values, direction="forward"):

@@ -301,3 +301,36 @@ def test_state_background_and_gains_restore(ctx):
     assert e <= 1e-10
     g2, b2 = s.calibration()
     assert np.array_equal(g2, gains) and np.array_equal(b2, bg)
+
+
+def test_config2_plain_c128_mpie_full_length(ctx):
+    """The measured workload's plain path at its full length: 225 LEDs, 128^2
+    -> 375^2, uint16 counts, mPIE 50 sweeps (375 momentum events, interval
+    30, friction/feedback 0.5) within 1e-10 of the oracle, velocity and
+    counter included."""
+    model, meas = _config2()
+    alg = Mpie(iterations=50, momentum_friction=0.5, momentum_feedback=0.5)
+    got, ref = run_both(model, meas, alg, 50)
+    assert_parity(got, ref, 1e-10)
+    assert got["counter"] == ref["counter"]
+    assert rel(got["velocity"], ref["velocity"]) <= 1e-10
+
+
+@pytest.mark.parametrize("loss", ["intensity_mse", "poisson_nll", "huber_amplitude"])
+def test_loss_types_match_oracle(ctx, loss):
+    """The non-default LossTypes (objective.rs:47-71) on the device: the
+    per-stage FP64 path within 1e-10 (object, pupil and the objective trace,
+    which is where the loss enters), and the fused batch (FP32 per-pixel
+    terms, FP64 frame sums) within 1e-5. 49 LEDs, 64^2, mPIE 3 sweeps."""
+    model, meas = _config2(n=64, k=7, peak=4e3)
+    alg = Mpie(iterations=3, momentum_interval=30, momentum_friction=0.5, momentum_feedback=0.5, loss_type=loss)
+    got, ref = run_both(model, meas, alg, 3)
+    assert_parity(got, ref, 1e-10)
+    s = runtime.Session(ReconstructionProblem(meas, model), alg, precision="f32")
+    b = runtime.Batch([s])
+    trace = b.run(3)[0]
+    te = np.abs(trace - ref["trace"]) / np.abs(ref["trace"])
+    e = rel(s.spectrum(), ref["object_spectrum"])
+    say(loss, "fused spectrum", e, "trace", te.max())
+    assert e <= 1e-5 and te.max() <= 1e-5
+    b.close()
