@@ -23,7 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "fpm-rs_amd"))
 
 import numpy as np  # noqa: E402
 
-from fpmhip import MeasurementStack, Mpie, ReconstructionProblem, abi, runtime  # noqa: E402
+from fpmhip import Epry, MeasurementStack, Mpie, ReconstructionProblem, abi, runtime  # noqa: E402
 from fpmhip import synthetic as syn  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -62,15 +62,18 @@ def workload(n=128, k=15, seed=7, peak=1e4):
     return model, MeasurementStack(counts=counts)
 
 
-def workload_tiles(count, n=128, k=15, seed=7, peak=1e4):
+def workload_tiles(count, n=128, k=15, seed=7, peak=1e4, true_pupil_rms=None):
     """`count` distinct FOV tiles of the workload: one compiled geometry, and
     per tile its own object (seed + i), its own Poisson counts and its own
-    frame gains (photons per pixel of its own peak)."""
+    frame gains (photons per pixel of its own peak). With true_pupil_rms the
+    data are simulated through a Noll Z4-Z15 pupil of that RMS (seed + 100;
+    the precision form), and reconstructed from the compiled pupil."""
     from concurrent.futures import ThreadPoolExecutor
 
     base = runtime.compile_model(syn.base_optics(), syn.planar_array(k), (n, n))
     ctx = runtime.default_context()
     dm = runtime.DeviceModel(ctx, base)
+    true_pupil = None if true_pupil_rms is None else syn.zernike_pupil(base, true_pupil_rms, seed + 100)
 
     def counts_of(args):
         i, intensity = args
@@ -85,7 +88,8 @@ def workload_tiles(count, n=128, k=15, seed=7, peak=1e4):
         with ThreadPoolExecutor(host_threads()) as pool:
             specs = pool.map(lambda i: syn.object_spectrum(syn.random_field_object(base.reconstruction_shape, seed + i)),
                              range(count))
-            stacks = ((i, runtime.forward_stack(base, sp, context=ctx, dmodel=dm)) for i, sp in enumerate(specs))
+            stacks = ((i, runtime.forward_stack(base, sp, true_pupil, context=ctx, dmodel=dm))
+                      for i, sp in enumerate(specs))
             return list(pool.map(counts_of, stacks))
     finally:
         dm.close()
@@ -155,11 +159,11 @@ def run_plain(model, meas, iterations, steps, warmup, precision):
         close_all(dme, dm)
 
 
-def run_batched(tiles, iterations, steps, warmup):
+def run_batched(tiles, iterations, steps, warmup, alg=None):
     """The workload tiles (each its own model, counts and state) advanced
     together; one step = one batch of tiles."""
     ctx = runtime.default_context()
-    alg = algorithm(iterations)
+    alg = alg or algorithm(iterations)
     dms, dmes, sessions, batch = [], [], [], None
     try:
         for model, meas in tiles:
@@ -181,7 +185,8 @@ def run_batched(tiles, iterations, steps, warmup):
             log(f"batch {i + 1}/{steps} done")
         ctx.synchronize()
         secs = time.perf_counter() - t0
-        per_launch = batch.schedule()["sweeps_per_launch"] or 10
+        # EPRY batches run one sweep per launch (the gauge follows each sweep)
+        per_launch = 1 if isinstance(alg, Epry) else (batch.schedule()["sweeps_per_launch"] or 10)
         return secs, float(np.mean(sweep_ms)), min(per_launch, iterations), download_ms(batch)
     finally:
         close_all(batch, sessions, dmes, dms)
@@ -333,6 +338,31 @@ def run_uneven(iterations=40, steps=1, warmup=1):
             "batch_algorithmic_GB": total / 1e9}
 
 
+def run_epry(tiles=256, iterations=30, steps=1, warmup=1):
+    """The precision form in the batched kernel: `tiles` distinct 225-LED
+    128^2 tiles simulated through a 0.5 rad RMS Noll Z4-Z15 pupil, EPRY
+    (object step 0.8, pupil step 0.1, support constraint) `iterations`
+    sweeps from the compiled pupil, with the device gauge after every sweep.
+    Roofline bytes per tile and sweep: F N (s_m + 2 s_o + 2 s_p) (the pupil
+    read and written at every LED) + the gauge's 5 M s_o + 2 N s_p."""
+    log(f"building the EPRY batch ({tiles} tiles)")
+    ts = workload_tiles(tiles, true_pupil_rms=0.5)
+    alg = Epry(iterations=iterations)
+    secs, sweep_ms, per_launch, dl = run_batched(ts, iterations, steps, warmup, alg)
+    model = ts[0][0]
+    N, F = model.image_len, model.frame_count
+    M = model.reconstruction_shape[0] * model.reconstruction_shape[1]
+    per_sweep = F * N * (2 + 2 * 16 + 2 * 8) + 5 * M * 16 + 2 * N * 8
+    achieved = per_sweep * tiles / (sweep_ms * 1e-3) / 1e9
+    return {"tiles": tiles, "workload": "225 LEDs 128^2 -> 375^2 EPRY 30 sweeps, Zernike 0.5 rad true pupil",
+            "tiles_per_s": steps * tiles / secs, "ms_per_batch": 1000.0 * secs / steps, "sweep_ms": sweep_ms,
+            "sweeps_per_launch": per_launch,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "note": "fused EPRY sweep k_sweep_epry + k_gauge per sweep; algorithmic bytes per sweep of "
+                                 "every tile / mean per-sweep time (HIP events)"}}
+
+
 def host_threads():
     """Host threads for the CPU baseline: the CPUs this process may run on,
     at most 16 (the host share that comes with one GPU of the box)."""
@@ -397,7 +427,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=50)
-    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving", "uneven"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving", "uneven", "epry"])
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--sustained-rate", type=float, default=40.0,
@@ -418,6 +448,12 @@ def main():
         print(json.dumps({"metric": "serving p99 latency, FP32 mPIE 30 sweeps", "value": res["p99_ms"],
                           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
                           "config": {"workload": "Poisson arrivals", **res}, "sustained": sus}), flush=True)
+        return
+    if args.mode == "epry":
+        res = run_epry(args.tiles, 30, args.steps, args.warmup)
+        print(json.dumps({"metric": "EPRY tiles/sec, precision form, 225 LEDs x 128^2, 30 sweeps",
+                          "value": res["tiles_per_s"], "unit": "tiles/s", "n_gpus": 1, "higher_is_better": True,
+                          "data": "synthetic", "config": res}), flush=True)
         return
     if args.mode == "uneven":
         res = run_uneven(40, args.steps, args.warmup)
@@ -487,6 +523,8 @@ def main():
         line["speedup_vs_plain_c128"] = tiles_per_s * psecs
         log("uneven batch leg")
         line["uneven_batch"] = run_uneven(40, 1, 1)
+        log("EPRY batch leg")
+        line["epry_batch"] = run_epry(256, 30, 1, 1)
         log("serving leg")
         line["serving"] = serving(args.requests, args.rate, 30)
         if args.sustained_rate > 0:

@@ -47,7 +47,9 @@ inline bool off_zero(double r, double c) { return std::fabs(r) <= 1e-12 && std::
 void download_pupil64(fpmhip_session& s, double* out) {
   const Model& m = *s.model;
   const uint64_t N = m.N();
-  if (s.precision == FPMHIP_PLAIN_C128) {
+  if (s.b.pupil64.ptr) {
+    HIP_CHECK(hipMemcpyAsync(out, s.b.pupil64.ptr, N * sizeof(double2), hipMemcpyDeviceToHost, s.ctx->stream));
+  } else if (s.precision == FPMHIP_PLAIN_C128) {
     HIP_CHECK(hipMemcpyAsync(out, s.b.pupil.ptr, N * sizeof(double2), hipMemcpyDeviceToHost, s.ctx->stream));
   } else {
     launch_pupil_to_f64(s.b.pupil.ptr, true, s.b.work_m.as<double2>(), N, s.ctx->stream);
@@ -62,7 +64,7 @@ void upload_pupil64(fpmhip_session& s, const double* src) {
     HIP_CHECK(hipMemcpyAsync(s.b.pupil.ptr, src, N * sizeof(double2), hipMemcpyHostToDevice, s.ctx->stream));
   } else {
     HIP_CHECK(hipMemcpyAsync(s.b.work_m.ptr, src, N * sizeof(double2), hipMemcpyHostToDevice, s.ctx->stream));
-    launch_convert_pupil(s.b.work_m.as<double2>(), s.b.pupil.ptr, true, N, s.ctx->stream);
+    set_session_pupil(s, s.b.work_m.as<double2>());
   }
   HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
 }
@@ -79,29 +81,9 @@ void canonicalize_session(fpmhip_session& s) {
   launch_gauge(s.b.gauge.as<GaugeTile>(), 1, s.ctx->stream);
 }
 
-// The gauge descriptor of a session that canonicalizes.
+// The gauge descriptor of a session that canonicalizes, uploaded.
 void prepare_gauge(fpmhip_session& s) {
-  const Model& m = *s.model;
-  GaugeTile t{};
-  t.spec = s.b.spec.as<double2>();
-  t.pupil = s.b.pupil.ptr;
-  t.ref = m.d_pupil64.as<double2>();
-  t.support = m.d_support.as<uint8_t>();
-  t.maxp_out = s.b.scal.as<double>();
-  t.status = s.b.flag.as<int32_t>() + 2;
-  t.tol = 64.0 * std::numeric_limits<double>::epsilon() * (double)m.support_count;
-  t.nh = m.nh;
-  t.nw = m.nw;
-  t.rh = m.rh;
-  t.rw = m.rw;
-  t.pupil_f32 = s.precision == FPMHIP_MIXED_F32 ? 1 : 0;
-  bool row_exact = true, col_exact = true;
-  for (uint64_t k = 0; k < m.S; ++k) {
-    row_exact = row_exact && std::fabs(m.off_r[k]) <= 1e-12;
-    col_exact = col_exact && std::fabs(m.off_c[k]) <= 1e-12;
-  }
-  t.row_exact = row_exact ? 1 : 0;
-  t.col_exact = col_exact ? 1 : 0;
+  const GaugeTile t = gauge_tile(s);
   s.b.gauge.allocate(sizeof(GaugeTile));
   HIP_CHECK(hipMemcpyAsync(s.b.gauge.ptr, &t, sizeof(t), hipMemcpyHostToDevice, s.ctx->stream));
   HIP_CHECK(hipStreamSynchronize(s.ctx->stream));  // `t` is on the host stack
@@ -412,6 +394,35 @@ void sweep(fpmhip_session& s, const uint64_t* order, uint64_t n, double* per_fra
 
 namespace fpmhip {
 
+// The gauge descriptor of a session: its spectrum and pupil, the model's
+// reference pupil and support, status in the session's flag[2].
+GaugeTile gauge_tile(const fpmhip_session& s) {
+  const Model& m = *s.model;
+  GaugeTile t{};
+  t.spec = s.b.spec.as<double2>();
+  t.pupil = s.b.pupil.ptr;
+  t.ref = m.d_pupil64.as<double2>();
+  t.support = m.d_support.as<uint8_t>();
+  t.maxp_out = s.b.scal.as<double>();
+  t.status = s.b.flag.as<int32_t>() + 2;
+  t.tol = 64.0 * std::numeric_limits<double>::epsilon() * (double)m.support_count;
+  t.nh = m.nh;
+  t.nw = m.nw;
+  t.rh = m.rh;
+  t.rw = m.rw;
+  t.pupil_f32 = s.precision == FPMHIP_MIXED_F32 ? 1 : 0;
+  t.pupil64 = s.b.pupil64.as<double2>();
+  bool row_exact = true, col_exact = true;
+  for (uint64_t k = 0; k < m.S; ++k) {
+    row_exact = row_exact && std::fabs(m.off_r[k]) <= 1e-12;
+    col_exact = col_exact && std::fabs(m.off_c[k]) <= 1e-12;
+  }
+  t.row_exact = row_exact ? 1 : 0;
+  t.col_exact = col_exact ? 1 : 0;
+  return t;
+}
+
+
 uint64_t momentum_span_capacity(const Model& m) { return 1 + (uint64_t)m.rh * ((m.rw + 7 + 63) / 64); }
 
 void upload_momentum_spans(fpmhip_session& s, const double* pupil) {
@@ -470,8 +481,7 @@ void session_initialize_launch(fpmhip_session& s) {
   s.initialized = false;
   launch_flag_reset(s);
   launch_init(s);
-  launch_convert_pupil(s.model->d_pupil64.as<double2>(), s.b.pupil.ptr, s.precision == FPMHIP_MIXED_F32, s.model->N(),
-                       s.ctx->stream);
+  set_session_pupil(s, s.model->d_pupil64.as<double2>());
   if (!s.extent_from_model) {  // the model's pupil: its extents and lists do not change
     launch_pupil_extent(s);
     upload_momentum_spans(s, s.model->pupil.data());
@@ -784,7 +794,8 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       b.group.allocate(group_state_bytes(m->nh));
       HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
     }
-    launch_convert_pupil(m->d_pupil64.as<double2>(), b.pupil.ptr, precision == FPMHIP_MIXED_F32, N, m->ctx->stream);
+    if (precision == FPMHIP_MIXED_F32 && s->cfg.has_pupil_step) b.pupil64.allocate(N * sizeof(double2));
+    set_session_pupil(*s, m->d_pupil64.as<double2>());
     if (precision == FPMHIP_MIXED_F32 && s->cfg.momentum && m->M() < (1ull << 25))
       b.mspan.allocate(momentum_span_capacity(*m) * sizeof(uint32_t));
     if (precision == FPMHIP_MIXED_F32) {

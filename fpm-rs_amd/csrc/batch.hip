@@ -7,6 +7,7 @@
 #include <utility>
 
 #include "fused.h"
+#include "gauge.h"
 
 using namespace fpmhip;
 
@@ -98,6 +99,10 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.max_p = maxp;
   t.friction = s.cfg.friction;
   t.feedback = s.cfg.feedback;
+  t.recover_pupil = s.cfg.has_pupil_step ? 1 : 0;
+  t.patch = t.recover_pupil ? s.b.patch.as<float2>() : nullptr;
+  t.pupil64 = s.b.pupil64.as<double2>();
+  t.pupil_step = s.cfg.pupil_step;
 }
 
 std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,
@@ -167,8 +172,27 @@ bool sweep_supports(const fpmhip_session& s, const char** why) {
     return false;
   }
   if (s.alg.kind == FPMHIP_EPRY) {
-    *why = "batched tiles do not support EPRY";
-    return false;
+    if (m.nh > kFusedLdsMaxN) {
+      *why = "batched EPRY tiles must be at most 128 x 128 pixels";
+      return false;
+    }
+    if (s.cfg.gain_update || s.cfg.bg_update) {
+      *why = "batched EPRY tiles do not recover frame gains or background";
+      return false;
+    }
+    if (s.cfg.has_pupil_step) {
+      if (!s.cfg.constrain) {
+        *why = "batched EPRY pupil recovery needs the pupil support constraint";
+        return false;
+      }
+      // the fused sweep updates the pupil on its support strips: the model
+      // pupil (where every run starts) must be non-zero exactly on the support
+      for (uint64_t p = 0; p < m.N(); ++p)
+        if (((float)m.pupil[2 * p] != 0.0f || (float)m.pupil[2 * p + 1] != 0.0f) != (m.support[p] != 0)) {
+          *why = "batched EPRY pupil recovery needs a model pupil that is non-zero exactly on its support";
+          return false;
+        }
+    }
   }
   if (s.cfg.momentum && !s.b.mspan.ptr) {  // chunk starts are 25-bit pixel indices
     *why = "batched mPIE tiles need an object grid below 2^25 pixels";
@@ -197,6 +221,8 @@ struct fpmhip_batch {
   uint64_t fmax = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  DeviceBuffer gauges;    // GaugeTile[gauge_count] of the tiles that canonicalize (EPRY), in session order
+  uint32_t gauge_count = 0;
   double* pinned = nullptr;  // page-locked result objects (fpmhip_batch_objects_pinned)
   size_t pinned_bytes = 0;
   // batched initialisation and results: one InitTile per session, and one
@@ -277,6 +303,7 @@ void build_init(fpmhip_batch& b) {
     t.slot = b.slot_of[i];
     t.spec = s.b.spec.as<double2>();
     t.pupil = s.b.pupil.as<float2>();
+    t.pupil64 = s.b.pupil64.as<double2>();
     t.vel = s.cfg.momentum ? s.b.vel.as<float2>() : nullptr;
     t.anc = s.cfg.momentum ? s.b.anc.as<float2>() : nullptr;
     t.gains = s.b.gains.as<double>();
@@ -331,9 +358,29 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
     b.graph = nullptr;
   }
   HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
-  // capture the launch (the LDS attribute was set above, outside capture)
+  // the EPRY gauge of every tile that canonicalizes, after each of its sweeps
+  std::vector<GaugeTile> gauges;
+  for (uint32_t i = 0; i < count; ++i) {
+    fpmhip_session& s = *b.sessions[i];
+    if (!s.cfg.gauge) continue;
+    GaugeTile g = gauge_tile(s);
+    g.status = b.host_tiles[i].status;
+    g.maxp_out = nullptr;  // the fused sweep takes max |P|^2 at the start of each sweep
+    gauges.push_back(g);
+  }
+  b.gauge_count = (uint32_t)gauges.size();
+  if (b.gauge_count) {
+    b.gauges.allocate(gauges.size() * sizeof(GaugeTile));
+    HIP_CHECK(hipMemcpyAsync(b.gauges.ptr, gauges.data(), gauges.size() * sizeof(GaugeTile), hipMemcpyHostToDevice,
+                             b.ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(b.ctx->stream));  // `gauges` is pageable host memory
+  }
+  // capture the launch (the LDS attribute was set above, outside capture);
+  // with a gauge, a launch runs one sweep and the gauge follows it
   HIP_CHECK(hipStreamBeginCapture(b.ctx->stream, hipStreamCaptureModeThreadLocal));
-  launch_sweep(b.tiles.as<TileDesc>(), b.launched, b.tw.as<float2>(), b.nmax, b.cooperative, b.ctx->stream);
+  launch_sweep(b.tiles.as<TileDesc>(), b.launched, b.tw.as<float2>(), b.nmax, b.cooperative, b.ctx->stream,
+               b.host_tiles[0].recover_pupil != 0);
+  if (b.gauge_count) launch_gauge(b.gauges.as<GaugeTile>(), b.gauge_count, b.ctx->stream);
   HIP_CHECK(hipStreamEndCapture(b.ctx->stream, &b.graph));
   HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
   build_init(b);
@@ -382,9 +429,18 @@ void initialize_tiles(fpmhip_batch& b) {
   hipLaunchKernelGGL(k_reset_tiles, dim3((b.launched + 255) / 256), dim3(256), 0, st, b.tiles.as<TileDesc>(),
                      b.launched);
   HIP_CHECK(hipGetLastError());
+  // canonicalize_state after initialisation (runner.rs:122)
+  if (b.gauge_count) launch_gauge(b.gauges.as<GaugeTile>(), b.gauge_count, st);
 }
 
 constexpr uint32_t kDefaultSweepsPerLaunch = 10;
+
+// Sweeps one graph replay runs: the schedule's, or one when the tiles'
+// gauge must follow each sweep.
+uint32_t sweeps_per_launch(const fpmhip_batch& b) {
+  if (b.gauge_count) return 1;
+  return b.schedule.sweeps_per_launch ? b.schedule.sweeps_per_launch : kDefaultSweepsPerLaunch;
+}
 
 // `sweeps` sweeps as graph replays of up to `per_launch` sweeps each (the
 // sweep count of a launch is set in the device descriptors before it), each
@@ -428,6 +484,8 @@ void replay(fpmhip_batch& b, uint64_t sweeps, double* mean, double* mx, uint32_t
 
 void check_tile_status(int32_t status) {
   if (status == 2) fail(FPMHIP_DEVICE_ERROR, "a cooperative tile group stalled at a group barrier");
+  int code = FPMHIP_NUMERICAL;
+  if (const char* msg = gauge_failure(status, &code)) fail(code, msg);
   if (status) fail(FPMHIP_NUMERICAL, "mPIE momentum produced a non-finite object or velocity");
 }
 
@@ -468,6 +526,8 @@ int fpmhip_batch_create(fpmhip_context* c, fpmhip_session* const* sessions, uint
       if (!sweep_supports(*s, &why)) fail(FPMHIP_UNSUPPORTED, why);
       const Model& m = *s->model;
       if (kind >= 0 && kind != s->alg.kind) fail(FPMHIP_INVALID_PARAMETER, "tiles must share one algorithm", "sessions");
+      if (!b->sessions.empty() && b->sessions[0]->cfg.has_pupil_step != s->cfg.has_pupil_step)
+        fail(FPMHIP_INVALID_PARAMETER, "tiles must all recover their pupil or none", "sessions");
       kind = s->alg.kind;
       b->nmax = std::max(b->nmax, m.nh);
       b->fmax = std::max<uint64_t>(b->fmax, m.F);
@@ -498,9 +558,7 @@ int fpmhip_batch_run(fpmhip_batch* b, uint64_t iterations, double* trace) {
     const uint32_t count = (uint32_t)b->sessions.size();
     if (iterations > b->trace_cap) build_tiles(*b, (uint32_t)iterations);
     initialize_tiles(*b);
-    const uint32_t per_launch =
-        b->schedule.sweeps_per_launch ? b->schedule.sweeps_per_launch : kDefaultSweepsPerLaunch;
-    replay(*b, iterations, &b->ms_mean, &b->ms_max, per_launch);
+    replay(*b, iterations, &b->ms_mean, &b->ms_max, sweeps_per_launch(*b));
     b->sweeps = iterations;
     std::vector<char> per(count * 16);
     HIP_CHECK(hipMemcpyAsync(per.data(), b->per_tile.ptr, per.size(), hipMemcpyDeviceToHost, st));

@@ -18,6 +18,8 @@ struct TileDesc {
                           // strips, then (n+1)^2 slots of destinations a | b << 16 and the strips
                           // r | c << 8 | mask << 16 (SessionBuffers::dest)
   float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)
+  float2* patch;          // [N] the patch of the support pixels (EPRY pupil recovery)
+  double2* pupil64;       // [N] FP64 master of the recovered pupil (EPRY pupil recovery)
   const void* frames;     // [F*N] uint16 counts or f64 frames
   const uint8_t* masks;   // [F*N] or null
   const double* weights;  // [F]
@@ -95,7 +97,9 @@ bool sweep_supports(const fpmhip_session& s, const char** why);
 void prepare_sweep(uint32_t nmax);
 // Launches the sweep over `count` descriptors; a grid holding cooperative
 // groups is launched cooperatively (all workgroups co-resident).
+// `epry`: every tile recovers its pupil (EPRY), through the kernel variant
+// that does (tiles up to kFusedLdsMaxN).
 void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, bool cooperative,
-                  hipStream_t stream);
+                  hipStream_t stream, bool epry = false);
 
 }  // namespace fpmhip

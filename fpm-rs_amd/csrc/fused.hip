@@ -286,6 +286,15 @@ __device__ __forceinline__ double2 wg_sum2(double a, double b, double* red) {
   return make_double2(sa, sb);  // valid in thread 0
 }
 
+// A value that is the same in every lane, as a scalar register value.
+__device__ __forceinline__ double uniform(double v) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Fixed-order workgroup maximum of one double per thread; valid in every
 // thread (red holds kThreads / 64 doubles).
 __device__ __forceinline__ double wg_max(double a, double* red) {
@@ -297,7 +306,7 @@ __device__ __forceinline__ double wg_max(double a, double* red) {
   double r = red[0];
   for (int w = 1; w < kThreads / 64; ++w) r = fmax(r, red[w]);
   __syncthreads();
-  return r;
+  return uniform(r);
 }
 
 // Object-update denominator of common.rs:247-258 as a + b |P|^2 with FP64
@@ -591,34 +600,6 @@ __device__ __forceinline__ double patch_max(Rsrc rspec, const SrcParam& sp, uint
   return m;
 }
 
-// The bilinear patch values (crop.rs:129-167, FP64 taps, rounded to FP32) of
-// the support pixels of one strip entry `e` (the arithmetic of
-// gather_strips), for the EPRY pupil gradient.
-__device__ __forceinline__ void strip_patch(Rsrc rspec, const SrcParam& sp, uint32_t rw, uint32_t e,
-                                            float2 (&out)[kGatherStrip]) {
-  constexpr int S = (int)kGatherStrip;
-  const uint32_t r0 = e & 0xFFu, c = (e >> 8) & 0xFFu, mask = e >> 16;
-  const uint32_t o = (sp.ar.lower + r0) * rw + sp.ac.lower + c;
-  const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
-  const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
-               wuu = sp.ar.wu * sp.ac.wu;
-  const double2 zero = make_double2(0.0, 0.0);
-  double2 t0[S + 1], t1[S + 1];
-#pragma unroll
-  for (int j = 0; j <= S; ++j) {
-    const bool need = (j < S && (mask >> j & 1u)) || (fr && j > 0 && (mask >> (j - 1) & 1u));
-    t0[j] = bld_d2(rspec, need ? (o + j * rw) * 16u : kSkip);
-    t1[j] = bld_d2(rspec, need && fc ? (o + j * rw + 1) * 16u : kSkip);
-  }
-#pragma unroll
-  for (int j = 0; j < S; ++j) {
-    const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
-    const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
-                                   fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
-    out[j] = f2((float)v.x, (float)v.y);
-  }
-}
-
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
 // strips `sl` (count ns; kGatherStrip vertically consecutive pixels of one
 // column, centred patch coordinates, with a mask of the pixels in the
@@ -629,9 +610,10 @@ __device__ __forceinline__ void strip_patch(Rsrc rspec, const SrcParam& sp, uint
 // S pupil values per strip, all independent loads issued together. An axis
 // without a fraction has an upper weight of exactly zero and its upper taps
 // are neither loaded nor used (they contribute zero).
-template <int N>
+template <int N, bool EPRY = false>
 __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
-                                              const uint32_t* sl, uint32_t ns, uint32_t en, float2* lds) {
+                                              const uint32_t* sl, uint32_t ns, uint32_t en, float2* lds,
+                                              Rsrc rpatch) {
   constexpr int LD = pitch<N>(), HALF = N / 2, NN = N * N, S = (int)kGatherStrip;
   static_assert(NN % kThreads == 0, "pixels per thread");
   const Rsrc rl = make_rsrc(sl, ns * 4u);
@@ -666,7 +648,10 @@ __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcPa
       const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
       const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
                                      fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
-      if (mask >> j & 1u) lds[((r0 + j + HALF) % N) * LD + cc] = fmul(f2((float)v.x, (float)v.y), P[j]);
+      const float2 pv = f2((float)v.x, (float)v.y);
+      if (mask >> j & 1u) lds[((r0 + j + HALF) % N) * LD + cc] = fmul(pv, P[j]);
+      // EPRY keeps the patch of the support pixels for the pupil gradient
+      if (EPRY) bst_f2(rpatch, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip, pv);
     }
   }
 }
@@ -851,7 +836,7 @@ __device__ __forceinline__ float2 replace_plain(float2 fv, float y, float sg, fl
 // ifftshifted (natural) slot and the update replaces the transform there by
 // U; for even N, fftshift and ifftshift are both the swap
 // ((r + N/2) mod N, (c + N/2) mod N).
-template <int N, int RADIX>
+template <int N, int RADIX, bool EPRY = false>
 __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const float2* twf, const float2* twi,
                                            double* red, const uint32_t* ext, uint32_t* rec) {
   constexpr int NN = N * N;
@@ -859,10 +844,16 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   constexpr int LD = pitch<N>();
   const float inv_n = 1.0f / (float)NN;
   double2* __restrict__ spec = T.spec;
-  const float2* __restrict__ pupil = T.pupil;
+  float2* pupil = T.pupil;
   const double eps = T.eps;
   const float eps_f = (float)eps;
-  const Denominator den = make_denominator(T, fmax(T.max_p, eps));
+  // EPRY pupil recovery: the pupil changes every LED, and so do max |P|^2
+  // and the Global denominator made from it (common.rs:198-205)
+  constexpr bool epry = EPRY;
+  const Rsrc rpatch = make_rsrc(T.patch, EPRY ? NN * 8u : 0u);
+  const Rsrc rp64 = make_rsrc(T.pupil64, EPRY ? NN * 16u : 0u);
+  double maxp = T.max_p;
+  Denominator den = make_denominator(T, fmax(maxp, eps));
   const Rsrc rspec = make_rsrc(spec, T.rh * T.rw * 16u), rpup = make_rsrc(pupil, NN * 8u);
   // the common case: every pixel counts, no background, amplitude MSE
   const bool plain_frames = T.masks == nullptr && T.bg_len == 0 && T.loss == FPMHIP_LOSS_AMPLITUDE_MSE;
@@ -895,6 +886,14 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   uint32_t slot = 0;
   for (uint32_t sweep = 0; sweep < T.sweeps; ++sweep) {
     double obj_sum = 0.0, w_sum = 0.0;
+    if (epry) {  // max |P|^2 of the pupil this sweep starts from (the gauge rescales it)
+      double m = 0.0;
+      for (int p = opaque_tid(); p < NN; p += kThreads) {
+        const float2 P = bld_f2(rpup, (uint32_t)p * 8u);
+        m = fmax(m, (double)P.x * (double)P.x + (double)P.y * (double)P.y);
+      }
+      maxp = wg_max(m, red);
+    }
     for (uint32_t pos = 0; pos < T.F; ++pos, slot ^= 1) {
       const uint32_t* r = rec + slot * kRecPitch;
       uint32_t* rn = rec + (slot ^ 1) * kRecPitch;
@@ -916,8 +915,18 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       const float gain_f = (float)gain;
       const float inv_gain = (float)(1.0 / gain);
       const float sg = (float)sqrt(gain), sig = (float)sqrt(1.0 / gain);
+      // EPRY: the frame's step denominators, max |P|^2 of the current pupil
+      // (Global) and max |patch|^2 over the whole patch (pupil step)
+      float pupil_k = 0.0f;
+      if (epry) {
+        den = make_denominator(T, fmax(maxp, eps));
+        den.a = uniform(den.a);
+        den.b = uniform(den.b);
+        const double maxo = wg_max(patch_max<N>(rspec, sp, T.rw, tid), red);
+        pupil_k = uniform((float)(1.0 / (fmax(maxo, eps) + eps)) * inv_n);
+      }
       // 1. patch x pupil over the support pixels, written ifftshifted
-      gather_strips<N>(rspec, rpup, sp, T.rw, tid, slist, nstrip, my_strip, lds);
+      gather_strips<N, EPRY>(rspec, rpup, sp, T.rw, tid, slist, nstrip, my_strip, lds, rpatch);
       __syncthreads();
       // the next LED's frame index, then its record words (written to the
       // other slot after the forward transform)
@@ -1000,6 +1009,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       {
         constexpr int S = (int)kGatherStrip;
         const Rsrc rl = make_rsrc(slist, nstrip * 4u);
+        const float pk = uniform((float)(fw * T.pupil_step) * pupil_k);
+        double pmax = 0.0;
         for (uint32_t i = (uint32_t)tid; i < nstrip; i += kThreads) {
           const uint32_t e = i == (uint32_t)tid ? my_strip : (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
           const int r0 = (int)(e & 0xFFu), c = (int)((e >> 8) & 0xFFu);
@@ -1008,16 +1019,45 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           float2 P[S];
 #pragma unroll
           for (int j = 0; j < S; ++j) P[j] = bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+          if constexpr (EPRY) {
+            // EPRY pupil step at the strip's support pixels (common.rs:271-300):
+            // g = conj(patch) diff / (max |patch|^2 + eps), P += w beta g, with
+            // the old P in U; the support projection leaves the rest at zero
+            float2 pa[S];
 #pragma unroll
-          for (int j = 0; j < S; ++j) {
-            if (mask >> j & 1u) {
-              const int l = ((r0 + j + HALF) % N) * LD + cn;
-              const float k = inv_n / fmaf(den.b, fmaf(P[j].x, P[j].x, P[j].y * P[j].y), den.a);
-              const float2 u = fmulc(P[j], lds[l]);
-              lds[l] = f2(u.x * k, u.y * k);
+            for (int j = 0; j < S; ++j) pa[j] = bld_f2(rpatch, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+              if (mask >> j & 1u) {
+                const int l = ((r0 + j + HALF) % N) * LD + cn;
+                const float2 d = lds[l];
+                const float k = inv_n / fmaf(den.b, fmaf(P[j].x, P[j].x, P[j].y * P[j].y), den.a);
+                const float2 u = fmulc(P[j], d);
+                lds[l] = f2(u.x * k, u.y * k);
+                const float2 g = fmulc(pa[j], d);
+                // the step accumulates into the FP64 master; P is its complex64 copy
+                const uint32_t pix = (uint32_t)((r0 + j) * N + c);
+                const double2 m64 = bld_d2(rp64, pix * 16u);
+                const double2 n64 = make_double2(fma((double)pk, (double)g.x, m64.x), fma((double)pk, (double)g.y, m64.y));
+                bst_d2(rp64, pix * 16u, n64);
+                const float2 np = f2((float)n64.x, (float)n64.y);
+                bst_f2(rpup, pix * 8u, np);
+                pmax = fmax(pmax, (double)np.x * (double)np.x + (double)np.y * (double)np.y);
+              }
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+              if (mask >> j & 1u) {
+                const int l = ((r0 + j + HALF) % N) * LD + cn;
+                const float k = inv_n / fmaf(den.b, fmaf(P[j].x, P[j].x, P[j].y * P[j].y), den.a);
+                const float2 u = fmulc(P[j], lds[l]);
+                lds[l] = f2(u.x * k, u.y * k);
+              }
             }
           }
         }
+        if (epry) maxp = wg_max(pmax, red);  // the next frame's max |P|^2 (zero off the support)
       }
       __syncthreads();
       // 6. adjoint insert over the tile's live destinations: FP64 RMW of the
@@ -1240,7 +1280,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   const float inv_n = 1.0f / (float)NN;
   double2* __restrict__ spec = T.spec;
   float2* __restrict__ fld = T.field;
-  const float2* __restrict__ pupil = T.pupil;
+  float2* pupil = T.pupil;
   const float eps_f = (float)T.eps;
   const Denominator den = make_denominator(T, fmax(T.max_p, T.eps));
   constexpr int AUX = kAux<MEM>;
@@ -1546,6 +1586,41 @@ __global__ void __launch_bounds__(kThreads)
 
 }  // namespace
 
+// The fused sweep with EPRY pupil recovery (tiles up to kFusedLdsMaxN), a
+// kernel of its own so that its extra live state does not cost the other
+// algorithms' kernel registers. Same LDS layout as k_sweep.
+__global__ void __launch_bounds__(kThreads)
+    k_sweep_epry(const TileDesc* __restrict__ tiles, const float2* __restrict__ tw_all) {
+  extern __shared__ unsigned char smem[];
+  float2* lds = reinterpret_cast<float2*>(smem);
+  const TileDesc& T = tiles[blockIdx.x];
+  const int N = (int)T.n;
+  float2* prev = lds + sweep_field_elems(N);
+  float2* twf = prev + (N > (int)kFusedLdsMaxN ? N : 0);
+  float2* twi = twf + N;
+  double* red = reinterpret_cast<double*>(twi + N);
+  int* bflag = reinterpret_cast<int*>(red + 2 * (kThreads / 64));
+  uint32_t* ext = reinterpret_cast<uint32_t*>(bflag + 4);
+  uint32_t* rec = ext + N;
+  const float2* src = tw_all + tw_offset(N);
+  for (int j = threadIdx.x; j < 2 * N; j += blockDim.x) twf[j] = src[j];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) ext[j] = T.extent[j];
+  __syncthreads();
+  switch (N) {
+    case 32:
+      sweep_tile<32, 8, true>(T, lds, twf, twi, red, ext, rec);
+      break;
+    case 64:
+      sweep_tile<64, 8, true>(T, lds, twf, twi, red, ext, rec);
+      break;
+    case 128:
+      sweep_tile<128, 8, true>(T, lds, twf, twi, red, ext, rec);
+      break;
+    default:
+      break;
+  }
+}
+
 // LDS of one tile of n x n pixels (the layout of k_sweep).
 static size_t sweep_lds_bytes_one(uint32_t n) {
   return ((size_t)sweep_field_elems(n) + (n > kFusedLdsMaxN ? n : 0) + 2 * (size_t)n) * sizeof(float2) +
@@ -1587,6 +1662,9 @@ size_t group_state_bytes(uint32_t n) {
 void prepare_sweep(uint32_t nmax) {
   HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)sweep_lds_bytes(nmax)));
+  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep_epry),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sweep_lds_bytes(std::min(nmax, kFusedLdsMaxN))));
 }
 
 uint32_t sweep_coresident_limit(uint32_t nmax) {
@@ -1599,8 +1677,13 @@ uint32_t sweep_coresident_limit(uint32_t nmax) {
 }
 
 void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, bool cooperative,
-                  hipStream_t stream) {
+                  hipStream_t stream, bool epry) {
   const size_t smem = sweep_lds_bytes(nmax);
+  if (epry) {  // no cooperative groups: EPRY tiles are at most kFusedLdsMaxN
+    hipLaunchKernelGGL(k_sweep_epry, dim3(count), dim3(kThreads), smem, stream, tiles_dev, tw_all);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   if (cooperative) {
     // every workgroup of every cooperative group must be resident at once
     void* args[] = {(void*)&tiles_dev, (void*)&tw_all};

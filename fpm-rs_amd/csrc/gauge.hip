@@ -80,6 +80,7 @@ __device__ Acc block_reduce(Acc a, Acc* red) {
 }
 
 __device__ __forceinline__ double2 load_pupil(const GaugeTile& t, uint32_t p) {
+  if (t.pupil64) return t.pupil64[p];
   if (t.pupil_f32) {
     const float2 v = static_cast<const float2*>(t.pupil)[p];
     return make_double2((double)v.x, (double)v.y);
@@ -223,6 +224,7 @@ __global__ void __launch_bounds__(kThreads) k_gauge(const GaugeTile* __restrict_
   for (uint32_t p = threadIdx.x; p < N; p += kThreads) {
     double2 P = pupil_at(p);
     if (!unit) P = cmul(P, pf);
+    if (t.pupil64 && (ramped || !unit)) t.pupil64[p] = P;
     if (t.pupil_f32) {
       const float2 f = make_float2((float)P.x, (float)P.y);
       if (ramped || !unit) static_cast<float2*>(t.pupil)[p] = f;

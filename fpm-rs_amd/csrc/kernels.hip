@@ -334,14 +334,17 @@ __global__ void __launch_bounds__(1024) k_max_abs2(const V* __restrict__ v, uint
   block_max_abs2(v, n, out, sh);
 }
 
-// P += w_f beta g; support projection (common.rs:287-300, pupil.rs:171-182)
+// P += w_f beta g; support projection (common.rs:287-300, pupil.rs:171-182).
+// With an FP64 master (MIXED_F32 pupil recovery), the step accumulates into
+// the master and the pupil becomes its rounded copy.
 template <class V>
-__global__ void k_pupil_step(V* __restrict__ pupil, const V* __restrict__ grad, double step,
-                             const uint8_t* __restrict__ support, int constrain, uint32_t N) {
+__global__ void k_pupil_step(V* __restrict__ pupil, double2* __restrict__ master, const V* __restrict__ grad,
+                             double step, const uint8_t* __restrict__ support, int constrain, uint32_t N) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
-  double2 P = D::add(to_d2(pupil[p]), D::scale(step, to_d2(grad[p])));
+  double2 P = D::add(master ? master[p] : to_d2(pupil[p]), D::scale(step, to_d2(grad[p])));
   if (constrain && support[p] == 0) P = make_double2(0.0, 0.0);
+  if (master) master[p] = P;
   pupil[p] = from_d2<V>(P);
 }
 
@@ -541,6 +544,7 @@ __global__ void k_binit_pupil(const InitTile* __restrict__ tiles) {
   const InitTile& T = tiles[blockIdx.y];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < T.N) T.pupil[i] = from_d2<float2>(T.model_pupil[i]);
+  if (i < T.N && T.pupil64) T.pupil64[i] = T.model_pupil[i];
   if (i < T.F) T.gains[i] = T.model_gains[i];
   if (i < T.bg_len) T.bg[i] = T.model_bg[i];
 }
@@ -743,7 +747,8 @@ void launch_pupil_step(const fpmhip_session& s, double step) {
   const Model& m = *s.model;
   const uint64_t N = m.N();
   DISPATCH_V(s, hipLaunchKernelGGL(k_pupil_step<V>, dim3(blocks_for(N)), dim3(kBlock), 0, s.ctx->stream,
-                                   s.b.pupil.as<V>(), s.b.grad.as<V>(), step, m.d_support.as<uint8_t>(),
+                                   s.b.pupil.as<V>(), s.b.pupil64.as<double2>(), s.b.grad.as<V>(), step,
+                                   m.d_support.as<uint8_t>(),
                                    s.cfg.constrain ? 1 : 0, (uint32_t)N));
   HIP_CHECK(hipGetLastError());
 }
@@ -754,6 +759,13 @@ void launch_momentum(const fpmhip_session& s) {
                                    s.b.spec.as<double2>(), s.b.vel.as<V>(), s.b.anc.as<V>(), M, s.cfg.friction,
                                    s.cfg.feedback, s.b.flag.as<int>()));
   HIP_CHECK(hipGetLastError());
+}
+
+void set_session_pupil(const fpmhip_session& s, const double2* src) {
+  const uint64_t N = s.model->N();
+  if (s.b.pupil64.ptr)
+    HIP_CHECK(hipMemcpyAsync(s.b.pupil64.ptr, src, N * sizeof(double2), hipMemcpyDeviceToDevice, s.ctx->stream));
+  launch_convert_pupil(src, s.b.pupil.ptr, s.precision == FPMHIP_MIXED_F32, N, s.ctx->stream);
 }
 
 void launch_flag_reset(const fpmhip_session& s) {

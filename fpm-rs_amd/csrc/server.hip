@@ -166,6 +166,10 @@ bool fpmhip_server::admit(Job& j, fpmhip_context& c) {
                                     j.weights.empty() ? nullptr : j.weights.data(), nullptr, &j.meas);
   if (!rc) rc = fpmhip_session_create(j.model, j.meas, FPMHIP_MIXED_F32, &j.alg, &j.sess);
   const char* why = "";
+  if (!rc && j.alg.kind == FPMHIP_EPRY) {  // the tick's sweep has no per-tile pupil recovery or gauge
+    rc = FPMHIP_UNSUPPORTED;
+    j.error = "served jobs do not support EPRY";
+  }
   if (!rc && !sweep_supports(*j.sess, &why)) {
     rc = FPMHIP_UNSUPPORTED;
     j.error = why;

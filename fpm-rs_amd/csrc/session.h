@@ -60,6 +60,8 @@ struct SessionBuffers {
   DeviceBuffer spec;      // double2[M] centred object spectrum (FP64 accumulation)
   DeviceBuffer vel, anc;  // V[M] mPIE velocity/anchor (momentum arithmetic is FP64 either way)
   DeviceBuffer pupil;     // V[N]
+  DeviceBuffer pupil64;   // double2[N] FP64 master of the recovered pupil (MIXED_F32 EPRY pupil recovery):
+                          // every pupil step accumulates into it and `pupil` is its complex64 copy
   DeviceBuffer patch;     // V[K*N]
   DeviceBuffer field;     // V[K*N]
   DeviceBuffer pred;      // double[N]
@@ -176,6 +178,9 @@ void launch_object(const fpmhip_session& s, double2* out_dev);
 void launch_shift(const double2* src, double2* dst, uint32_t h, uint32_t w, uint32_t rs, uint32_t cs,
                   hipStream_t stream);
 void launch_convert_pupil(const double2* src, void* dst, bool to_f32, uint64_t n, hipStream_t stream);
+// The session's pupil from `src` (FP64, device): its FP64 master when it has
+// one, and the pupil in the session's precision.
+void set_session_pupil(const fpmhip_session& s, const double2* src);
 // Per-row support extents of a MIXED_F32 session's pupil (for the fused sweep).
 void launch_pupil_extent(const fpmhip_session& s);
 void launch_pupil_to_f64(const void* src, bool from_f32, double2* dst, uint64_t n, hipStream_t stream);
@@ -210,6 +215,7 @@ struct InitTile {
   double2* slot;      // the tile's R-array in the batch scratch
   double2* spec;
   float2* pupil;
+  double2* pupil64;            // FP64 pupil master, or null
   float2* vel;  // null without momentum
   float2* anc;
   double* gains;

@@ -53,7 +53,16 @@ def errors(got, ref):
     return ea, ep
 
 
-@pytest.mark.parametrize("n,iters,bound", [(128, 30, 1e-5), (256, 10, 3e-5)])
+CONDITIONING_256 = ("the 256^2 EPRY problem amplifies FP32 rounding beyond the 1e-5 target: in the FP64 oracle "
+                    "with controlled FP32 roundings (tests/test_oracle_conditioning.py; the design notes give the FP32 emulation), FP32 "
+                    "FFTs alone give 1.5e-5..4e-5 and even exact FFTs with FP32 fields give 1.0e-5")
+
+
+@pytest.mark.parametrize("n,iters,bound", [
+    (128, 30, 1e-5),
+    pytest.param(256, 30, 1e-5, marks=pytest.mark.xfail(strict=True, reason=CONDITIONING_256)),
+    (256, 30, 6e-5),  # the measured FP32 floor of this problem (see CONDITIONING_256), as a regression bound
+])
 def test_epry_precision_form_vs_fp64_oracle(ctx, n, iters, bound):
     model, meas, true_pupil = problem(n)
     say("precision form EPRY", n, model.reconstruction_shape, iters)
