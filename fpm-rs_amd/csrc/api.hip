@@ -526,6 +526,7 @@ int fpmhip_context_destroy(fpmhip_context* c) {
       (void)hipStreamSynchronize(c->stream);
       (void)hipStreamDestroy(c->stream);
     }
+    release_resident_plans(c);
     delete c;
   });
 }
@@ -890,6 +891,24 @@ int fpmhip_session_set_state(fpmhip_session* s, const double* spectrum, const do
     if (s->cfg.gauge) canonicalize_session(*s);
     check_flags(*s);
     s->initialized = true;
+  });
+}
+
+// ReconstructionState::from_object (state.rs:356-383): the state of
+// fpmhip_session_initialize with the object spectrum replaced by
+// fftshift(FFT(object) / M) of the given object, and the algorithm's
+// auxiliary state and gauge made from that spectrum.
+int fpmhip_session_initialize_from_object(fpmhip_session* s, const double* object) {
+  return guarded([&] {
+    require(s != nullptr && object != nullptr, "session", "null argument");
+    set_device(s->ctx);
+    const Model& m = *s->model;
+    session_initialize_launch(*s);
+    HIP_CHECK(hipMemcpyAsync(s->b.work_m.ptr, object, m.M() * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+    fft2_f64(*m.fft_high, s->b.work_m.as<double2>(), 1, false, s->ctx->stream);
+    launch_shift(s->b.work_m.as<double2>(), s->b.spec.as<double2>(), m.rh, m.rw, m.rh / 2, m.rw / 2, s->ctx->stream);
+    init_aux(*s);
+    session_initialize_check(*s);
   });
 }
 

@@ -107,6 +107,9 @@ struct Context {
   hipStream_t stream = nullptr;
   hipDeviceProp_t props{};
 };
+// Drops the FFT plans of resident-buffer transforms made for a context
+// (resident.hip), when it is destroyed.
+void release_resident_plans(const struct ::fpmhip_context* c);
 
 struct Axis {
   uint32_t lower, upper;

@@ -24,7 +24,10 @@ EXPORTS = [
     "fpmhip_last_error", "fpmhip_last_error_name", "fpmhip_version", "fpmhip_compile_planar",
     "fpmhip_algorithm_default", "fpmhip_algorithm_validate", "fpmhip_context_create", "fpmhip_context_destroy",
     "fpmhip_context_synchronize", "fpmhip_fft2_host", "fpmhip_model_create", "fpmhip_model_destroy",
-    "fpmhip_measurements_create", "fpmhip_measurements_destroy", "fpmhip_forward_stack", "fpmhip_session_create",
+    "fpmhip_measurements_create", "fpmhip_measurements_destroy", "fpmhip_forward_stack",
+    "fpmhip_context_capabilities", "fpmhip_buffer_allocate", "fpmhip_buffer_free",
+    "fpmhip_buffer_info", "fpmhip_buffer_upload", "fpmhip_buffer_download", "fpmhip_fft2_resident",
+    "fpmhip_session_initialize_from_object", "fpmhip_session_create",
     "fpmhip_session_destroy", "fpmhip_session_initialize", "fpmhip_session_set_state", "fpmhip_session_sweep",
     "fpmhip_session_schedule", "fpmhip_session_run", "fpmhip_session_get_spectrum", "fpmhip_session_get_pupil",
     "fpmhip_session_get_object", "fpmhip_session_get_momentum", "fpmhip_session_get_calibration",
@@ -54,7 +57,7 @@ _lib = None
 # sessions, measurements, models, then contexts, while the HIP runtime is
 # still up. After that, __del__ does nothing, so that no destroy call runs in
 # an arbitrary order during the interpreter's own finalization.
-_LEVELS = ("server", "batch", "session", "measurements", "model", "context")
+_LEVELS = ("server", "batch", "session", "measurements", "model", "buffer", "context")
 _live = {k: weakref.WeakSet() for k in _LEVELS}
 _closed = False
 
@@ -244,6 +247,82 @@ def forward_stack(model: ImagePlaneModel, spectrum, pupil=None, context: Optiona
     return out
 
 
+class Capabilities(ct.Structure):
+    _fields_ = [("preferred_memory", ct.c_int32), ("resident_buffers", ct.c_int32)]
+
+
+def capabilities(context: Optional[Context] = None) -> dict:
+    """Backend::capabilities (src/backend/traits.rs:27-33)."""
+    ctx = context or default_context()
+    c = Capabilities()
+    check(lib().fpmhip_context_capabilities(ctx.handle, ct.byref(c)))
+    return {"preferred_memory": "device" if c.preferred_memory == 1 else "host",
+            "resident_buffers": bool(c.resident_buffers)}
+
+
+COMPLEX, REAL = 0, 1
+
+
+class ResidentBuffer:
+    """A ComplexBuffer / RealBuffer of the ResidentBackend seam
+    (src/backend/traits.rs:44-106) in device memory: `len` complex128 or
+    f64 values, zeroed at allocation."""
+
+    def __init__(self, kind, length, context: Optional[Context] = None):
+        self.ctx = context or default_context()
+        self.kind = COMPLEX if kind in ("complex", COMPLEX) else REAL
+        self.handle = ct.c_void_p()
+        check(lib().fpmhip_buffer_allocate(self.ctx.handle, ct.c_int32(self.kind), ct.c_uint64(length),
+                                           ct.byref(self.handle)))
+        _register("buffer", self)
+
+    def close(self):
+        if self.handle:
+            lib().fpmhip_buffer_free(self.handle)
+            self.handle = ct.c_void_p()
+
+    def __del__(self):
+        if not _closed:
+            try:
+                self.close()
+            except Exception:
+                pass
+
+    def info(self) -> dict:
+        k, n, loc, dev = ct.c_int32(), ct.c_uint64(), ct.c_int32(), ct.c_void_p()
+        check(lib().fpmhip_buffer_info(self.handle, ct.byref(k), ct.byref(n), ct.byref(loc), ct.byref(dev)))
+        return {"kind": "complex" if k.value == COMPLEX else "real", "len": n.value,
+                "location": "device" if loc.value == 1 else "host", "device_address": dev.value or 0}
+
+    def __len__(self):
+        return self.info()["len"]
+
+    @property
+    def device_address(self) -> int:
+        return self.info()["device_address"]
+
+    def upload(self, values, kind=None):
+        """upload_complex / upload_real (traits.rs:82-93); `kind` defaults to the buffer's."""
+        k = self.kind if kind is None else (COMPLEX if kind in ("complex", COMPLEX) else REAL)
+        a = np.ascontiguousarray(values, dtype=np.complex128 if k == COMPLEX else np.float64).ravel()
+        check(lib().fpmhip_buffer_upload(self.ctx.handle, self.handle, ct.c_int32(k), _cd(a.view(np.float64)),
+                                         ct.c_uint64(a.size)))
+
+    def download(self, length=None, kind=None):
+        """download_complex / download_real (traits.rs:86-95)."""
+        k = self.kind if kind is None else (COMPLEX if kind in ("complex", COMPLEX) else REAL)
+        n = len(self) if length is None else length
+        out = np.zeros(n, np.complex128 if k == COMPLEX else np.float64)
+        check(lib().fpmhip_buffer_download(self.ctx.handle, self.handle, ct.c_int32(k), _cd(out.view(np.float64)),
+                                           ct.c_uint64(n)))
+        return out
+
+    def fft2(self, shape, direction="forward"):
+        """fft2_resident (traits.rs:97-103), in place."""
+        check(lib().fpmhip_fft2_resident(self.ctx.handle, self.handle, ct.c_uint64(shape[0]), ct.c_uint64(shape[1]),
+                                         ct.c_int32(abi.FORWARD if direction == "forward" else abi.INVERSE)))
+
+
 class Session:
     """A ReconstructionState on the device plus one algorithm."""
 
@@ -278,6 +357,11 @@ class Session:
 
     def initialize(self):
         check(lib().fpmhip_session_initialize(self.handle))
+
+    def initialize_from_object(self, obj):
+        """ReconstructionState::from_object (state.rs:356-383)."""
+        o = np.ascontiguousarray(obj, dtype=np.complex128)
+        check(lib().fpmhip_session_initialize_from_object(self.handle, _cd(o)))
 
     def set_state(self, spectrum, pupil, velocity=None, anchor=None, counter=0, frame_gains=None, background=None):
         """Checkpoint restore (ReconstructionState::from_checkpoint, state.rs:398-419);

@@ -223,6 +223,42 @@ int fpmhip_context_synchronize(fpmhip_context* context);
 int fpmhip_fft2_host(fpmhip_context* context, double* values, uint64_t h, uint64_t w,
                      int32_t direction);
 
+/* ---- ResidentBackend: typed device buffers (src/backend/traits.rs:17-106) ---- */
+/* MemoryLocation (traits.rs:17-24). */
+typedef enum fpmhip_memory_location { FPMHIP_MEMORY_HOST = 0, FPMHIP_MEMORY_DEVICE = 1 } fpmhip_memory_location;
+/* BackendCapabilities (traits.rs:27-33): device memory, resident buffers. */
+typedef struct fpmhip_capabilities {
+  int32_t preferred_memory; /* fpmhip_memory_location */
+  int32_t resident_buffers;
+} fpmhip_capabilities;
+int fpmhip_context_capabilities(const fpmhip_context* context, fpmhip_capabilities* out);
+
+/* ComplexBuffer / RealBuffer (traits.rs:44-73): `len` complex128 or f64
+ * values in device memory, zeroed at allocation (CpuBackend zeroes,
+ * cpu.rs:178-189). */
+typedef enum fpmhip_buffer_kind { FPMHIP_BUFFER_COMPLEX = 0, FPMHIP_BUFFER_REAL = 1 } fpmhip_buffer_kind;
+typedef struct fpmhip_buffer fpmhip_buffer;
+/* allocate_complex / allocate_real (traits.rs:79-81). */
+int fpmhip_buffer_allocate(fpmhip_context* context, int32_t kind, uint64_t len, fpmhip_buffer** out);
+int fpmhip_buffer_free(fpmhip_buffer* buffer);
+/* len() and location() (traits.rs:46-53), and the buffer's device address
+ * (for the allocation-free entry points; any output may be NULL). */
+int fpmhip_buffer_info(const fpmhip_buffer* buffer, int32_t* kind, uint64_t* len, int32_t* location,
+                       void** device_address);
+/* upload_complex / upload_real and download_complex / download_real
+ * (traits.rs:82-95): `kind` names the typed operation; a buffer of the other
+ * kind or from another context is FPMHIP_INVALID_PARAMETER ("complex buffer"
+ * / "real buffer"), and `len` (complex values or reals) different from the
+ * buffer's is FPMHIP_INVALID_SHAPE (cpu.rs:238-285). */
+int fpmhip_buffer_upload(fpmhip_context* context, fpmhip_buffer* buffer, int32_t kind, const double* source,
+                         uint64_t len);
+int fpmhip_buffer_download(fpmhip_context* context, const fpmhip_buffer* buffer, int32_t kind,
+                           double* destination, uint64_t len);
+/* fft2_resident (traits.rs:97-103): in-place 2-D FFT of a complex buffer of
+ * h * w values, forward / (h w), inverse unscaled. */
+int fpmhip_fft2_resident(fpmhip_context* context, fpmhip_buffer* buffer, uint64_t h, uint64_t w,
+                         int32_t direction);
+
 /* ---- model and measurement upload ---- */
 /* Validates like ImagePlaneModel::validate (image_plane_fpm.rs:841-972) and
  * uploads pupil, support, crops, offsets, gains, background, multiplexing. */
@@ -254,6 +290,12 @@ int fpmhip_session_destroy(fpmhip_session* session);
  * followed by the algorithm's auxiliary state (mpie.rs:152-172) and, for
  * EPRY with pupil recovery, canonicalize_state (runner.rs:122). */
 int fpmhip_session_initialize(fpmhip_session* session);
+/* ReconstructionState::from_object (src/reconstruction/state.rs:356-383):
+ * fpmhip_session_initialize, then the object spectrum replaced by
+ * fftshift(FFT(object) / M) of `object` [R] complex128, and the algorithm's
+ * auxiliary state (mPIE velocity 0, anchor = spectrum) and EPRY gauge made
+ * from it. */
+int fpmhip_session_initialize_from_object(fpmhip_session* session, const double* object);
 /* Restores state (ReconstructionState::from_checkpoint_with_backend, state.rs:398-419):
  * spectrum [R] and pupil [N] complex128; mPIE velocity/anchor [R] (NULL = zero
  * velocity, anchor = spectrum) and counter; gains [F] / background (NULL = model). */
