@@ -513,6 +513,7 @@ int fpmhip_context_create(int32_t device, fpmhip_context** out) {
     HIP_CHECK(hipSetDevice(device));
     HIP_CHECK(hipGetDeviceProperties(&c->props, device));
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->scratch = noalloc_scratch_address(device);
     *out = c.release();
   });
 }

@@ -106,7 +106,11 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipDeviceProp_t props{};
+  void* scratch = nullptr;  // device address of this device's forward-stack scratch array (noalloc.hip)
 };
+// Device address of the allocation-free forward stack's scratch array on the
+// current device (noalloc.hip), looked up once per context.
+void* noalloc_scratch_address(int device);
 // Drops the FFT plans of resident-buffer transforms made for a context
 // (resident.hip), when it is destroyed.
 void release_resident_plans(const struct ::fpmhip_context* c);

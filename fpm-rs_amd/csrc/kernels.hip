@@ -1013,6 +1013,22 @@ void launch_forward_stack(const Model& m, const double2* spec, const double2* pu
   HIP_CHECK(hipGetLastError());
 }
 
+void launch_forward_stack_frames(const Model& m, const double2* spec, const double2* pupil, double2* fields,
+                                 double* out, uint64_t f0, uint64_t f1, hipStream_t stream) {
+  const uint64_t N = m.N(), count = f1 - f0, bg_len = m.background.size();
+  // without multiplexing frame f is source f: the frames' sources are
+  // consecutive, and a per-frame background starts at frame f0
+  const double* bg = bg_len == m.F * N && bg_len != N ? m.d_background.as<double>() + f0 * N
+                                                      : m.d_background.as<double>();
+  hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(N), (uint32_t)count), dim3(kBlock), 0, stream, spec, m.rw,
+                     m.d_src.as<SrcParam>() + f0, m.nh, m.nw, pupil, fields);
+  HIP_CHECK(hipGetLastError());
+  fft2_f64(*m.fft_low, fields, (uint32_t)count, true, stream);
+  hipLaunchKernelGGL(k_intensity, dim3(blocks_for(N), (uint32_t)count), dim3(kBlock), 0, stream, fields, (uint32_t)N,
+                     nullptr, nullptr, nullptr, m.d_gains.as<double>() + f0, bg, bg_len, out + f0 * N);
+  HIP_CHECK(hipGetLastError());
+}
+
 // difference of mode `mode` of frame `frame` (multiplexed frames only)
 void launch_difference_frame(const fpmhip_session& s, uint32_t mode, uint64_t frame) {
   const Model& m = *s.model;

@@ -25,7 +25,7 @@ EXPORTS = [
     "fpmhip_algorithm_default", "fpmhip_algorithm_validate", "fpmhip_context_create", "fpmhip_context_destroy",
     "fpmhip_context_synchronize", "fpmhip_fft2_host", "fpmhip_model_create", "fpmhip_model_destroy",
     "fpmhip_measurements_create", "fpmhip_measurements_destroy", "fpmhip_forward_stack",
-    "fpmhip_context_capabilities", "fpmhip_buffer_allocate", "fpmhip_buffer_free",
+    "fpmhip_forward_stack_noalloc", "fpmhip_context_capabilities", "fpmhip_buffer_allocate", "fpmhip_buffer_free",
     "fpmhip_buffer_info", "fpmhip_buffer_upload", "fpmhip_buffer_download", "fpmhip_fft2_resident",
     "fpmhip_session_initialize_from_object", "fpmhip_session_create",
     "fpmhip_session_destroy", "fpmhip_session_initialize", "fpmhip_session_set_state", "fpmhip_session_sweep",
@@ -321,6 +321,15 @@ class ResidentBuffer:
         """fft2_resident (traits.rs:97-103), in place."""
         check(lib().fpmhip_fft2_resident(self.ctx.handle, self.handle, ct.c_uint64(shape[0]), ct.c_uint64(shape[1]),
                                          ct.c_int32(abi.FORWARD if direction == "forward" else abi.INVERSE)))
+
+
+def forward_stack_noalloc(context: Context, dmodel: DeviceModel, spectrum_dev: int, pupil_dev: int, out_dev: int):
+    """fpmhip_forward_stack_noalloc: the forward stack over caller-owned
+    device buffers (addresses as ints; pupil_dev 0 = the model pupil), with
+    no device allocation. Enqueued on the context's stream: synchronise the
+    context before reading `out_dev`."""
+    check(lib().fpmhip_forward_stack_noalloc(context.handle, dmodel.handle, ct.c_void_p(spectrum_dev),
+                                             ct.c_void_p(pupil_dev or None), ct.c_void_p(out_dev)))
 
 
 class Session:

@@ -281,6 +281,22 @@ int fpmhip_measurements_destroy(fpmhip_measurements* measurements);
 int fpmhip_forward_stack(fpmhip_context* context, const fpmhip_model* model,
                          const double* spectrum, const double* pupil, double* out);
 
+/* Allocation-free variant of fpmhip_forward_stack. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). `spectrum` [R] and `pupil` [N] (NULL = model pupil) are
+ * complex128 and `out` [F*N] f64 in the caller's DEVICE memory; the call
+ * enqueues the work on the context's stream and returns (results are
+ * complete once the stream is synchronised, e.g. fpmhip_context_synchronize)
+ * and makes no device allocation. The per-source fields live in a fixed
+ * 16 MiB device array of the library, one per GPU, held until the process
+ * ends and shared by every call on that GPU: calls that overlap on one GPU
+ * race (calls on one context's stream are ordered). Models with multiplexed
+ * frames are Unsupported; an image above 1 Mi pixels (the array's 2^20
+ * complex128 values) is InvalidShape. The output equals fpmhip_forward_stack's. */
+int fpmhip_forward_stack_noalloc(fpmhip_context* context, const fpmhip_model* model,
+                                 const double* spectrum, const double* pupil, double* out);
+
 /* ---- reconstruction session: ReconstructionState + one algorithm ---- */
 int fpmhip_session_create(const fpmhip_model* model, const fpmhip_measurements* measurements,
                           int32_t precision, const fpmhip_algorithm* algorithm,
