@@ -119,6 +119,27 @@ class Algorithm(ct.Structure):
     ]
 
 
+class Camera(ct.Structure):
+    _fields_ = [
+        ("photons_per_pixel", c_dbl),
+        ("gain_counts_per_electron", c_dbl),
+        ("offset_counts", c_dbl),
+        ("read_noise_electrons", c_dbl),
+        ("dark_current_electrons", c_dbl),
+        ("shot_noise", c_i32),
+        ("quantize", c_i32),
+        ("has_bit_depth", c_i32),
+        ("bit_depth", c_i32),
+        ("has_saturation", c_i32),
+        ("has_bad_pixel_value", c_i32),
+        ("saturation_counts", c_dbl),
+        ("bad_pixel_value_counts", c_dbl),
+        ("pixel_sensitivity", P_dbl),
+        ("bad_pixels", ct.POINTER(c_u64)),
+        ("bad_pixel_count", c_u64),
+    ]
+
+
 def ptr(array, ctype):
     """Pointer to a C-contiguous numpy array's data (None for None)."""
     if array is None:

@@ -36,6 +36,7 @@ EXPORTS = [
     "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_objects_pinned", "fpmhip_batch_objects_device", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
     "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
     "fpmhip_server_poll", "fpmhip_server_wait", "fpmhip_server_result", "fpmhip_server_stats",
+    "fpmhip_camera_validate", "fpmhip_camera_measure", "fpmhip_simulate",
 ]
 
 

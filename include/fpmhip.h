@@ -297,6 +297,56 @@ int fpmhip_forward_stack(fpmhip_context* context, const fpmhip_model* model,
 int fpmhip_forward_stack_noalloc(fpmhip_context* context, const fpmhip_model* model,
                                  const double* spectrum, const double* pupil, double* out);
 
+/* ---- simulation: CameraModel (src/simulation/camera.rs:7-34) and Simulator::simulate
+ * (src/simulation/simulator.rs:107-207) ----
+ * Each Option of CameraModel is a has_ flag plus its value. pixel_sensitivity
+ * [frame_len] and bad_pixels [bad_pixel_count] are host arrays (NULL / 0 = none).
+ * Noise: shot noise is Poisson(expected electrons) and read noise
+ * Normal(0, read_noise_electrons), drawn on the device from a counter-based
+ * Philox4x64-10 stream: pixel p of frame f takes the blocks of counter
+ * (p, f, block, 0) under key (seed, 0), so a seed gives the same counts on any
+ * grid shape. The reference's StdRng stream is not reproduced (same
+ * distributions, other draws). */
+typedef struct fpmhip_camera {
+  double photons_per_pixel;        /* > 0 */
+  double gain_counts_per_electron; /* > 0 */
+  double offset_counts;
+  double read_noise_electrons;     /* >= 0 */
+  double dark_current_electrons;   /* >= 0 */
+  int32_t shot_noise;
+  int32_t quantize;
+  int32_t has_bit_depth;
+  int32_t bit_depth;               /* 1..=32 when present */
+  int32_t has_saturation;
+  int32_t has_bad_pixel_value;
+  double saturation_counts;        /* finite, > 0 when present */
+  double bad_pixel_value_counts;   /* finite, >= 0 when present */
+  const double* pixel_sensitivity;
+  const uint64_t* bad_pixels;
+  uint64_t bad_pixel_count;
+} fpmhip_camera;
+
+/* CameraModel::validate (camera.rs:145-223), and validate_for_frame
+ * (camera.rs:226-245) when frame_len > 0. InvalidParameter named as the
+ * reference names it ("photons_per_pixel", "camera noise/offset", "bit_depth", ...). */
+int fpmhip_camera_validate(const fpmhip_camera* camera, uint64_t frame_len);
+
+/* CameraModel::measure_frame (camera.rs:248-287) on frame_count frames of
+ * frame_len pixels, in place (host memory); frame f draws counter (p, f, ...). */
+int fpmhip_camera_measure(fpmhip_context* context, const fpmhip_camera* camera, double* frames,
+                          uint64_t frame_count, uint64_t frame_len, uint64_t seed);
+
+/* The device part of Simulator::simulate (simulator.rs:138-189) for a true
+ * model whose acquisition errors (source permutation, frame-gain variation)
+ * the caller has already applied to it: object [R] complex128 (row-major
+ * field) -> centred spectrum (forward FFT / R, fftshift; simulator.rs:209-226)
+ * -> forward intensity stack -> frames listed in missing_frames [missing_count]
+ * (unique, < F) replaced by the model's optical background -> camera (NULL =
+ * direct optical intensities) -> out [F*N] f64. */
+int fpmhip_simulate(fpmhip_context* context, const fpmhip_model* true_model, const double* object,
+                    const fpmhip_camera* camera, const uint64_t* missing_frames, uint64_t missing_count,
+                    uint64_t seed, double* out);
+
 /* ---- reconstruction session: ReconstructionState + one algorithm ---- */
 int fpmhip_session_create(const fpmhip_model* model, const fpmhip_measurements* measurements,
                           int32_t precision, const fpmhip_algorithm* algorithm,

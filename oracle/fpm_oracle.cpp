@@ -1236,6 +1236,126 @@ Compiled compile_planar(const fpmhip_optics* o, const fpmhip_planar_array* a, ui
   return out;
 }
 
+// ---------------------------------------------------------------------------
+// Camera (src/simulation/camera.rs:248-327). The deterministic steps restate
+// measure_frame; the noise is not the reference's (rand's StdRng / ChaCha12
+// stream and rand_distr's samplers are not vendored in the reference, so its
+// draws cannot be reproduced): it restates the product's own generator —
+// Philox4x64-10 (Salmon et al., SC 2011) with counter (pixel, frame, block, 0)
+// and key (seed, 0), uniforms (x >> 11) 2^-53, Box-Muller for read noise,
+// inversion below mean 10 and Hoermann's PTRS (1993) above — so that device
+// draws can be checked draw for draw. Pins: Philox against numpy's Philox
+// bit generator; the samplers' moments and distribution against the exact
+// Poisson / normal laws; the deterministic steps against the reference's own
+// camera tests (tests/simulation_tests.rs:77-122,208-222,554-573).
+
+struct Philox {
+  uint64_t v[4];
+};
+
+Philox philox4x64_10(Philox c, uint64_t k0, uint64_t k1) {
+  const uint64_t m0 = 0xD2E7470EE14C6C93ull, m1 = 0xCA5A826395121157ull;
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B97F4A7C15ull;
+      k1 += 0xBB67AE8584CAA73Bull;
+    }
+    const unsigned __int128 p0 = (unsigned __int128)m0 * c.v[0];
+    const unsigned __int128 p1 = (unsigned __int128)m1 * c.v[2];
+    const uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    const uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    c = Philox{{hi1 ^ c.v[1] ^ k0, lo1, hi0 ^ c.v[3] ^ k1, lo0}};
+  }
+  return c;
+}
+
+struct Uniforms {
+  uint64_t pixel, frame, seed, block = 0;
+  int used = 4;
+  Philox words{};
+  void at_block(uint64_t b) {
+    block = b;
+    used = 4;
+  }
+  double next() {
+    if (used == 4) {
+      words = philox4x64_10(Philox{{pixel, frame, block, 0}}, seed, 0);
+      ++block;
+      used = 0;
+    }
+    return (double)(words.v[used++] >> 11) * 0x1.0p-53;
+  }
+};
+
+double draw_poisson(double mean, Uniforms& u) {
+  u.at_block(1);
+  if (mean < 10.0) {
+    const double x = u.next();
+    double term = std::exp(-mean), cdf = term;
+    int k = 0;
+    while (x >= cdf && k < 1000) {
+      ++k;
+      term = term * mean / (double)k;
+      cdf = cdf + term;
+    }
+    return (double)k;
+  }
+  const double root = std::sqrt(mean), lmean = std::log(mean);
+  const double b = 0.931 + 2.53 * root;
+  const double a = -0.059 + 0.02483 * b;
+  const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4);
+  const double v_r = 0.9277 - 3.6224 / (b - 2.0);
+  for (int t = 0; t < 256; ++t) {
+    const double U = u.next() - 0.5;
+    const double V = u.next();
+    const double us = 0.5 - std::fabs(U);
+    const double k = std::floor((2.0 * a / us + b) * U + mean + 0.43);
+    if (us >= 0.07 && V <= v_r) return k;
+    if (k < 0.0 || (us < 0.013 && V > us)) continue;
+    if (std::log(V) + std::log(inv_alpha) - std::log(a / (us * us) + b) <= -mean + k * lmean - std::lgamma(k + 1.0))
+      return k;
+  }
+  return std::floor(mean);
+}
+
+double draw_normal(Uniforms& u) {
+  u.at_block(0);
+  const double u1 = 1.0 - u.next();
+  const double u2 = u.next();
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+// CameraModel::maximum_count (camera.rs:320-327)
+double camera_maximum(const fpmhip_camera& c) {
+  const double inf = std::numeric_limits<double>::infinity();
+  const double digital = c.has_bit_depth ? std::pow(2.0, c.bit_depth) - 1.0 : inf;
+  return std::min(c.has_saturation ? c.saturation_counts : inf, digital);
+}
+
+// f64::clamp(0, hi): NaN passes through
+double clamp0(double x, double hi) { return x < 0.0 ? 0.0 : (x > hi ? hi : x); }
+
+// CameraModel::measure_frame (camera.rs:248-287) on frame `frame` of n pixels
+void camera_measure_frame(const fpmhip_camera& c, double* v, uint64_t n, uint64_t frame, uint64_t seed) {
+  const double top = camera_maximum(c);
+  for (uint64_t p = 0; p < n; ++p) {
+    Uniforms u{p, frame, seed};
+    const double sensitivity = c.pixel_sensitivity ? c.pixel_sensitivity[p] : 1.0;
+    const double expected = std::fmax(v[p], 0.0) * sensitivity * c.photons_per_pixel + c.dark_current_electrons;
+    double electrons = (c.shot_noise && expected > 0.0) ? draw_poisson(expected, u) : expected;
+    if (c.read_noise_electrons > 0.0) electrons += 0.0 + c.read_noise_electrons * draw_normal(u);
+    double counts = electrons * c.gain_counts_per_electron + c.offset_counts;
+    counts = clamp0(counts, top);
+    if (c.quantize) counts = std::round(counts);
+    v[p] = counts;
+  }
+  for (uint64_t i = 0; i < c.bad_pixel_count; ++i) {
+    double value = clamp0(c.has_bad_pixel_value ? c.bad_pixel_value_counts : top, top);
+    if (c.quantize) value = std::round(value);
+    v[c.bad_pixels[i]] = value;
+  }
+}
+
 template <class F>
 int guarded(F&& body) {
   g_error.clear();
@@ -1285,6 +1405,33 @@ typedef struct orc_outputs {
 
 const char* orc_last_error(void) { return g_error.c_str(); }
 const char* orc_last_error_name(void) { return g_error_name.c_str(); }
+
+// Philox4x64-10 of one counter block under a key (4 words out).
+int orc_philox(const uint64_t* counter, const uint64_t* key, uint64_t* out) {
+  return guarded([&] {
+    Philox r = philox4x64_10(Philox{{counter[0], counter[1], counter[2], counter[3]}}, key[0], key[1]);
+    std::copy(r.v, r.v + 4, out);
+  });
+}
+
+// The camera's samplers on pixels 0..n-1 of frame 0: Poisson(mean) or, with
+// mean < 0, standard normal draws.
+int orc_camera_draws(double mean, uint64_t n, uint64_t seed, double* out) {
+  return guarded([&] {
+    for (uint64_t p = 0; p < n; ++p) {
+      Uniforms u{p, 0, seed};
+      out[p] = mean < 0.0 ? draw_normal(u) : draw_poisson(mean, u);
+    }
+  });
+}
+
+// CameraModel::measure_frame over `frames` frames of n pixels, in place.
+// The camera is expected valid (the product validates).
+int orc_camera_measure(const fpmhip_camera* camera, double* values, uint64_t frames, uint64_t n, uint64_t seed) {
+  return guarded([&] {
+    for (uint64_t f = 0; f < frames; ++f) camera_measure_frame(*camera, values + f * n, n, f, seed);
+  });
+}
 
 // point_loss (src/algorithms/objective.rs:47-71) of n value pairs.
 int orc_point_loss(const double* predicted, const double* measured, uint64_t n, int32_t loss, double* out) {

@@ -188,6 +188,61 @@ def compile_model(optics, array, image_shape, reconstruction_shape="smooth"):
                            frame_gain=gains, k_vectors=kv)
 
 
+def philox(counter, key):
+    """Philox4x64-10 of one counter block: 4 uint64 words."""
+    c = np.ascontiguousarray(counter, dtype=np.uint64)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    out = np.zeros(4, np.uint64)
+    _check(lib().orc_philox(c.ctypes.data_as(abi.P_u64), k.ctypes.data_as(abi.P_u64), out.ctypes.data_as(abi.P_u64)))
+    return out
+
+
+def camera_draws(mean, n, seed):
+    """The camera samplers on pixels 0..n-1 of frame 0: Poisson(mean), or
+    standard normal draws for mean < 0."""
+    out = np.zeros(n, np.float64)
+    _check(lib().orc_camera_draws(ct.c_double(mean), ct.c_uint64(n), ct.c_uint64(seed), _cd(out)))
+    return out
+
+
+def _camera_struct(cam):
+    """abi.Camera from an object with CameraModel's fields (camera.rs:7-34)."""
+    c = abi.Camera()
+    c.photons_per_pixel = cam.photons_per_pixel
+    c.gain_counts_per_electron = cam.gain_counts_per_electron
+    c.offset_counts = cam.offset_counts
+    c.read_noise_electrons = cam.read_noise_electrons
+    c.dark_current_electrons = cam.dark_current_electrons
+    c.shot_noise = int(bool(cam.shot_noise))
+    c.quantize = int(bool(cam.quantize))
+    c.has_bit_depth = int(cam.bit_depth is not None)
+    c.bit_depth = int(cam.bit_depth or 0)
+    c.has_saturation = int(cam.saturation_counts is not None)
+    c.saturation_counts = cam.saturation_counts or 0.0
+    c.has_bad_pixel_value = int(cam.bad_pixel_value_counts is not None)
+    c.bad_pixel_value_counts = cam.bad_pixel_value_counts or 0.0
+    keep = []
+    if cam.pixel_sensitivity is not None:
+        s = np.ascontiguousarray(cam.pixel_sensitivity, dtype=np.float64).ravel()
+        keep.append(s)
+        c.pixel_sensitivity = _cd(s)
+    bp = np.ascontiguousarray(np.asarray(cam.bad_pixels, dtype=np.uint64))
+    keep.append(bp)
+    c.bad_pixels = bp.ctypes.data_as(abi.P_u64) if bp.size else None
+    c.bad_pixel_count = bp.size
+    return c, keep
+
+
+def camera_measure(cam, frames, seed, frame_len=None):
+    """CameraModel::measure_frame (camera.rs:248-287) over consecutive frames
+    of frame_len pixels (default: the last two axes); returns a new array."""
+    v = np.array(frames, dtype=np.float64, order="C", copy=True)
+    n = frame_len or (int(np.prod(v.shape[1:])) if v.ndim > 1 else v.size)
+    c, keep = _camera_struct(cam)
+    _check(lib().orc_camera_measure(ct.byref(c), _cd(v), ct.c_uint64(v.size // n), ct.c_uint64(n), ct.c_uint64(seed)))
+    return v
+
+
 def forward_stack(model: ImagePlaneModel, spectrum, pupil=None):
     d = model.desc()
     s = np.ascontiguousarray(spectrum, dtype=np.complex128)
