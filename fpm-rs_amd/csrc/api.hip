@@ -116,30 +116,44 @@ void init_aux(fpmhip_session& s) {
   s.counter = 0;
 }
 
-std::vector<uint64_t> schedule_order(const fpmhip_session& s, int schedule) {
+// Philox4x64-10 (Salmon et al., SC 2011) of one counter block, host side
+void philox4x64_10(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B97F4A7C15ull;
+      k1 += 0xBB67AE8584CAA73Bull;
+    }
+    const unsigned __int128 p0 = (unsigned __int128)0xD2E7470EE14C6C93ull * c[0];
+    const unsigned __int128 p1 = (unsigned __int128)0xCA5A826395121157ull * c[2];
+    const uint64_t n0 = (uint64_t)(p1 >> 64) ^ c[1] ^ k0, n2 = (uint64_t)(p0 >> 64) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint64_t)p1;
+    c[2] = n2;
+    c[3] = (uint64_t)p0;
+  }
+}
+
+// FrameSchedule::order_for_problem (schedule.rs:35-104) for zero-based iteration `iteration`
+std::vector<uint64_t> schedule_order(const fpmhip_session& s, const fpmhip_frame_schedule& sched, uint64_t iteration) {
   const Model& m = *s.model;
-  const Measurements& me = *s.meas;
   std::vector<uint64_t> order(m.F);
   std::iota(order.begin(), order.end(), 0);
+  const int schedule = sched.kind;
   if (schedule == FPMHIP_SCHEDULE_SEQUENTIAL) return order;
-  if (schedule == FPMHIP_SCHEDULE_SNR_WEIGHTED) {  // schedule.rs:84-128
-    std::vector<double> score(m.F);
-    const uint64_t N = m.N();
-    for (uint64_t f = 0; f < m.F; ++f) {
-      if (me.weight[f] == 0.0) {
-        score[f] = -std::numeric_limits<double>::infinity();
-        continue;
-      }
-      double sum = 0.0;
-      uint64_t valid = 0;
-      for (uint64_t p = 0; p < N; ++p)
-        if (me.masks.empty() || me.masks[f * N + p]) {
-          const double b = m.background.empty() ? 0.0 : m.background[m.background.size() == N ? p : f * N + p];
-          sum += std::max(me.value(f, p) - b, 0.0);
-          ++valid;
-        }
-      score[f] = valid ? me.weight[f] * std::sqrt(sum / (double)valid) : -std::numeric_limits<double>::infinity();
+  if (schedule == FPMHIP_SCHEDULE_RANDOM_SHUFFLE) {
+    // a per-iteration seeded permutation (Fisher-Yates from the top)
+    const uint64_t key = sched.seed + iteration * 0x9e3779b97f4a7c15ull;
+    for (uint64_t i = m.F; i-- > 1;) {
+      uint64_t c[4] = {i, 0, 0, 0};
+      philox4x64_10(c, key, 1);
+      const double u = (double)(c[0] >> 11) * 0x1.0p-53;
+      const uint64_t j = std::min<uint64_t>(i, (uint64_t)(u * (double)(i + 1)));
+      std::swap(order[i], order[j]);
     }
+    return order;
+  }
+  if (schedule == FPMHIP_SCHEDULE_SNR_WEIGHTED) {  // schedule.rs:84-104, scores reduced on the device
+    const std::vector<double> score = snr_scores(s);
     std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return score[a] > score[b]; });
     return order;
   }
@@ -937,20 +951,41 @@ int fpmhip_session_sweep(fpmhip_session* s, const uint64_t* order, uint64_t orde
 int fpmhip_session_schedule(fpmhip_session* s, int32_t schedule, uint64_t* order) {
   return guarded([&] {
     require(s && order, "schedule", "null argument");
-    std::vector<uint64_t> o = schedule_order(*s, schedule);
+    fpmhip_frame_schedule sched{schedule, 0, 0};
+    std::vector<uint64_t> o = schedule_order(*s, sched, 0);
     std::copy(o.begin(), o.end(), order);
   });
 }
 
 int fpmhip_session_run(fpmhip_session* s, uint64_t iterations, int32_t schedule, double* trace) {
+  if (!s) return guarded([&] { require(false, "session", "session is null"); });
+  const fpmhip_frame_schedule sched{schedule, 0, 0};
+  return fpmhip_session_run_schedule(s, iterations, &sched, 0, trace);
+}
+
+int fpmhip_session_schedule_at(fpmhip_session* s, const fpmhip_frame_schedule* schedule, uint64_t iteration,
+                               uint64_t* order) {
   return guarded([&] {
-    require(s != nullptr, "session", "session is null");
+    require(s && schedule && order, "schedule", "null argument");
+    set_device(s->ctx);
+    std::vector<uint64_t> o = schedule_order(*s, *schedule, iteration);
+    std::copy(o.begin(), o.end(), order);
+  });
+}
+
+int fpmhip_session_run_schedule(fpmhip_session* s, uint64_t iterations, const fpmhip_frame_schedule* schedule,
+                                uint64_t first_iteration, double* trace) {
+  return guarded([&] {
+    require(s != nullptr && schedule != nullptr, "session", "null argument");
     set_device(s->ctx);
     if (!s->initialized) {
       if (int rc = fpmhip_session_initialize(s)) throw Failure{rc};
     }
-    std::vector<uint64_t> order = schedule_order(*s, schedule);
+    // orders that do not depend on the iteration are computed once
+    const bool per_iteration = schedule->kind == FPMHIP_SCHEDULE_RANDOM_SHUFFLE;
+    std::vector<uint64_t> order = schedule_order(*s, *schedule, first_iteration);
     for (uint64_t it = 0; it < iterations; ++it) {
+      if (per_iteration && it) order = schedule_order(*s, *schedule, first_iteration + it);
       double obj = 0.0;
       sweep(*s, order.data(), order.size(), nullptr, &obj);
       if (trace) trace[it] = obj;

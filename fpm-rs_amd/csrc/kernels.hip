@@ -999,8 +999,9 @@ void launch_fill_zero(float2* dst, uint64_t n, hipStream_t stream) {
   HIP_CHECK(hipGetLastError());
 }
 
-void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields, double* out,
-                          hipStream_t stream) {
+void launch_forward_stack_with(const Model& m, const double2* spec, const double2* pupil, double2* fields,
+                               double* out, const double* gains, const double* bg, uint64_t bg_len,
+                               hipStream_t stream) {
   const uint64_t N = m.N();
   hipLaunchKernelGGL(k_extract_all, dim3(blocks_for(N), (uint32_t)m.S), dim3(kBlock), 0, stream, spec, m.rw,
                      m.d_src.as<SrcParam>(), m.nh, m.nw, pupil, fields);
@@ -1008,9 +1009,14 @@ void launch_forward_stack(const Model& m, const double2* spec, const double2* pu
   fft2_f64(*m.fft_low, fields, (uint32_t)m.S, true, stream);
   hipLaunchKernelGGL(k_intensity, dim3(blocks_for(N), (uint32_t)m.F), dim3(kBlock), 0, stream, fields,
                      (uint32_t)N, m.mux ? m.d_mux_ptr.as<uint32_t>() : nullptr, m.d_mux_src.as<uint32_t>(),
-                     m.d_mux_w.as<double>(), m.d_gains.as<double>(), m.d_background.as<double>(),
-                     (uint64_t)m.background.size(), out);
+                     m.d_mux_w.as<double>(), gains, bg, bg_len, out);
   HIP_CHECK(hipGetLastError());
+}
+
+void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields, double* out,
+                          hipStream_t stream) {
+  launch_forward_stack_with(m, spec, pupil, fields, out, m.d_gains.as<double>(), m.d_background.as<double>(),
+                            (uint64_t)m.background.size(), stream);
 }
 
 void launch_forward_stack_frames(const Model& m, const double2* spec, const double2* pupil, double2* fields,

@@ -188,6 +188,14 @@ void launch_fill_zero(double2* dst, uint64_t n, hipStream_t stream);
 void launch_fill_zero(float2* dst, uint64_t n, hipStream_t stream);
 void launch_forward_stack(const Model& m, const double2* spec, const double2* pupil, double2* fields,
                           double* out, hipStream_t stream);
+// The forward stack with explicit frame gains [F] and background (length
+// bg_len: 0, N or F N) in place of the model's.
+void launch_forward_stack_with(const Model& m, const double2* spec, const double2* pupil, double2* fields,
+                               double* out, const double* gains, const double* bg, uint64_t bg_len,
+                               hipStream_t stream);
+// SnrWeighted frame scores (schedule.rs:106-128) from the resident
+// measurements and the model's background, reduced on the device.
+std::vector<double> snr_scores(const fpmhip_session& s);
 
 // Capacity (uint32 entries) of a session's momentum chunk table, and its
 // upload for a given pupil (interleaved complex128; the table follows the

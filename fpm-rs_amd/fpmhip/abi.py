@@ -33,7 +33,7 @@ AP, FPIE, MPIE, EPRY = 0, 1, 2, 3
 LOSS = {"amplitude_mse": 0, "intensity_mse": 1, "poisson_nll": 2, "huber_amplitude": 3}
 SHAPE = {"exact": 0, "minimum": 1, "smooth": 2, "power_of_two": 3}
 PLAIN_C128, MIXED_F32 = 0, 1
-SCHEDULE = {"sequential": 0, "brightfield_first": 1, "spiral_out": 2, "snr_weighted": 3}
+SCHEDULE = {"sequential": 0, "brightfield_first": 1, "spiral_out": 2, "snr_weighted": 3, "random_shuffle": 4}
 
 
 class Optics(ct.Structure):
@@ -138,6 +138,28 @@ class Camera(ct.Structure):
         ("bad_pixels", ct.POINTER(c_u64)),
         ("bad_pixel_count", c_u64),
     ]
+
+
+class FrameSchedule(ct.Structure):
+    _fields_ = [("kind", c_i32), ("pad_", c_i32), ("seed", c_u64)]
+
+
+class IntensityStats(ct.Structure):
+    _fields_ = [("mean", c_dbl), ("std", c_dbl), ("min", c_dbl), ("max", c_dbl), ("sum", c_dbl),
+                ("saturated_pixels", c_u64), ("zero_pixels", c_u64)]
+
+
+class FrameComparison(ct.Structure):
+    _fields_ = [("reference_sum", c_dbl), ("estimate_sum", c_dbl), ("residual_l1", c_dbl), ("residual_l2", c_dbl),
+                ("residual_mean", c_dbl), ("residual_std", c_dbl), ("residual_max_abs", c_dbl),
+                ("normalized_l2", c_dbl)]
+
+
+class Diagnostics(ct.Structure):
+    _fields_ = [("per_frame_objective", P_dbl), ("raw_frame_statistics", ct.POINTER(IntensityStats)),
+                ("frame_summaries", ct.POINTER(FrameComparison)), ("residual_images", P_dbl),
+                ("object_amplitude", P_dbl), ("object_phase", P_dbl), ("pupil_amplitude", P_dbl),
+                ("pupil_phase", P_dbl)]
 
 
 def ptr(array, ctype):

@@ -37,14 +37,14 @@ class _Base:
         return a
 
     def run(self, problem, precision="c128", schedule="sequential", context=None, callbacks=None,
-            resume_from=None):
+            resume_from=None, schedule_seed=0):
         """Runner::run on the GPU (one session); returns a ReconstructionResult.
         `callbacks` may hold CheckpointEvery; `resume_from` a
         ReconstructionCheckpoint (python/fpm_rs/__init__.pyi:3316-3336)."""
         from .runtime import run_algorithm
 
         return run_algorithm(self, problem, precision=precision, schedule=schedule, context=context,
-                             callbacks=callbacks, resume_from=resume_from)
+                             callbacks=callbacks, resume_from=resume_from, schedule_seed=schedule_seed)
 
 
 @dataclass

@@ -36,7 +36,8 @@ EXPORTS = [
     "fpmhip_batch_timing", "fpmhip_batch_objects", "fpmhip_batch_objects_pinned", "fpmhip_batch_objects_device", "fpmhip_batch_get_schedule", "fpmhip_batch_set_schedule", "fpmhip_batch_tune",
     "fpmhip_server_create", "fpmhip_server_destroy", "fpmhip_server_submit",
     "fpmhip_server_poll", "fpmhip_server_wait", "fpmhip_server_result", "fpmhip_server_stats",
-    "fpmhip_camera_validate", "fpmhip_camera_measure", "fpmhip_simulate",
+    "fpmhip_camera_validate", "fpmhip_camera_measure", "fpmhip_simulate", "fpmhip_session_schedule_at",
+    "fpmhip_session_run_schedule", "fpmhip_session_diagnostics",
 ]
 
 
@@ -398,17 +399,70 @@ class Session:
                                          ct.c_uint64(0 if o is None else o.size), _cd(per), ct.byref(obj)))
         return obj.value, per
 
-    def schedule(self, name="sequential"):
+    def schedule(self, name="sequential", iteration=0, seed=0):
+        """FrameSchedule::order_for_problem (schedule.rs:35-104) for a
+        zero-based iteration; `seed` is RandomShuffle's."""
         order = np.zeros(self.F, np.uint64)
-        check(lib().fpmhip_session_schedule(self.handle, ct.c_int32(abi.SCHEDULE[name]),
-                                            abi.ptr(order, ct.c_uint64)))
+        sched = abi.FrameSchedule(abi.SCHEDULE[name], 0, seed)
+        check(lib().fpmhip_session_schedule_at(self.handle, ct.byref(sched), ct.c_uint64(iteration),
+                                               abi.ptr(order, ct.c_uint64)))
         return order
 
-    def run(self, iterations, schedule="sequential"):
+    def run(self, iterations, schedule="sequential", seed=0, first_iteration=0):
+        """`iterations` sweeps, iteration i (zero-based, from first_iteration)
+        in its schedule order (runner.rs:156-163); returns the objective trace."""
         trace = np.zeros(iterations, np.float64)
-        check(lib().fpmhip_session_run(self.handle, ct.c_uint64(iterations), ct.c_int32(abi.SCHEDULE[schedule]),
-                                       _cd(trace)))
+        sched = abi.FrameSchedule(abi.SCHEDULE[schedule], 0, seed)
+        check(lib().fpmhip_session_run_schedule(self.handle, ct.c_uint64(iterations), ct.byref(sched),
+                                                ct.c_uint64(first_iteration), _cd(trace)))
         return trace
+
+    def diagnostics(self, requests, iteration=None):
+        """build_diagnostics (runner.rs:298-433) of the current state for a
+        set of DiagnosticRequest names (diagnostics/context.rs:7-24):
+        "per_frame_error", "raw_frame_stats", "frame_summaries",
+        "residual_images", "object_amplitude", "object_phase", "pupil".
+        Computed on the device; returns a Diagnostics."""
+        req = set(requests)
+        unknown = req - set(DIAGNOSTIC_REQUESTS)
+        if unknown:
+            raise ValueError(f"unknown diagnostic requests {sorted(unknown)}")
+        F, n_img, M = self.F, self.N[0] * self.N[1], self.R[0] * self.R[1]
+        d = abi.Diagnostics()
+        arrays = {}
+
+        def want(name, key, shape, ctype=None):
+            if name in req:
+                a = np.zeros(shape, np.float64) if ctype is None else (ctype * F)()
+                arrays[key] = a
+                setattr(d, key, _cd(a) if ctype is None else ct.cast(a, ct.POINTER(ctype)))
+
+        want("per_frame_error", "per_frame_objective", F)
+        want("raw_frame_stats", "raw_frame_statistics", F, abi.IntensityStats)
+        want("frame_summaries", "frame_summaries", F, abi.FrameComparison)
+        want("residual_images", "residual_images", (F,) + self.N)
+        want("object_amplitude", "object_amplitude", self.R)
+        want("object_phase", "object_phase", self.R)
+        want("pupil", "pupil_amplitude", self.N)
+        want("pupil", "pupil_phase", self.N)
+        check(lib().fpmhip_session_diagnostics(self.handle, ct.byref(d)))
+        out = Diagnostics()
+        out.per_frame_objective = arrays.get("per_frame_objective")
+        if "raw_frame_statistics" in arrays:
+            out.raw_frame_statistics = [
+                {"frame_index": f, "metrics": {k: getattr(r, k) for k, _ in abi.IntensityStats._fields_}}
+                for f, r in enumerate(arrays["raw_frame_statistics"])]
+        if "frame_summaries" in arrays:
+            out.frame_diagnostics = [
+                {"iteration": iteration, "frame_index": f, "illumination_index": f,
+                 "metrics": dict({k: getattr(r, k) for k, _ in abi.FrameComparison._fields_}, saturated_pixels=None)}
+                for f, r in enumerate(arrays["frame_summaries"])]
+        out.residual_images = arrays.get("residual_images")
+        out.object_amplitude = arrays.get("object_amplitude")
+        out.object_phase = arrays.get("object_phase")
+        out.pupil_amplitude = arrays.get("pupil_amplitude")
+        out.pupil_phase = arrays.get("pupil_phase")
+        return out
 
     def spectrum(self):
         out = np.zeros(self.R, np.complex128)
@@ -454,6 +508,25 @@ class Session:
         c = ct.c_uint64(0)
         check(lib().fpmhip_session_get_momentum(self.handle, _cd(v), _cd(a), ct.byref(c)))
         return v, a, c.value
+
+
+DIAGNOSTIC_REQUESTS = ("per_frame_error", "raw_frame_stats", "frame_summaries", "residual_images",
+                       "object_amplitude", "object_phase", "pupil")
+
+
+@dataclass
+class Diagnostics:
+    """Diagnostics (src/diagnostics/context.rs:26-48); None = not requested."""
+
+    objective: Optional[float] = None
+    per_frame_objective: Optional[np.ndarray] = None
+    raw_frame_statistics: Optional[list] = None
+    frame_diagnostics: Optional[list] = None
+    object_amplitude: Optional[np.ndarray] = None
+    object_phase: Optional[np.ndarray] = None
+    pupil_amplitude: Optional[np.ndarray] = None
+    pupil_phase: Optional[np.ndarray] = None
+    residual_images: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -510,14 +583,16 @@ def _restore(s: Session, alg: abi.Algorithm, problem: ReconstructionProblem, ck)
 
 
 def run_algorithm(algorithm, problem: ReconstructionProblem, precision="c128", schedule="sequential",
-                  context=None, callbacks=None, resume_from=None) -> ReconstructionResult:
+                  context=None, callbacks=None, resume_from=None, schedule_seed=0) -> ReconstructionResult:
     """Runner::run (src/reconstruction/runner.rs:76-285) on one device session.
 
     `resume_from` restores a ReconstructionCheckpoint and continues from its
     completed iterations up to `algorithm.iterations` (runner.rs:94-121);
     `callbacks` takes CheckpointEvery (callbacks/checkpoint.rs). The trace
     holds the checkpoint's records followed by the new ones, with elapsed
-    seconds continuing from the last restored record."""
+    seconds continuing from the last restored record. `schedule_seed` is
+    RandomShuffle's seed; iteration i (zero-based) is ordered for i, also
+    after a resume."""
     import time
 
     from .checkpoint import CheckpointEvery, ReconstructionCheckpoint
@@ -546,14 +621,14 @@ def run_algorithm(algorithm, problem: ReconstructionProblem, precision="c128", s
     mparams = mpie_parameters(alg) if alg.kind == abi.MPIE else None
     if callbacks:
         for it in range(first, total):
-            obj = float(s.run(1, schedule)[0])
+            obj = float(s.run(1, schedule, schedule_seed, first_iteration=it)[0])
             records.append((it + 1, obj, previous + time.perf_counter() - started))
             for cb in callbacks:
                 if cb.due(it + 1):
                     ck = ReconstructionCheckpoint.capture(it + 1, s, records, mparams, metrics, carried_aux)
                     cb.write(ck)
     elif total > first:
-        objs = s.run(total - first, schedule)
+        objs = s.run(total - first, schedule, schedule_seed, first_iteration=first)
         end = previous + time.perf_counter() - started
         # one timestamp for the whole device run: the records share it
         records += [(first + k + 1, float(o), end) for k, o in enumerate(objs)]

@@ -101,14 +101,25 @@ typedef enum fpmhip_precision {
   FPMHIP_MIXED_F32 = 1
 } fpmhip_precision;
 
-/* FrameSchedule (src/reconstruction/schedule.rs:10-27). RandomShuffle needs
- * rand's StdRng stream and is expressed by passing an explicit order instead. */
+/* FrameSchedule (src/reconstruction/schedule.rs:10-27). RandomShuffle's
+ * permutation for (seed, iteration) is this library's own (a Fisher-Yates
+ * shuffle driven by Philox4x64-10 under key (seed + iteration *
+ * 0x9e3779b97f4a7c15, 1), counter (i, 0, 0, 0)): seeded per iteration as the
+ * reference's is, but not rand's StdRng permutation. */
 typedef enum fpmhip_schedule {
   FPMHIP_SCHEDULE_SEQUENTIAL = 0,
   FPMHIP_SCHEDULE_BRIGHTFIELD_FIRST = 1,
   FPMHIP_SCHEDULE_SPIRAL_OUT = 2,
-  FPMHIP_SCHEDULE_SNR_WEIGHTED = 3
+  FPMHIP_SCHEDULE_SNR_WEIGHTED = 3,
+  FPMHIP_SCHEDULE_RANDOM_SHUFFLE = 4
 } fpmhip_schedule;
+
+/* A FrameSchedule value: kind plus RandomShuffle's seed. */
+typedef struct fpmhip_frame_schedule {
+  int32_t kind;
+  int32_t pad_;
+  uint64_t seed;
+} fpmhip_frame_schedule;
 
 /* ---- physical inputs of ImagePlaneModel::from_experiment ---- */
 
@@ -382,6 +393,59 @@ int fpmhip_session_schedule(fpmhip_session* session, int32_t schedule, uint64_t*
  * order; trace [iterations] of objectives (may be NULL). */
 int fpmhip_session_run(fpmhip_session* session, uint64_t iterations, int32_t schedule,
                        double* trace);
+/* FrameSchedule::order_for_problem for zero-based iteration `iteration`
+ * (schedule.rs:35-104; only RandomShuffle depends on it). SnrWeighted scores
+ * the frames on the device from the resident measurements. */
+int fpmhip_session_schedule_at(fpmhip_session* session, const fpmhip_frame_schedule* schedule,
+                               uint64_t iteration, uint64_t* order);
+/* The Runner's iteration loop (runner.rs:156-163) for iterations
+ * first_iteration .. first_iteration + iterations - 1 (zero-based), each in
+ * its own schedule order; trace [iterations] (may be NULL). */
+int fpmhip_session_run_schedule(fpmhip_session* session, uint64_t iterations,
+                                const fpmhip_frame_schedule* schedule, uint64_t first_iteration,
+                                double* trace);
+
+/* ---- diagnostics: build_diagnostics (src/reconstruction/runner.rs:298-433) ----
+ * Of the current state, computed on the device: the forward prediction of
+ * every frame with the state's frame gains and background, then per frame
+ * one reduction over its pixels. IntensityStats (src/metrics/intensity/
+ * single.rs:7-61) of the measured frames and IntensityComparisonMetrics
+ * (src/metrics/intensity/compare.rs:88-110,134-226; reference = measured,
+ * estimate = predicted, unmasked pixels) without a saturation threshold,
+ * as the runner computes them. */
+typedef struct fpmhip_intensity_stats {
+  double mean, std, min, max, sum;
+  uint64_t saturated_pixels, zero_pixels;
+} fpmhip_intensity_stats;
+
+typedef struct fpmhip_frame_comparison {
+  double reference_sum, estimate_sum, residual_l1, residual_l2, residual_mean, residual_std,
+      residual_max_abs, normalized_l2;
+} fpmhip_frame_comparison;
+
+/* Each non-NULL output is a DiagnosticRequest (src/diagnostics/context.rs:7-24):
+ * per_frame_objective [F] (PerFrameError without a step: mean squared
+ * amplitude residual over unmasked pixels, 0 for a fully masked frame),
+ * raw_frame_statistics [F] (RawFrameStats), frame_summaries [F]
+ * (FrameSummaries), residual_images [F*N] (ResidualImages: predicted -
+ * measured, 0 where masked), object_amplitude / object_phase [M]
+ * (ObjectAmplitude / ObjectPhase of state_object), pupil_amplitude /
+ * pupil_phase [N] (Pupil). Errors as the reference's: a non-finite measured
+ * value in RawFrameStats or a non-finite pair in FrameSummaries is
+ * Numerical; a frame with no unmasked pixel in FrameSummaries is
+ * InvalidParameter "valid_mask". */
+typedef struct fpmhip_diagnostics {
+  double* per_frame_objective;
+  fpmhip_intensity_stats* raw_frame_statistics;
+  fpmhip_frame_comparison* frame_summaries;
+  double* residual_images;
+  double* object_amplitude;
+  double* object_phase;
+  double* pupil_amplitude;
+  double* pupil_phase;
+} fpmhip_diagnostics;
+
+int fpmhip_session_diagnostics(fpmhip_session* session, const fpmhip_diagnostics* out);
 /* Downloads (complex arrays interleaved). object = state_object (result.rs:397-413). */
 int fpmhip_session_get_spectrum(fpmhip_session* session, double* out);
 int fpmhip_session_get_pupil(fpmhip_session* session, double* out);

@@ -1433,6 +1433,90 @@ int orc_camera_measure(const fpmhip_camera* camera, double* values, uint64_t fra
   });
 }
 
+// stats (src/metrics/intensity/single.rs:25-61): out = mean, std, min, max,
+// sum, saturated_pixels, zero_pixels.
+int orc_intensity_stats(const double* v, uint64_t n, int32_t has_saturation, double saturation, double* out) {
+  return guarded([&] {
+    if (n == 0) fail(FPMHIP_INVALID_SHAPE, "intensity statistics require non-empty values");
+    double sum = 0.0, sq = 0.0, lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    uint64_t sat = 0, zero = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      const double x = v[i];
+      if (!std::isfinite(x)) fail(FPMHIP_NUMERICAL, "intensity statistics contain a non-finite value");
+      sum += x;
+      sq += x * x;
+      lo = std::min(lo, x);
+      hi = std::max(hi, x);
+      zero += x == 0.0;
+      sat += has_saturation && x >= saturation;
+    }
+    const double mean = sum / (double)n;
+    out[0] = mean;
+    out[1] = std::sqrt(std::max(sq / (double)n - mean * mean, 0.0));
+    out[2] = lo;
+    out[3] = hi;
+    out[4] = sum;
+    out[5] = (double)sat;
+    out[6] = (double)zero;
+  });
+}
+
+// compare_intensity_u8_masked (src/metrics/intensity/compare.rs:134-226):
+// out = reference_sum, estimate_sum, residual_l1, residual_l2, residual_mean,
+// residual_std, residual_max_abs, normalized_l2, saturated_pixels.
+int orc_compare_intensity(const double* reference, const double* estimate, const uint8_t* mask, uint64_t n,
+                          int32_t has_saturation, double saturation, double* out) {
+  return guarded([&] {
+    if (n == 0) fail(FPMHIP_INVALID_SHAPE, "intensity comparison inputs have lengths 0 and 0");
+    double rs = 0.0, es = 0.0, l1 = 0.0, r2 = 0.0, rsum = 0.0, rmax = 0.0, ref2 = 0.0;
+    uint64_t sat = 0, count = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (mask && mask[i] == 0) continue;
+      const double y = reference[i], e = estimate[i];
+      if (!std::isfinite(y) || !std::isfinite(e))
+        fail(FPMHIP_NUMERICAL, "intensity comparison contains a non-finite value");
+      const double r = e - y;
+      rs += y;
+      es += e;
+      l1 += std::fabs(r);
+      r2 += r * r;
+      rsum += r;
+      rmax = std::max(rmax, std::fabs(r));
+      ref2 += y * y;
+      sat += has_saturation && y >= saturation;
+      ++count;
+    }
+    if (count == 0) fail(FPMHIP_INVALID_PARAMETER, "must select at least one pixel", "valid_mask");
+    const double c = (double)count, mean = rsum / c;
+    out[0] = rs;
+    out[1] = es;
+    out[2] = l1;
+    out[3] = std::sqrt(r2);
+    out[4] = mean;
+    out[5] = std::sqrt(std::max(r2 / c - mean * mean, 0.0));
+    out[6] = rmax;
+    out[7] = std::sqrt(r2) / (std::sqrt(ref2) + std::numeric_limits<double>::epsilon());
+    out[8] = (double)sat;
+  });
+}
+
+// The runner's per-frame objective of a state (runner.rs:361-373): mean of
+// (sqrt(max(predicted, 0)) - sqrt(max(measured, 0)))^2 over unmasked pixels.
+int orc_frame_objective(const double* predicted, const double* measured, const uint8_t* mask, uint64_t n,
+                        double* out) {
+  return guarded([&] {
+    double sum = 0.0;
+    uint64_t valid = 0;
+    for (uint64_t i = 0; i < n; ++i)
+      if (!mask || mask[i] != 0) {
+        const double r = std::sqrt(std::max(predicted[i], 0.0)) - std::sqrt(std::max(measured[i], 0.0));
+        sum += r * r;
+        ++valid;
+      }
+    *out = valid == 0 ? 0.0 : sum / (double)valid;
+  });
+}
+
 // point_loss (src/algorithms/objective.rs:47-71) of n value pairs.
 int orc_point_loss(const double* predicted, const double* measured, uint64_t n, int32_t loss, double* out) {
   return guarded([&] {

@@ -341,3 +341,40 @@ def run(model, meas, algorithm: abi.Algorithm, iterations, schedule_name="sequen
     res["amplitude"] = np.abs(res["object"])
     res["phase"] = np.angle(res["object"])
     return res
+
+
+def intensity_stats(values, saturation=None):
+    """stats (src/metrics/intensity/single.rs:25-61) as a dict."""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    out = np.zeros(7)
+    _check(lib().orc_intensity_stats(_cd(v), ct.c_uint64(v.size), ct.c_int32(saturation is not None),
+                                     ct.c_double(saturation or 0.0), _cd(out)))
+    keys = ("mean", "std", "min", "max", "sum", "saturated_pixels", "zero_pixels")
+    return {k: (int(x) if k.endswith("pixels") else float(x)) for k, x in zip(keys, out)}
+
+
+def compare_intensity(reference, estimate, mask=None, saturation=None):
+    """compare_intensity_u8_masked (src/metrics/intensity/compare.rs:134-226) as a dict."""
+    r = np.ascontiguousarray(reference, dtype=np.float64).ravel()
+    e = np.ascontiguousarray(estimate, dtype=np.float64).ravel()
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    out = np.zeros(9)
+    _check(lib().orc_compare_intensity(_cd(r), _cd(e), None if m is None else abi.ptr(m, ct.c_uint8),
+                                       ct.c_uint64(r.size), ct.c_int32(saturation is not None),
+                                       ct.c_double(saturation or 0.0), _cd(out)))
+    keys = ("reference_sum", "estimate_sum", "residual_l1", "residual_l2", "residual_mean", "residual_std",
+            "residual_max_abs", "normalized_l2")
+    d = {k: float(x) for k, x in zip(keys, out)}
+    d["saturated_pixels"] = int(out[8]) if saturation is not None else None
+    return d
+
+
+def frame_objective(predicted, measured, mask=None):
+    """The runner's per-frame amplitude objective of a state (runner.rs:361-373)."""
+    p = np.ascontiguousarray(predicted, dtype=np.float64).ravel()
+    y = np.ascontiguousarray(measured, dtype=np.float64).ravel()
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    out = ct.c_double(0.0)
+    _check(lib().orc_frame_objective(_cd(p), _cd(y), None if m is None else abi.ptr(m, ct.c_uint8),
+                                     ct.c_uint64(p.size), ct.byref(out)))
+    return out.value
