@@ -62,7 +62,7 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.field = m.nh > kFusedLdsMaxN ? s.b.field.as<float2>() : nullptr;  // the session's N-pixel field buffer
   t.group = s.b.group.as<uint64_t>();
   t.partial = s.b.group.ptr ? reinterpret_cast<double2*>(s.b.group.as<char>() + 64) : nullptr;
-  t.edge = s.b.group.ptr ? reinterpret_cast<float2*>(s.b.group.as<char>() + 64 + 16 * kGroupMembers) : nullptr;
+  t.edge = s.b.group.ptr ? reinterpret_cast<float2*>(s.b.group.as<char>() + 64 + 16 * kMaxGroupMembers) : nullptr;
   t.member = 0;
   t.members = 1;
   t.frames = me.d_frames.ptr;
@@ -117,16 +117,22 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
     if (sorted[i].n > kFusedLdsMaxN && sorted[i].group) big.push_back(i);
   std::vector<char> grouped(sorted.size(), 0);
   if (members > 1 && !big.empty()) {
-    const uint64_t room = grid_limit > sorted.size() ? (grid_limit - sorted.size()) / (members - 1) : 0;
+    // each grouped tile adds group_members(n) - 1 workgroups to the grid
+    uint64_t room = grid_limit > sorted.size() ? grid_limit - sorted.size() : 0;
     std::stable_sort(big.begin(), big.end(), [&](uint32_t x, uint32_t y) { return sorted[x].F > sorted[y].F; });
-    for (uint64_t k = 0; k < big.size() && k < room; ++k) grouped[big[k]] = 1;
+    for (uint32_t k : big) {
+      const uint64_t extra = group_members(sorted[k].n) - 1;
+      if (extra > room) continue;
+      grouped[k] = 1;
+      room -= extra;
+    }
   }
   std::vector<TileDesc> out;
-  out.reserve(sorted.size() + big.size() * (members - 1));
+  out.reserve(sorted.size() + big.size() * (kMaxGroupMembers - 1));
   for (uint32_t i = 0; i < sorted.size(); ++i) {
     TileDesc t = sorted[i];
-    if (radix) t.plan = radix[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3];
-    const uint32_t g = grouped[i] ? members : 1;
+    if (radix && t.n <= 256) t.plan = radix[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3];
+    const uint32_t g = grouped[i] ? group_members(t.n) : 1;
     for (uint32_t m = 0; m < g; ++m) {
       out.push_back(t);
       out.back().member = m;
@@ -137,9 +143,9 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
 }
 
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream) {
-  // packed tables W_n^j (forward, inverse) for n = 32, 64, 128, 256
+  // packed tables W_n^j (forward, inverse) for n = 32, 64, 128, 256, 512
   std::vector<float> tw(2 * kTwTotal);
-  for (uint32_t n : {32u, 64u, 128u, 256u}) {
+  for (uint32_t n : {32u, 64u, 128u, 256u, 512u}) {
     const uint32_t off = tw_offset(n);
     for (uint32_t j = 0; j < n; ++j)
       for (int d = 0; d < 2; ++d) {
@@ -163,8 +169,8 @@ bool sweep_supports(const fpmhip_session& s, const char** why) {
     *why = "batched tiles must use MIXED_F32 sessions";
     return false;
   }
-  if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128 && m.nh != 256)) {
-    *why = "batched tiles must be square 32, 64, 128 or 256 pixel images";
+  if (m.nh != m.nw || (m.nh != 32 && m.nh != 64 && m.nh != 128 && m.nh != 256 && m.nh != 512)) {
+    *why = "batched tiles must be square 32, 64, 128, 256 or 512 pixel images";
     return false;
   }
   if (m.mux) {

@@ -16,7 +16,7 @@ struct TileDesc {
   const uint32_t* extent; // [n] per-row support of the pupil: columns [lo, hi) as lo | hi << 16
   const uint32_t* dest;   // live-pixel lists, tiles <= 128^2: [0] insert destinations, [1] support
                           // strips, then (n+1)^2 slots of destinations a | b << 16 and the strips
-                          // r | c << 8 | mask << 16 (SessionBuffers::dest)
+                          // r | c << 10 | mask << 20 (SessionBuffers::dest)
   float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)
   float2* patch;          // [N] the patch of the support pixels (EPRY pupil recovery)
   double2* pupil64;       // [N] FP64 master of the recovered pupil (EPRY pupil recovery)
@@ -38,7 +38,7 @@ struct TileDesc {
   // share one tile; this descriptor is workgroup `member` of the group
   uint64_t* group;        // [0] barrier arrivals, [1] barriers completed before this launch, [2] stall flag
   double2* partial;       // [members] per-member objective sums (loss, count)
-  float2* edge;           // [N / kBigChunk][N] last row of U of each chunk
+  float2* edge;           // [N / big_chunk(N)][N] last row of U of each chunk
   uint32_t member, members;
   uint32_t plan;          // first-stage radix of the LDS line transforms (0: default)
   uint32_t trace_len;
@@ -53,15 +53,15 @@ struct TileDesc {
 };
 
 // Offset (in float2 units) of the forward/inverse twiddle tables of length n
-// inside the packed table: [32 fwd, 32 inv, 64 fwd, 64 inv, ..., 256 inv].
+// inside the packed table: [32 fwd, 32 inv, 64 fwd, 64 inv, ..., 512 inv].
 __host__ __device__ inline uint32_t tw_offset(uint32_t n) {
-  return n == 32 ? 0u : n == 64 ? 64u : n == 128 ? 192u : 448u;
+  return n == 32 ? 0u : n == 64 ? 64u : n == 128 ? 192u : n == 256 ? 448u : 960u;
 }
-constexpr uint32_t kTwTotal = 2 * (32 + 64 + 128 + 256);
+constexpr uint32_t kTwTotal = 2 * (32 + 64 + 128 + 256 + 512);
 // Largest tile whose whole field is LDS-resident; larger tiles (up to
 // kFusedMaxN) keep it in TileDesc::field.
 constexpr uint32_t kFusedLdsMaxN = 128;
-constexpr uint32_t kFusedMaxN = 256;
+constexpr uint32_t kFusedMaxN = 512;
 
 size_t sweep_lds_bytes(uint32_t nmax);
 // Descriptor of one MIXED_F32 session's tile; per16 points at 16 device bytes
@@ -70,20 +70,25 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
                     uint32_t trace_len);
 // Launch order of a sweep's tiles: largest first (stable), so the longest
 // workgroups are dispatched first and spread over the XCDs, or session
-// order. Tiles above kFusedLdsMaxN become cooperative groups of `members`
-// descriptors (1 or kGroupMembers) as far as the whole grid stays within
-// `grid_limit` resident workgroups, tiles with more LEDs first; the others
-// run on one workgroup. radix[4]: first-stage radix per tile size 32, 64,
-// 128, 256 (0: default; null: keep the descriptors' plans).
+// order. With `members` > 1, tiles above kFusedLdsMaxN become cooperative
+// groups of group_members(n) descriptors as far as the whole grid stays
+// within `grid_limit` resident workgroups, tiles with more LEDs first; the
+// others run on one workgroup. radix[4]: first-stage radix per tile size
+// 32, 64, 128, 256 (0: default; null: keep the descriptors' plans; 512^2
+// tiles have one plan).
 std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,
                                   bool largest_first, uint32_t grid_limit);
 // First-stage radix choices of the fused sweep for an n x n tile; the first
 // entry is the default. Returns the number of choices (0: size unsupported).
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]);
-// Lines per chunk of a 256^2 tile's field, and the workgroups of its
-// cooperative group (one chunk each).
-constexpr uint32_t kBigChunk = 32;
-constexpr uint32_t kGroupMembers = kFusedMaxN / kBigChunk;
+// Lines per chunk of a large tile's field (256^2: 32, 512^2: 16, so that
+// a chunk fits LDS), and the workgroups of its cooperative group (one chunk
+// each). kGroupMembers is the group size of a 256^2 tile, the value that
+// turns cooperative groups on in a batch schedule.
+__host__ __device__ constexpr uint32_t big_chunk(uint32_t n) { return n >= 512 ? 16u : 32u; }
+__host__ __device__ constexpr uint32_t group_members(uint32_t n) { return n / big_chunk(n); }
+constexpr uint32_t kGroupMembers = group_members(256);
+constexpr uint32_t kMaxGroupMembers = group_members(kFusedMaxN);
 // Bytes of a session's cooperative-group state for an n x n tile.
 size_t group_state_bytes(uint32_t n);
 // Largest grid whose workgroups are all resident at once (one per CU at the

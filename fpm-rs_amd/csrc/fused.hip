@@ -629,7 +629,7 @@ __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcPa
   const double2 zero = make_double2(0.0, 0.0);
   for (uint32_t i = (uint32_t)tid; i < ns; i += kThreads) {
     const uint32_t e = i == (uint32_t)tid ? en : (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
-    const uint32_t r0 = e & 0xFFu, c = (e >> 8) & 0xFFu, mask = e >> 16;
+    const uint32_t r0 = e & 0x3FFu, c = (e >> 10) & 0x3FFu, mask = e >> 20;
     const uint32_t o = (sp.ar.lower + r0) * rw + sp.ac.lower + c;
     double2 t0[S + 1], t1[S + 1];
     float2 P[S];
@@ -675,8 +675,8 @@ __device__ __forceinline__ void gather_strips_rows(Rsrc rspec, Rsrc rpup, const 
   const double2 zero = make_double2(0.0, 0.0);
   for (uint32_t i = s0 + (uint32_t)tid; i < s1; i += kThreads) {
     const uint32_t e = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
-    const int r0 = (int)(e & 0xFFu), c = (int)((e >> 8) & 0xFFu);
-    uint32_t mask = e >> 16;
+    const int r0 = (int)(e & 0x3FFu), c = (int)((e >> 10) & 0x3FFu);
+    uint32_t mask = e >> 20;
 #pragma unroll
     for (int j = 0; j < S; ++j)  // rows of other chunks
       if (r0 + j < R0 || r0 + j >= R0 + nk) mask &= ~(1u << j);
@@ -1013,8 +1013,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         double pmax = 0.0;
         for (uint32_t i = (uint32_t)tid; i < nstrip; i += kThreads) {
           const uint32_t e = i == (uint32_t)tid ? my_strip : (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
-          const int r0 = (int)(e & 0xFFu), c = (int)((e >> 8) & 0xFFu);
-          const uint32_t mask = e >> 16;
+          const int r0 = (int)(e & 0x3FFu), c = (int)((e >> 10) & 0x3FFu);
+          const uint32_t mask = e >> 20;
           const int cn = (c + HALF) % N;
           float2 P[S];
 #pragma unroll
@@ -1244,20 +1244,38 @@ __device__ __forceinline__ bool group_barrier(const TileDesc& T, uint64_t& passe
 // and arithmetic are those of sweep_tile, and the spectra do not depend on
 // the group size.
 template <int N>
-struct BigPlan;
-template <>
-struct BigPlan<256> {
-  static constexpr int CH = (int)kBigChunk;  // lines per chunk
+struct BigPlan {
+  static constexpr int CH = (int)big_chunk(N);  // lines per chunk
 };
+
+// The line transforms of a large tile: two stages (radix P0, then N / P0)
+// for 256, three radix-8 stages for 512 (radices of at most 16 per stage).
+template <int N, int LINES, int LS, int ES, bool INV, bool LIVE = false>
+__device__ __forceinline__ void big_lines(float2* lds, const float2* tw, int tid, LiveRows lr = {LINES, 0, 0.0f}) {
+  if constexpr (N == 512) {
+    stage<N, LINES, 8, 1, LS, ES, INV, LIVE>(lds, tw, tid, lr);
+    stage<N, LINES, 8, 8, LS, ES, INV, LIVE>(lds, tw, tid, lr);
+    stage<N, LINES, 8, 64, LS, ES, INV, LIVE>(lds, tw, tid, lr);
+  } else {
+    constexpr int P0 = DefaultPlan<N>::P0, P1 = N / P0;
+    stage<N, LINES, P0, 1, LS, ES, INV, LIVE>(lds, tw, tid, lr);
+    stage<N, LINES, P1, P0, LS, ES, INV, LIVE>(lds, tw, tid, lr);
+  }
+}
 
 template <int N>
 __host__ __device__ constexpr int big_field_elems() {
   return BigPlan<N>::CH * (N + 1) > N * (BigPlan<N>::CH + 1) ? BigPlan<N>::CH * (N + 1) : N * (BigPlan<N>::CH + 1);
 }
 
+// One chunk region for every large tile size (a single constant keeps the
+// LDS offsets of all sizes one select apart).
+constexpr uint32_t kBigFieldElems =
+    big_field_elems<256>() > big_field_elems<512>() ? big_field_elems<256>() : big_field_elems<512>();
+
 // float2 elements of the LDS field region for an N x N tile.
 __host__ __device__ constexpr uint32_t sweep_field_elems(uint32_t n) {
-  return n <= kFusedLdsMaxN ? n * (n + 1) : (uint32_t)big_field_elems<256>();
+  return n <= kFusedLdsMaxN ? n * (n + 1) : kBigFieldElems;
 }
 
 template <int N, class MEM>
@@ -1270,7 +1288,6 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   constexpr int NCH = N / CH;
   constexpr int LDR = N + 1;   // row chunk: CH rows of N (+1 pad)
   constexpr int LDC = CH + 1;  // column chunk: N rows of CH (+1 pad)
-  constexpr int P0 = DefaultPlan<N>::P0, P1 = N / P0;
   constexpr bool COOP = MEM::kCoherent;
   static_assert((N & (N - 1)) == 0 && HALF % CH == 0, "big tile shape");
   // a cooperative group has one member per chunk; a lone workgroup runs them all
@@ -1352,8 +1369,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
                                           nk, lds);
         }
         __syncthreads();
-        stage<N, CH, P0, 1, LDR, 1, true, true>(lds, twi, tid, live_rows(nk));
-        stage<N, CH, P1, P0, LDR, 1, true, true>(lds, twi, tid, live_rows(nk));
+        big_lines<N, CH, LDR, 1, true, true>(lds, twi, tid, live_rows(nk));
         rows_out<CH, N, LDR, MEM>(lds, fld, nk, opaque_tid(), [&](int j) { return (R0 + j) ^ HALF; });
         __syncthreads();
       }
@@ -1371,8 +1387,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           return cr >= rlo && cr <= rhi;
         });
         __syncthreads();
-        stage<N, CH, P0, 1, 1, LDC, true>(lds, twi, tid);
-        stage<N, CH, P1, P0, 1, LDC, true>(lds, twi, tid);
+        big_lines<N, CH, 1, LDC, true>(lds, twi, tid);
         if (plain_frames) {
           // branch-free (as in sweep_tile): pixel (row0 + j RSTEP, c0 + col)
           constexpr int PP = N * CH / kThreads, RSTEP = kThreads / CH;
@@ -1423,8 +1438,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           lsum += (double)lsum_f;
         }
         __syncthreads();
-        stage<N, CH, P0, 1, 1, LDC, false>(lds, twf, tid);
-        stage<N, CH, P1, P0, 1, LDC, false>(lds, twf, tid);
+        big_lines<N, CH, 1, LDC, false>(lds, twf, tid);
         copy_out<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid());
         __syncthreads();
       }
@@ -1464,8 +1478,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         const int nk = chunk_rows(q), R0 = rlo + q * crows;
         rows_in<CH, N, LDR, MEM>(lds, fld, nk, opaque_tid(), [&](int j) { return (R0 + j) ^ HALF; });
         __syncthreads();
-        stage<N, CH, P0, 1, LDR, 1, false, true>(lds, twf, tid, live_rows(nk));
-        stage<N, CH, P1, P0, LDR, 1, false, true>(lds, twf, tid, live_rows(nk));
+        big_lines<N, CH, LDR, 1, false, true>(lds, twf, tid, live_rows(nk));
         for (int i = tid; i < nk * N; i += kThreads) {
           const int lr = i / N, kc = i % N;
           const bool in = in_extent(ext[R0 + lr], kc ^ HALF);  // U is zero outside the support
@@ -1533,6 +1546,10 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   if (COOP && T.member == 0 && threadIdx.x == 0) T.group[1] = passed;
 }
 
+// The fused sweep: k_sweep<false> takes tiles up to 256^2, k_sweep<true>
+// every size up to 512^2 (two kernels, so that the 512^2 path's registers
+// do not cost the batches without such tiles).
+template <bool LARGE>
 __global__ void __launch_bounds__(kThreads)
     k_sweep(const TileDesc* __restrict__ tiles, const float2* __restrict__ tw_all) {
   extern __shared__ unsigned char smem[];
@@ -1579,10 +1596,21 @@ __global__ void __launch_bounds__(kThreads)
       else
         sweep_tile_big<256, PlainMem>(T, lds, prev, twf, twi, red, bflag, ext, rec);
       break;
+    case 512:
+      if constexpr (LARGE) {
+        if (T.members > 1)
+          sweep_tile_big<512, CoherentMem>(T, lds, prev, twf, twi, red, bflag, ext, rec);
+        else
+          sweep_tile_big<512, PlainMem>(T, lds, prev, twf, twi, red, bflag, ext, rec);
+      }
+      break;
     default:
       break;
   }
 }
+
+template __global__ void k_sweep<false>(const TileDesc*, const float2*);
+template __global__ void k_sweep<true>(const TileDesc*, const float2*);
 
 }  // namespace
 
@@ -1649,18 +1677,26 @@ uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
     case 256:
       out[0] = 16;
       return 1;
+    case 512:
+      out[0] = 8;  // three radix-8 stages
+      return 1;
     default:
       return 0;
   }
 }
 
 size_t group_state_bytes(uint32_t n) {
-  // [0, 24) counters, [64, 64 + 16 kGroupMembers) partial sums, then n / kBigChunk edge rows
-  return 64 + 16 * (size_t)kGroupMembers + (size_t)(n / kBigChunk) * n * sizeof(float2);
+  // [0, 24) counters, [64, 64 + 16 kMaxGroupMembers) partial sums, then group_members(n) edge rows
+  return 64 + 16 * (size_t)kMaxGroupMembers + (size_t)group_members(n) * n * sizeof(float2);
+}
+
+// The sweep kernel for tiles up to nmax.
+static const void* sweep_kernel_for(uint32_t nmax) {
+  return nmax > 256 ? reinterpret_cast<const void*>(&k_sweep<true>) : reinterpret_cast<const void*>(&k_sweep<false>);
 }
 
 void prepare_sweep(uint32_t nmax) {
-  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep), hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIP_CHECK(hipFuncSetAttribute(sweep_kernel_for(nmax), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)sweep_lds_bytes(nmax)));
   HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep_epry),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1671,7 +1707,7 @@ uint32_t sweep_coresident_limit(uint32_t nmax) {
   int dev = 0, per_cu = 0, cus = 0;
   HIP_CHECK(hipGetDevice(&dev));
   HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_sweep), kThreads,
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel_for(nmax), kThreads,
                                                          sweep_lds_bytes(nmax)));
   return (uint32_t)(std::max(per_cu, 0) * std::max(cus, 0));
 }
@@ -1687,10 +1723,13 @@ void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_al
   if (cooperative) {
     // every workgroup of every cooperative group must be resident at once
     void* args[] = {(void*)&tiles_dev, (void*)&tw_all};
-    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_sweep), dim3(count), dim3(kThreads), args,
-                                         (unsigned)smem, stream));
+    HIP_CHECK(hipLaunchCooperativeKernel(sweep_kernel_for(nmax), dim3(count), dim3(kThreads), args, (unsigned)smem,
+                                         stream));
+  } else if (nmax > 256) {
+    hipLaunchKernelGGL(k_sweep<true>, dim3(count), dim3(kThreads), smem, stream, tiles_dev, tw_all);
+    HIP_CHECK(hipGetLastError());
   } else {
-    hipLaunchKernelGGL(k_sweep, dim3(count), dim3(kThreads), smem, stream, tiles_dev, tw_all);
+    hipLaunchKernelGGL(k_sweep<false>, dim3(count), dim3(kThreads), smem, stream, tiles_dev, tw_all);
     HIP_CHECK(hipGetLastError());
   }
 }

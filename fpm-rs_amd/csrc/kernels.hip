@@ -892,8 +892,8 @@ __global__ void k_pupil_extent(const float2* __restrict__ pupil, uint32_t h, uin
 // the update U is zero outside the support. Support pixels: row r, columns
 // of its extent (the gather reads nothing elsewhere). Layout: dest[0] and
 // dest[1] the two counts, then the destinations a | b << 16 and the support
-// pixels r | c << 16, both row-major.
-constexpr uint32_t kDestMaxRows = 257;
+// strips r | c << 10 | mask << 20, both row-major.
+constexpr uint32_t kDestMaxRows = 513;
 __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, uint32_t* __restrict__ dest) {
   __shared__ uint32_t lo_s[kDestMaxRows], len_s[kDestMaxRows], start_s[kDestMaxRows + 1];
   __shared__ uint32_t top_s[kDestMaxRows], cnt_s[kDestMaxRows], cstart_s[kDestMaxRows];
@@ -961,7 +961,7 @@ __global__ void k_dest_table(const uint32_t* __restrict__ extent, uint32_t n, ui
         const uint32_t e = extent[r0 + i];
         if (c >= (e & 0xFFFFu) && c < (e >> 16)) mask |= 1u << i;
       }
-      sl[cstart_s[k] + j] = r0 | (c << 8) | (mask << 16);
+      sl[cstart_s[k] + j] = r0 | (c << 10) | (mask << 20);
     }
   }
 }

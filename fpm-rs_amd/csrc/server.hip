@@ -463,7 +463,7 @@ int fpmhip_server_create(fpmhip_context* c, const fpmhip_server_config* config, 
     }
     s->ctx = &s->own;
     s->cfg = *config;
-    s->wg_limit = std::max<uint32_t>(sweep_coresident_limit(kFusedMaxN), kGroupMembers);
+    s->wg_limit = std::max<uint32_t>(sweep_coresident_limit(kFusedMaxN), kGroupMembers);  // tiles up to 256^2
     make_sweep_twiddles(s->tw, s->own.stream);
     s->tiles.allocate(config->max_tiles * sizeof(TileDesc));
     fpmhip_server* raw = s.get();
@@ -498,6 +498,10 @@ int fpmhip_server_submit(fpmhip_server* s, const fpmhip_model_desc* d, const uin
       fail(FPMHIP_INVALID_MEASUREMENTS, "exactly one of uint16 counts and f64 frames is required");
     if (int rc = fpmhip_algorithm_validate(alg)) throw Failure{rc};
     if (alg->iterations == 0) fail(FPMHIP_INVALID_PARAMETER, "a served job needs at least one iteration", "iterations");
+    // the tick balance is calibrated for groups of up to 256^2 tiles; larger
+    // tiles run in a Batch
+    if (d->image_h > 256 || d->image_w > 256)
+      fail(FPMHIP_UNSUPPORTED, "served jobs take images of at most 256 x 256 pixels");
     auto j = std::make_unique<Job>();
     const uint64_t N = d->image_h * d->image_w, S = d->source_count;
     const uint64_t F = d->mux_row_ptr ? d->frame_count : S;

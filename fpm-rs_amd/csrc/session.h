@@ -79,7 +79,7 @@ struct SessionBuffers {
   DeviceBuffer dest;      // uint32[dest_table_words(n)] live-pixel lists of the fused sweep (square
                           // MIXED_F32 tiles): [0] destinations of the adjoint insert, [1] support
                           // strips; then the destinations a | b << 16 (row-major), the strips
-                          // r | c << 8 | mask << 16 (kGatherStrip rows r.. of column c, bit j: pixel
+                          // r | c << 10 | mask << 20 (kGatherStrip rows r.. of column c, bit j: pixel
                           // r + j is in the support; band of rows by band, columns in order within a
                           // band) and the index of each window row's first destination
   DeviceBuffer mspan;     // uint32 momentum chunks of the fused sweep (MIXED_F32 mPIE): [0] count, then
