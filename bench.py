@@ -521,6 +521,13 @@ def main():
         psecs = run_plain(model, meas, args.iterations, 1, 1, "c128")
         line["plain_c128_tiles_per_s"] = 1.0 / psecs
         line["speedup_vs_plain_c128"] = tiles_per_s * psecs
+        # the plain path's whole-tile algorithmic bytes (object window, pupil,
+        # velocity and anchor all complex128) over its wall time
+        b128 = tile_bytes(model, args.iterations, 30, 2, 16, 16, 16)
+        line["plain_c128_roofline"] = {"bound": "hbm", "achieved": b128 / psecs / 1e9, "peak": HBM_PEAK_GBS,
+                                       "unit": "GB/s", "frac": b128 / psecs / 1e9 / HBM_PEAK_GBS,
+                                       "note": "per-stage kernels, one tile at a time, each sweep replayed "
+                                               "from a captured graph"}
         log("uneven batch leg")
         line["uneven_batch"] = run_uneven(40, 1, 1)
         log("EPRY batch leg")
