@@ -421,13 +421,36 @@ def cpu_baseline(tiles, iterations, threads):
             "fft_vs_pocketfft": fft}
 
 
+def copy_calibration(gib=1, reps=3):
+    """Counter calibration: `reps` device-to-device copies of `gib` GiB
+    (one __amd_rocclr_copyBuffer dispatch each). rocprofv3's FETCH_SIZE and
+    WRITE_SIZE of these dispatches, divided by the bytes each copy reads and
+    writes, are the factors that turn the counters of another kernel into
+    HBM bytes. Device memory comes from torch (plumbing only)."""
+    import torch
+
+    n = gib << 28  # float32 elements
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    return {"metric": "device copy (counter calibration)", "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
+            "copies": reps, "best_GBs": 2 * n * 4 / min(times) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=50)
-    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "mixed", "serving", "uneven", "epry"])
+    ap.add_argument("--mode", default="batched",
+                    choices=["batched", "plain", "mixed", "serving", "uneven", "epry", "copy"])
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--sustained-rate", type=float, default=40.0,
@@ -442,6 +465,9 @@ def main():
               "so more GPUs would each run their own batch of tiles, and no figure over several GPUs "
               "is measured", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "copy":
+        print(json.dumps(copy_calibration()), flush=True)
+        return
     if args.mode == "serving":
         res = serving(args.requests, args.rate, 30)
         sus = serving(args.requests, args.sustained_rate, 30) if args.sustained_rate > 0 else None

@@ -807,7 +807,7 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
     b.gains.allocate(std::max<uint64_t>(m->F, 1) * sizeof(double));
     load_calibration(*s);
     if (precision == FPMHIP_MIXED_F32 && m->nh == m->nw && m->nh > kFusedLdsMaxN && m->nh <= kFusedMaxN) {
-      b.group.allocate(group_state_bytes(m->nh));
+      b.group.allocate(group_state_bytes());
       HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
     }
     if (precision == FPMHIP_MIXED_F32 && s->cfg.has_pupil_step) b.pupil64.allocate(N * sizeof(double2));

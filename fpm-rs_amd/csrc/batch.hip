@@ -62,7 +62,6 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.field = m.nh > kFusedLdsMaxN ? s.b.field.as<float2>() : nullptr;  // the session's N-pixel field buffer
   t.group = s.b.group.as<uint64_t>();
   t.partial = s.b.group.ptr ? reinterpret_cast<double2*>(s.b.group.as<char>() + 64) : nullptr;
-  t.edge = s.b.group.ptr ? reinterpret_cast<float2*>(s.b.group.as<char>() + 64 + 16 * kMaxGroupMembers) : nullptr;
   t.member = 0;
   t.members = 1;
   t.frames = me.d_frames.ptr;

@@ -38,7 +38,6 @@ struct TileDesc {
   // share one tile; this descriptor is workgroup `member` of the group
   uint64_t* group;        // [0] barrier arrivals, [1] barriers completed before this launch, [2] stall flag
   double2* partial;       // [members] per-member objective sums (loss, count)
-  float2* edge;           // [N / big_chunk(N)][N] last row of U of each chunk
   uint32_t member, members;
   uint32_t plan;          // first-stage radix of the LDS line transforms (0: default)
   uint32_t trace_len;
@@ -89,8 +88,8 @@ __host__ __device__ constexpr uint32_t big_chunk(uint32_t n) { return n >= 512 ?
 __host__ __device__ constexpr uint32_t group_members(uint32_t n) { return n / big_chunk(n); }
 constexpr uint32_t kGroupMembers = group_members(256);
 constexpr uint32_t kMaxGroupMembers = group_members(kFusedMaxN);
-// Bytes of a session's cooperative-group state for an n x n tile.
-size_t group_state_bytes(uint32_t n);
+// Bytes of a session's cooperative-group state (counters and per-member sums).
+size_t group_state_bytes();
 // Largest grid whose workgroups are all resident at once (one per CU at the
 // sweep kernel's LDS size).
 uint32_t sweep_coresident_limit(uint32_t nmax);

@@ -5,10 +5,12 @@
 //
 //   gather 4-tap bilinear patch from the FP64 object spectrum (crop.rs:129-167)
 //   x pupil -> ifftshift -> LDS
-//   inverse 2-D FFT in LDS (Stockham radix-16/8 stages, FP32)
+//   inverse 2-D FFT in LDS (in-place radix-16/8 stages, decimation in
+//   frequency, FP32; 256^2 and 512^2 tiles: Stockham stages over chunks)
 //   |field|^2, objective terms (FP64), amplitude replacement (common.rs:123-228),
-//   kept as the field change psi' - psi
-//   forward 2-D FFT in LDS (x 1/N), fftshift: the exit-wave difference
+//   kept as the field change psi' - psi, in the inverse's digit-reversed order
+//   forward 2-D FFT in LDS (decimation in time, back to natural order; x 1/N),
+//   fftshift: the exit-wave difference
 //   U = conj(P) difference / denominator (common.rs:240-261)
 //   adjoint insert as a gather over the (n+1)^2 window, FP64 RMW (crop.rs:188-256)
 //   every `interval` counted frames: mPIE momentum over the tile's R^2 grid
@@ -111,48 +113,92 @@ __device__ __forceinline__ void stage(float2* lds, const float2* tw, int tid, Li
   __syncthreads();
 }
 
-// Two-stage radix plans of an N-point line transform: first stage radix P0,
-// second N / P0 (each at most 16). kDefaultP0 is the plan used unless a
-// batch schedule picks another.
+// Two-stage radix plans of an N-point line transform of a large tile: first
+// stage radix P0, second N / P0 (each at most 16).
 template <int N>
 struct DefaultPlan;
-template <>
-struct DefaultPlan<32> {
-  static constexpr int P0 = 8;
-};
-template <>
-struct DefaultPlan<64> {
-  static constexpr int P0 = 8;
-};
-template <>
-struct DefaultPlan<128> {
-  static constexpr int P0 = 16;
-};
 template <>
 struct DefaultPlan<256> {
   static constexpr int P0 = 16;
 };
 
-// Unscaled 2-D DFT in LDS with the direction given by `tw`. Only the rows
-// in `live` can be non-zero on input (inverse) or are read on output
-// (forward): the inverse transforms those rows and then every column; the
-// forward transforms every column and then those rows.
-template <int N, bool INV, int P0>
-__device__ __forceinline__ void fft2_lds(float2* lds, const float2* tw, int tid, LiveRows live) {
-  constexpr int P1 = N / P0;
-  constexpr int LD = pitch<N>();
-  static_assert(P0 * P1 == N && P0 <= 16 && P1 <= 16, "plan");
-  if (INV) {
-    stage<N, N, P0, 1, LD, 1, INV, true>(lds, tw, tid, live);
-    stage<N, N, P1, P0, LD, 1, INV, true>(lds, tw, tid, live);
-    stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
-    stage<N, N, P1, P0, 1, LD, INV>(lds, tw, tid);
-  } else {
-    stage<N, N, P0, 1, 1, LD, INV>(lds, tw, tid);
-    stage<N, N, P1, P0, 1, LD, INV>(lds, tw, tid);
-    stage<N, N, P0, 1, LD, 1, INV, true>(lds, tw, tid, live);
-    stage<N, N, P1, P0, LD, 1, INV, true>(lds, tw, tid, live);
+// Line of work item b of a stage over the live lines (b = t * nl + li,
+// the quotient from the FP32 reciprocal, corrected): returns the line and
+// sets t.
+__device__ __forceinline__ int live_line(int b, int lines, LiveRows lr, int& t) {
+  t = (int)((float)b * lr.rcp);
+  int li = b - t * lr.nl;
+  if (li < 0) {
+    --t;
+    li += lr.nl;
+  } else if (li >= lr.nl) {
+    ++t;
+    li -= lr.nl;
   }
+  return (lr.l0 + li) & (lines - 1);
+}
+
+// In-place two-stage transforms of the LDS-resident tile (N = P0 P1, each
+// radix at most 16). The inverse is decimation in frequency: from natural
+// order, stage 1 (radix P0) takes the samples g + P1 k of butterfly g < P1
+// and multiplies its output k by W^(g k); stage 2 (radix P1) transforms the
+// contiguous groups g P1 + m, g < P0. Position j1 + P1 j0 then holds output
+// j0 + P0 j1 (digit-reversed order). The forward transform is decimation in
+// time, the inverse's steps reversed: stage 1 (radix P1) on the contiguous
+// groups, output m of group g multiplied by W^(-g m), then stage 2 (radix P0)
+// on the samples g + P1 k, which leaves natural order. A butterfly writes
+// back to the slots it read and no other thread touches them during the
+// stage, so a stage needs one barrier (after its writes), not one between
+// its reads and writes as well.
+template <int P0, int P1>
+__device__ __forceinline__ int digit_rev(int p) {
+  return p / P1 + P0 * (p % P1);
+}
+
+// One in-place stage over LINES lines (sample e of line l at l * LS + e * ES):
+// radix P on the strided samples g + k N/P (STRIDED) or on the contiguous
+// groups g P + k; with TW, output k is multiplied by tw[g k]. Consecutive
+// lanes take consecutive lines, as in `stage`. With LIVE only the live lines
+// are transformed (work items packed onto the first threads).
+template <int N, int LINES, int P, bool STRIDED, bool TW, int LS, int ES, bool INV, bool LIVE = false>
+__device__ __forceinline__ void ip_stage(float2* lds, const float2* tw, int tid, LiveRows lr = {LINES, 0, 0.0f}) {
+  asm volatile("" : "+v"(tid));
+  constexpr int BPL = N / P;
+  constexpr int TOTAL = LINES * BPL;
+  constexpr int PER = (TOTAL + kThreads - 1) / kThreads;
+  constexpr int STEP = (STRIDED ? BPL : 1) * ES;
+  const int total = LIVE ? lr.nl * BPL : TOTAL;
+  float2 a[PER][P];
+  int base[PER], g[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int b = tid + u * kThreads;
+    int line;
+    if (LIVE) {
+      line = live_line(b, LINES, lr, g[u]);
+    } else {
+      line = b % LINES;
+      g[u] = b / LINES;
+    }
+    base[u] = line * LS + (STRIDED ? g[u] : g[u] * P) * ES;
+    if (b < total) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) a[u][k] = lds[base[u] + k * STEP];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    if (tid + u * kThreads < total) {
+      reg_dft<P, INV>(a[u]);
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        float2 v = a[u][k];
+        if (TW && k > 0) v = fmul(v, tw[g[u] * k]);
+        lds[base[u] + k * STEP] = v;
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // The thread index, hidden from the optimiser so that per-thread addresses
@@ -186,6 +232,22 @@ __device__ __forceinline__ double2 wg_sum2(double a, double b, double* red) {
       sb += red[kThreads / 64 + w];
     }
   return make_double2(sa, sb);  // valid in thread 0
+}
+
+// The first half of wg_sum2 without its barriers: each wave's sums of a and
+// b into red[wave] and red[kThreads / 64 + wave]. The caller's next barrier
+// publishes them (and a barrier must separate this from the last read of
+// red); thread 0 then adds the slots in wave order.
+__device__ __forceinline__ void wave_sums2_to(double* red, double a, double b) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = a;
+    red[kThreads / 64 + (threadIdx.x >> 6)] = b;
+  }
 }
 
 // A value that is the same in every lane, as a scalar register value.
@@ -834,73 +896,109 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       // other slot after the forward transform)
       uint32_t next_f = 0, next_w = 0;
       if (fetch) next_f = frame_at(T, npos);
-      // 2. inverse FFT (unscaled)
-      fft2_lds<N, true, RADIX>(lds, twi, tid, live);
+      // 2.-4. inverse 2-D FFT, amplitude replacement, forward 2-D FFT, in place:
+      //    inverse rows and columns by decimation in frequency (natural in,
+      //    digit-reversed out), then the inverse's last column stage, the
+      //    replacement and the forward's first column stage on the same
+      //    registers, then the forward columns and rows by decimation in time
+      //    (back to natural order; x 1/N applied in phase 5). Row stages take
+      //    only the live rows.
+      constexpr int P0 = RADIX, P1 = N / RADIX;
+      static_assert(P0 * P1 == N && P0 <= 16 && P1 <= 16, "plan");
+      ip_stage<N, N, P0, true, true, LD, 1, true, true>(lds, twi, tid, live);
+      ip_stage<N, N, P1, false, false, LD, 1, true, true>(lds, twi, tid, live);
       if (fetch) next_w = rec_word(T, next_f, threadIdx.x);
-      // 3. amplitude replacement and objective terms (pointwise, natural order).
-      //    The LDS field becomes psi' - psi: the forward transform is linear,
-      //    so FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
-      //    without cancelling two nearly equal spectra in FP32. Per-pixel terms
-      //    are FP32; the frame objective is accumulated in FP64.
+      // 3. amplitude replacement and objective terms, pointwise at the
+      //    natural pixel (digit_rev(r'), digit_rev(c')) of LDS slot (r', c').
+      //    The field becomes psi' - psi: the forward transform is linear, so
+      //    FFT(psi' - psi)/N equals FFT(psi')/N - patch P (common.rs:240-245)
+      //    without cancelling two nearly equal spectra in FP32. Per-pixel
+      //    terms are FP32; the frame objective is accumulated in FP64.
+      //    Work item b: column b % N, rows (b / N) P1 .. + P1 - 1.
+      constexpr int GROUPS = N * P0;
+      constexpr int PG = (GROUPS + kThreads - 1) / kThreads;
       const uint64_t fbase = (uint64_t)f * NN;
-      float lsum_f = 0.0f;
-      int lcnt_i = 0;
-      if (plain_frames) {
-        // branch-free: measurement loads at constant offsets, both cases of
-        // the ratio (common.rs:180-190) evaluated and selected
-        static_assert(NN % kThreads == 0, "pixels per thread");
-        constexpr int PP = NN / kThreads, RSTEP = kThreads / N;
-        const int row0 = tid / N, col = tid % N;
-        float y[PP];
-        if (T.is_u16) {
-          const Rsrc rm = make_rsrc(static_cast<const uint16_t*>(T.frames) + fbase, NN * 2u);
+      ip_stage<N, N, P0, true, true, 1, LD, true>(lds, twi, tid);
+      {
+        float lsum_f = 0.0f;
+        int lcnt_i = 0;
+        if (plain_frames) {
+          // the inverse's last column stage, the replacement (branch-free: both
+          // cases of the ratio, common.rs:180-190, evaluated and selected) and
+          // the forward's first column stage on the same registers
+          const int t = opaque_tid();
+          const Rsrc rm = T.is_u16 ? make_rsrc(static_cast<const uint16_t*>(T.frames) + fbase, NN * 2u)
+                                   : make_rsrc(static_cast<const double*>(T.frames) + fbase, NN * 8u);
 #pragma unroll
-          for (int j = 0; j < PP; ++j) y[j] = bld_u16f(rm, (uint32_t)(tid + j * kThreads) * 2u);
-        } else {
-          const Rsrc rm = make_rsrc(static_cast<const double*>(T.frames) + fbase, NN * 8u);
+          for (int u = 0; u < PG; ++u) {
+            const int b = t + u * kThreads, c = b % N, j0 = b / N, nc = digit_rev<P0, P1>(c);
+            if (b < GROUPS) {
+              float y[P1];
+              if (T.is_u16) {
 #pragma unroll
-          for (int j = 0; j < PP; ++j) y[j] = bld_f64f(rm, (uint32_t)(tid + j * kThreads) * 8u);
-        }
+                for (int m = 0; m < P1; ++m) y[m] = bld_u16f(rm, (uint32_t)((j0 + P0 * m) * N + nc) * 2u);
+              } else {
 #pragma unroll
-        for (int j = 0; j < PP; ++j) {
-          const int lp = (row0 + j * RSTEP) * LD + col;
-          const float2 fv = lds[lp];
-          lds[lp] = replace_plain(fv, y[j], sg, sig, eps_f, lsum_f);
-        }
-        lcnt_i = PP;
-      } else {
-        constexpr int PP = (NN + kThreads - 1) / kThreads;  // pixels per thread
-        float y[PP];
-        // issue every measurement load of the thread before using any of them
+                for (int m = 0; m < P1; ++m) y[m] = bld_f64f(rm, (uint32_t)((j0 + P0 * m) * N + nc) * 8u);
+              }
+              float2 a[P1];
 #pragma unroll
-        for (int j = 0; j < PP; ++j) {
-          const int p = tid + j * kThreads;
-          y[j] = 0.0f;
-          if (p < NN) y[j] = measured(T, fbase + p);
-        }
+              for (int m = 0; m < P1; ++m) a[m] = lds[(j0 * P1 + m) * LD + c];
+              reg_dft<P1, true>(a);
 #pragma unroll
-        for (int j = 0; j < PP; ++j) {
-          const int p = tid + j * kThreads;
-          if (p < NN) {
-            const int lp = (p / N) * LD + p % N;
-            lds[lp] =
-                replace_amplitude(T, fbase, (uint64_t)NN, p, lds[lp], y[j], gain_f, inv_gain, eps_f, lsum_f, lcnt_i);
+              for (int m = 0; m < P1; ++m) a[m] = replace_plain(a[m], y[m], sg, sig, eps_f, lsum_f);
+              reg_dft<P1, false>(a);
+#pragma unroll
+              for (int m = 0; m < P1; ++m) lds[(j0 * P1 + m) * LD + c] = m > 0 ? fmul(a[m], twf[j0 * m]) : a[m];
+              lcnt_i += P1;
+            }
           }
+        } else {
+          // masks, backgrounds or other losses: the stages and the
+          // replacement as separate passes over LDS
+          ip_stage<N, N, P1, false, false, 1, LD, true>(lds, twi, tid);
+          constexpr int PP = NN / kThreads;  // pixels per thread
+          static_assert(NN % kThreads == 0, "pixels per thread");
+          const int t = opaque_tid();
+#pragma unroll
+          for (int g0 = 0; g0 < PP; g0 += 4) {
+            float y[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int i = t + (g0 + j) * kThreads;
+              y[j] = measured(T, fbase + (uint64_t)(digit_rev<P0, P1>(i / N) * N + digit_rev<P0, P1>(i % N)));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int i = t + (g0 + j) * kThreads, lp = (i / N) * LD + i % N;
+              lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN,
+                                          digit_rev<P0, P1>(i / N) * N + digit_rev<P0, P1>(i % N), lds[lp], y[j],
+                                          gain_f, inv_gain, eps_f, lsum_f, lcnt_i);
+            }
+          }
+          __syncthreads();
+          ip_stage<N, N, P1, false, true, 1, LD, false>(lds, twf, tid);
         }
+        // per-thread partials (at most 16 terms) in FP32; wave sums in FP64
+        // into the reduction slots, read by thread 0 after the barrier
+        wave_sums2_to(red, (double)lsum_f, (double)lcnt_i);
       }
-      // per-thread partials (at most 16 terms) in FP32, workgroup reduction in FP64
-      const double lsum = (double)lsum_f, lcnt = (double)lcnt_i;
-      const double2 lsc = wg_sum2(lsum, lcnt, red);
-      const double ls = lsc.x, lc = lsc.y;
+      __syncthreads();
       if (tid == 0) {
+        double ls = 0.0, lc = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) {
+          ls += red[w];
+          lc += red[kThreads / 64 + w];
+        }
         const double L = lc > 0.0 ? ls / lc : 0.0;
         T.frame_loss[pos] = L;
         obj_sum += fw * L;
         w_sum += fw;
       }
-      __syncthreads();
-      // 4. forward FFT (x 1/N applied in phase 5)
-      fft2_lds<N, false, RADIX>(lds, twf, tid, live);
+      // 4. the rest of the forward transform
+      ip_stage<N, N, P0, true, false, 1, LD, false>(lds, twf, tid);
+      ip_stage<N, N, P1, false, true, LD, 1, false, true>(lds, twf, tid, live);
+      ip_stage<N, N, P0, true, false, LD, 1, false, true>(lds, twf, tid, live);
       if (fetch) rn[threadIdx.x] = next_w;
       // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den at the support pixels,
       //    in place: U of centred pixel (r, c) replaces the transform value at
@@ -1041,7 +1139,7 @@ __device__ __forceinline__ void rows_out(const float2* lds, float2* __restrict__
 
 template <int CH, int N, int LD, class MEM, class RowOf>
 __device__ __forceinline__ void rows_in(float2* lds, const float2* __restrict__ g, int nk, int tid, RowOf row_of) {
-  constexpr int PER = CH * N / kThreads;
+  constexpr int PER = (CH * N + kThreads - 1) / kThreads;
   float2 v[PER];
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
@@ -1076,7 +1174,7 @@ __device__ __forceinline__ void copy_in_live(float2* lds, const float2* __restri
 }
 
 // Group barrier of a cooperative tile (members > 1). Every access of data
-// shared between the members (the field, chunk edges, partial sums, the
+// shared between the members (the field, partial sums, the
 // spectrum and momentum arrays of the tile) is device-coherent: relaxed
 // agent-scope atomics or buffer accesses with the sc1 policy bit, which the
 // hardware serves from the device-coherent level. Each wave first waits for
@@ -1136,12 +1234,13 @@ __device__ __forceinline__ bool group_barrier(const TileDesc& T, uint64_t& passe
 //   C  the rows of A, in centred order: forward row transforms, fftshift by
 //      index, U = conj(P) diff / N / den in LDS, then the adjoint insert of
 //      the destination rows this chunk completes (the row above the chunk is
-//      the previous chunk's last row of U)
+//      the previous chunk's last row of U: kept from that chunk when one
+//      workgroup runs them all, transformed once more by a group member)
 //
 // One workgroup runs the chunks in turn (MEM = PlainMem), or a cooperative
 // group of N/CH workgroups takes one chunk each (MEM = CoherentMem): group
-// barriers separate A, B and C, the insert from the publication of the
-// chunk edges, and one LED from the next; per-member objective sums are
+// barriers separate A, B and C, and one LED from the next (three per LED,
+// one more after a momentum event); per-member objective sums are
 // combined by member 0 in member order. The LED loop, objective, momentum
 // and arithmetic are those of sweep_tile, and the spectra do not depend on
 // the group size.
@@ -1165,9 +1264,12 @@ __device__ __forceinline__ void big_lines(float2* lds, const float2* tw, int tid
   }
 }
 
+// float2 elements of a large tile's chunk region: CH + 1 rows of N (+1 pad)
+// in pass C of a group member, or N rows of CH (+1 pad) in pass B.
 template <int N>
 __host__ __device__ constexpr int big_field_elems() {
-  return BigPlan<N>::CH * (N + 1) > N * (BigPlan<N>::CH + 1) ? BigPlan<N>::CH * (N + 1) : N * (BigPlan<N>::CH + 1);
+  return (BigPlan<N>::CH + 1) * (N + 1) > N * (BigPlan<N>::CH + 1) ? (BigPlan<N>::CH + 1) * (N + 1)
+                                                                     : N * (BigPlan<N>::CH + 1);
 }
 
 // One chunk region for every large tile size (a single constant keeps the
@@ -1372,37 +1474,37 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       // C. chunk q: centred rows R0 .. R0 + nk - 1 (natural rows (R0 + j) ^ HALF),
       //    forward row transforms, U, then the insert of destination rows
       //    R0 .. R0 + nk - 1 (the last chunk with rows also takes the row below
-      //    its last, for a row fraction); the row above comes from the
-      //    previous chunk (zero above the first)
+      //    its last, for a row fraction). The insert also reads the row of U
+      //    above the chunk (zero above the first chunk): a group member
+      //    transforms that row (the previous chunk's last) as well, as LDS row
+      //    0, so that no row of U passes between members; one workgroup keeps
+      //    it from the previous chunk in `prev`.
       const double scale = fw * T.object_step;
+      constexpr int CROWS = COOP ? CH + 1 : CH;  // rows in LDS
+      constexpr int CLINES = COOP ? 2 * CH : CH;  // power-of-two line count of the live-row stages
       for (int q = first; q < NCH; q += step) {
         tid = opaque_tid();
         const int nk = chunk_rows(q), R0 = rlo + q * crows;
-        rows_in<CH, N, LDR, MEM>(lds, fld, nk, opaque_tid(), [&](int j) { return (R0 + j) ^ HALF; });
+        const int ex = (COOP && q > 0 && nk > 0) ? 1 : 0;  // the row above, transformed here
+        const int RB = R0 - ex, nb = nk + ex;              // centred rows RB .. RB + nb - 1 in LDS rows 0 .. nb - 1
+        rows_in<CROWS, N, LDR, MEM>(lds, fld, nb, opaque_tid(), [&](int j) { return (RB + j) ^ HALF; });
         __syncthreads();
-        big_lines<N, CH, LDR, 1, false, true>(lds, twf, tid, live_rows(nk));
-        for (int i = tid; i < nk * N; i += kThreads) {
+        big_lines<N, CLINES, LDR, 1, false, true>(lds, twf, tid, live_rows(nb));
+        for (int i = tid; i < nb * N; i += kThreads) {
           const int lr = i / N, kc = i % N;
-          const bool in = in_extent(ext[R0 + lr], kc ^ HALF);  // U is zero outside the support
-          const float2 P = bld_f2(rpup, in ? (uint32_t)((R0 + lr) * N + (kc ^ HALF)) * 8u : kSkip);
+          const bool in = in_extent(ext[RB + lr], kc ^ HALF);  // U is zero outside the support
+          const float2 P = bld_f2(rpup, in ? (uint32_t)((RB + lr) * N + (kc ^ HALF)) * 8u : kSkip);
           const float kk = inv_n / fmaf(den.b, fmaf(P.x, P.x, P.y * P.y), den.a);
           const float2 u = fmulc(P, lds[lr * LDR + kc]);
           lds[lr * LDR + kc] = in ? f2(u.x * kk, u.y * kk) : f2(0.0f, 0.0f);
         }
         __syncthreads();
         const int a0 = R0;
-        if (COOP) {
-          // publish this chunk's last row of U for the next chunk's insert
-          if (q + 1 < NCH)
-            for (int i = opaque_tid(); i < N; i += kThreads)
-              MEM::st(T.edge + q * N + i, nk > 0 ? lds[(nk - 1) * LDR + i] : f2(0.0f, 0.0f));
-          if (!group_barrier(T, passed, bflag)) return;
-        }
         // destination rows of this chunk
         if (sp.zero) {
           for (int i = opaque_tid(); i < nk * N; i += kThreads) {
             const int lr = i / N, c = i % N;
-            const float2 u = lds[lr * LDR + (c ^ HALF)];
+            const float2 u = lds[(lr + ex) * LDR + (c ^ HALF)];
             if (u.x != 0.0f || u.y != 0.0f) {
               double2* d = spec + (sp.sr + a0 + lr) * T.rw + sp.sc + c;
               const double2 o = MEM::ld(d);
@@ -1412,7 +1514,6 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
         } else {
           // the chunk's destination rows a0 .. a1 - 1 of the tile's list
           const int a1 = a0 + nk + ((q == clast && sp.ar.upper != sp.ar.lower) ? 1 : 0);
-          const float2* above = COOP ? T.edge + (q - 1) * N : prev;
           const uint32_t* rows = T.dest + dest_row_starts_offset(N);
           const uint32_t d0 = nk > 0 ? __builtin_amdgcn_readfirstlane(gptr(rows)[a0]) : 0u;
           const uint32_t d1 = nk > 0 ? __builtin_amdgcn_readfirstlane(gptr(rows)[a1]) : 0u;
@@ -1422,9 +1523,9 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
           insert_list<N, MEM>(spec, T.rh * T.rw, sp, T.rw, scale, itid, T.dest + 2 + d0, d1 - d0, dfirst,
                               [&](int sr_, int sc_) {
                                 if (sr_ >= a0 + nk) return f2(0.0f, 0.0f);  // below the support rows
-                                if (sr_ >= a0) return lds[(sr_ - a0) * LDR + (sc_ ^ HALF)];
+                                if (sr_ >= RB) return lds[(sr_ - RB) * LDR + (sc_ ^ HALF)];
                                 if (q == 0) return f2(0.0f, 0.0f);  // above the support rows
-                                return COOP ? MEM::ld(above + (sc_ ^ HALF)) : above[sc_ ^ HALF];
+                                return prev[sc_ ^ HALF];            // one workgroup: the previous chunk's row
                               });
         }
         __syncthreads();
@@ -1487,10 +1588,10 @@ __global__ void __launch_bounds__(kThreads)
         sweep_tile<64, 8>(T, lds, twf, twi, red, ext, rec);
       break;
     case 128:
-      if (T.plan == 16)
-        sweep_tile<128, 16>(T, lds, twf, twi, red, ext, rec);
-      else
+      if (T.plan == 8)
         sweep_tile<128, 8>(T, lds, twf, twi, red, ext, rec);
+      else
+        sweep_tile<128, 16>(T, lds, twf, twi, red, ext, rec);
       break;
     case 256:
       if (T.members > 1)
@@ -1574,7 +1675,7 @@ uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
       out[0] = 8, out[1] = 4, out[2] = 16;
       return 3;
     case 128:
-      out[0] = 8, out[1] = 16;
+      out[0] = 16, out[1] = 8;
       return 2;
     case 256:
       out[0] = 16;
@@ -1587,9 +1688,9 @@ uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]) {
   }
 }
 
-size_t group_state_bytes(uint32_t n) {
-  // [0, 24) counters, [64, 64 + 16 kMaxGroupMembers) partial sums, then group_members(n) edge rows
-  return 64 + 16 * (size_t)kMaxGroupMembers + (size_t)group_members(n) * n * sizeof(float2);
+size_t group_state_bytes() {
+  // [0, 24) counters, [64, 64 + 16 kMaxGroupMembers) partial sums
+  return 64 + 16 * (size_t)kMaxGroupMembers;
 }
 
 // The sweep kernel for tiles up to nmax.

@@ -1,5 +1,5 @@
 // radix.h — complex64 helpers and the in-register radix-2^k DFT of the
-// fused sweep's Stockham stages (also used by the DFT trial program).
+// fused sweep's transform stages (also used by the DFT trial program).
 #pragma once
 
 #include <hip/hip_runtime.h>
