@@ -126,19 +126,64 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
       room -= extra;
     }
   }
-  std::vector<TileDesc> out;
-  out.reserve(sorted.size() + big.size() * (kMaxGroupMembers - 1));
+  // Grid positions. Workgroups are dispatched to the XCDs round-robin
+  // (workgroup i to XCD i mod xccs), so the members of a cooperative group
+  // take positions xccs apart, all on one XCD, on the XCD with the most free
+  // positions; single-workgroup tiles fill the other positions in order.
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < sorted.size(); ++i) total += grouped[i] ? group_members(sorted[i].n) : 1;
+  const uint32_t xccs = std::max<uint32_t>(device_xccs(), 1);
+  std::vector<std::vector<uint32_t>> free_at(xccs);  // free positions per XCD, ascending
+  for (uint32_t p = total; p-- > 0;) free_at[p % xccs].push_back(p);  // back() = lowest
+  std::vector<TileDesc> out(total);
+  std::vector<char> used(total, 0);
+  std::vector<uint32_t> singles;
   for (uint32_t i = 0; i < sorted.size(); ++i) {
     TileDesc t = sorted[i];
     if (radix && t.n <= 256) t.plan = radix[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3];
     const uint32_t g = grouped[i] ? group_members(t.n) : 1;
+    if (g == 1) {
+      sorted[i] = t;
+      singles.push_back(i);
+      continue;
+    }
+    uint32_t x = 0;
+    for (uint32_t k = 1; k < xccs; ++k)
+      if (free_at[k].size() > free_at[x].size()) x = k;
     for (uint32_t m = 0; m < g; ++m) {
-      out.push_back(t);
-      out.back().member = m;
-      out.back().members = g;
+      uint32_t p;
+      if (!free_at[x].empty()) {
+        p = free_at[x].back();
+        free_at[x].pop_back();
+      } else {  // no XCD has room for the whole group: any free position
+        uint32_t y = 0;
+        for (uint32_t k = 1; k < xccs; ++k)
+          if (free_at[k].size() > free_at[y].size()) y = k;
+        p = free_at[y].back();
+        free_at[y].pop_back();
+      }
+      used[p] = 1;
+      out[p] = t;
+      out[p].member = m;
+      out[p].members = g;
     }
   }
+  uint32_t p = 0;
+  for (uint32_t i : singles) {
+    while (used[p]) ++p;
+    used[p] = 1;
+    out[p] = sorted[i];
+    out[p].member = 0;
+    out[p].members = 1;
+  }
   return out;
+}
+
+uint32_t device_xccs() {
+  int dev = 0, n = 1;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess) n = 1;
+  return (uint32_t)std::max(n, 1);
 }
 
 void make_sweep_twiddles(DeviceBuffer& out, hipStream_t stream) {

@@ -72,11 +72,15 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
 // order. With `members` > 1, tiles above kFusedLdsMaxN become cooperative
 // groups of group_members(n) descriptors as far as the whole grid stays
 // within `grid_limit` resident workgroups, tiles with more LEDs first; the
-// others run on one workgroup. radix[4]: first-stage radix per tile size
+// others run on one workgroup; a group's members are placed on one XCD
+// (positions device_xccs() apart). radix[4]: first-stage radix per tile size
 // 32, 64, 128, 256 (0: default; null: keep the descriptors' plans; 512^2
 // tiles have one plan).
 std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,
                                   bool largest_first, uint32_t grid_limit);
+// XCDs (accelerator complex dies) of the current device; workgroups are
+// dispatched to them round-robin.
+uint32_t device_xccs();
 // First-stage radix choices of the fused sweep for an n x n tile; the first
 // entry is the default. Returns the number of choices (0: size unsupported).
 uint32_t sweep_radix_choices(uint32_t n, uint32_t out[3]);
