@@ -998,16 +998,25 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       // 4. the rest of the forward transform
       ip_stage<N, N, P0, true, false, 1, LD, false>(lds, twf, tid);
       ip_stage<N, N, P1, false, true, LD, 1, false, true>(lds, twf, tid, live);
+      // the pupil at this thread's first support strip and the first insert
+      // destinations, loaded during the last stage (the update and the
+      // insert then start without a memory round trip)
+      constexpr int S = (int)kGatherStrip;
+      float2 Pfirst[S];
+      {
+        const uint32_t mask = my_strip >> 20, r0 = my_strip & 0x3FFu, c = (my_strip >> 10) & 0x3FFu;
+#pragma unroll
+        for (int j = 0; j < S; ++j) Pfirst[j] = bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+      }
+      uint32_t dfirst[kInsertGroup];
+      insert_list_first(T.dest + 2, ndest, tid, dfirst);
       ip_stage<N, N, P0, true, false, LD, 1, false, true>(lds, twf, tid, live);
       if (fetch) rn[threadIdx.x] = next_w;
       // 5. U = conj(P) fftshift(FFT(psi' - psi))/N / den at the support pixels,
       //    in place: U of centred pixel (r, c) replaces the transform value at
       //    its natural slot ((r + N/2) mod N, (c + N/2) mod N). Outside the
       //    support the field keeps values that the insert does not read.
-      uint32_t dfirst[kInsertGroup];
-      insert_list_first(T.dest + 2, ndest, tid, dfirst);
       {
-        constexpr int S = (int)kGatherStrip;
         const Rsrc rl = make_rsrc(slist, nstrip * 4u);
         const float pk = uniform((float)(fw * T.pupil_step) * pupil_k);
         double pmax = 0.0;
@@ -1018,7 +1027,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
           const int cn = (c + HALF) % N;
           float2 P[S];
 #pragma unroll
-          for (int j = 0; j < S; ++j) P[j] = bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+          for (int j = 0; j < S; ++j)
+            P[j] = i == (uint32_t)tid ? Pfirst[j] : bld_f2(rpup, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
           if constexpr (EPRY) {
             // EPRY pupil step at the strip's support pixels (common.rs:271-300):
             // g = conj(patch) diff / (max |patch|^2 + eps), P += w beta g, with
