@@ -1655,7 +1655,7 @@ __global__ void __launch_bounds__(kThreads)
       sweep_tile<64, 8, true>(T, lds, twf, twi, red, ext, rec);
       break;
     case 128:
-      sweep_tile<128, 8, true>(T, lds, twf, twi, red, ext, rec);
+      sweep_tile<128, 16, true>(T, lds, twf, twi, red, ext, rec);
       break;
     default:
       break;
