@@ -1033,9 +1033,17 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
             // EPRY pupil step at the strip's support pixels (common.rs:271-300):
             // g = conj(patch) diff / (max |patch|^2 + eps), P += w beta g, with
             // the old P in U; the support projection leaves the rest at zero
+            // the strip's patch values and FP64 master pupil values, all loads
+            // issued before the first store
             float2 pa[S];
+            double2 m64[S];
 #pragma unroll
-            for (int j = 0; j < S; ++j) pa[j] = bld_f2(rpatch, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip);
+            for (int j = 0; j < S; ++j) {
+              const bool in = mask >> j & 1u;
+              const uint32_t pix = (uint32_t)((r0 + j) * N + c);
+              pa[j] = bld_f2(rpatch, in ? pix * 8u : kSkip);
+              m64[j] = bld_d2(rp64, in ? pix * 16u : kSkip);
+            }
 #pragma unroll
             for (int j = 0; j < S; ++j) {
               if (mask >> j & 1u) {
@@ -1047,8 +1055,8 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
                 const float2 g = fmulc(pa[j], d);
                 // the step accumulates into the FP64 master; P is its complex64 copy
                 const uint32_t pix = (uint32_t)((r0 + j) * N + c);
-                const double2 m64 = bld_d2(rp64, pix * 16u);
-                const double2 n64 = make_double2(fma((double)pk, (double)g.x, m64.x), fma((double)pk, (double)g.y, m64.y));
+                const double2 n64 =
+                    make_double2(fma((double)pk, (double)g.x, m64[j].x), fma((double)pk, (double)g.y, m64[j].y));
                 bst_d2(rp64, pix * 16u, n64);
                 const float2 np = f2((float)n64.x, (float)n64.y);
                 bst_f2(rpup, pix * 8u, np);
