@@ -535,7 +535,7 @@ __device__ __forceinline__ bool in_extent(uint32_t e, int c) {
 // tap rows shared between consecutive pixels, in groups of G rows.
 template <int N>
 __device__ __forceinline__ double patch_max(Rsrc rspec, const SrcParam& sp, uint32_t rw, int tid) {
-  constexpr int PP = N * N / kThreads, G = PP < 4 ? PP : 4;
+  constexpr int PP = N * N / kThreads, G = PP < 8 ? PP : 8;
   static_assert(PP >= 1 && PP % G == 0, "patch rows per thread");
   const int c = tid % N, r0 = (tid / N) * PP;
   const bool fc = sp.ac.upper != sp.ac.lower, fr = sp.ar.upper != sp.ar.lower;
