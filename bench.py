@@ -554,6 +554,13 @@ def main():
                                        "unit": "GB/s", "frac": b128 / psecs / 1e9 / HBM_PEAK_GBS,
                                        "note": "per-stage kernels, one tile at a time, each sweep replayed "
                                                "from a captured graph"}
+        # the fused FP32 kernel on a batch of one tile: the per-tile part of the
+        # batched / plain ratio (the rest is 256 tiles running side by side)
+        log("one-tile fused leg")
+        fsecs, fsweep_ms, _, _ = run_batched(tiles[:1], args.iterations, 1, 1)
+        line["fused_one_tile"] = {"tiles_per_s": 1.0 / fsecs, "sweep_ms": fsweep_ms,
+                                  "speedup_vs_plain_c128": psecs / fsecs,
+                                  "batching_factor": tiles_per_s * fsecs}
         log("uneven batch leg")
         line["uneven_batch"] = run_uneven(40, 1, 1)
         log("EPRY batch leg")

@@ -114,16 +114,27 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
   std::vector<uint32_t> big;
   for (uint32_t i = 0; i < sorted.size(); ++i)
     if (sorted[i].n > kFusedLdsMaxN && sorted[i].group) big.push_back(i);
-  std::vector<char> grouped(sorted.size(), 0);
-  if (members > 1 && !big.empty()) {
-    // each grouped tile adds group_members(n) - 1 workgroups to the grid
+  // workgroups of each tile: 1, or its cooperative group's size
+  std::vector<uint32_t> gsize(sorted.size(), 1);
+  if (members != 1 && !big.empty()) {
+    // each grouped tile adds its group size - 1 workgroups to the grid
     uint64_t room = grid_limit > sorted.size() ? grid_limit - sorted.size() : 0;
     std::stable_sort(big.begin(), big.end(), [&](uint32_t x, uint32_t y) { return sorted[x].F > sorted[y].F; });
+    auto wanted = [&](uint32_t n) { return n == 256 && members > 1 ? members : group_members(n); };
+    uint32_t n256 = 0;
     for (uint32_t k : big) {
-      const uint64_t extra = group_members(sorted[k].n) - 1;
+      const uint64_t extra = wanted(sorted[k].n) - 1;
       if (extra > room) continue;
-      grouped[k] = 1;
+      gsize[k] = wanted(sorted[k].n);
       room -= extra;
+      n256 += sorted[k].n == 256;
+    }
+    if (members == 0 && n256 > 0) {
+      // automatic: the co-resident room left over goes to the 256^2 groups,
+      // the same number of members each, up to kMaxGroupMembers
+      const uint32_t add = (uint32_t)std::min<uint64_t>(room / n256, kMaxGroupMembers - group_members(256));
+      for (uint32_t k : big)
+        if (sorted[k].n == 256 && gsize[k] > 1) gsize[k] += add;
     }
   }
   // Grid positions. Workgroups are dispatched to the XCDs round-robin
@@ -131,7 +142,7 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
   // take positions xccs apart, all on one XCD, on the XCD with the most free
   // positions; single-workgroup tiles fill the other positions in order.
   uint32_t total = 0;
-  for (uint32_t i = 0; i < sorted.size(); ++i) total += grouped[i] ? group_members(sorted[i].n) : 1;
+  for (uint32_t i = 0; i < sorted.size(); ++i) total += gsize[i];
   const uint32_t xccs = std::max<uint32_t>(device_xccs(), 1);
   std::vector<std::vector<uint32_t>> free_at(xccs);  // free positions per XCD, ascending
   for (uint32_t p = total; p-- > 0;) free_at[p % xccs].push_back(p);  // back() = lowest
@@ -141,7 +152,7 @@ std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t m
   for (uint32_t i = 0; i < sorted.size(); ++i) {
     TileDesc t = sorted[i];
     if (radix && t.n <= 256) t.plan = radix[t.n == 32 ? 0 : t.n == 64 ? 1 : t.n == 128 ? 2 : 3];
-    const uint32_t g = grouped[i] ? group_members(t.n) : 1;
+    const uint32_t g = gsize[i];
     if (g == 1) {
       sorted[i] = t;
       singles.push_back(i);
@@ -258,7 +269,7 @@ struct fpmhip_batch {
   std::vector<fpmhip_session*> sessions;
   std::vector<TileDesc> host_tiles;    // in session order
   std::vector<TileDesc> device_order;  // launch order (largest tiles first, groups expanded)
-  fpmhip_batch_schedule schedule{kGroupMembers, 1, {0, 0, 0, 0}, 0};
+  fpmhip_batch_schedule schedule{0, 1, {0, 0, 0, 0}, 0};
   uint32_t launched = 0;               // workgroups per sweep launch
   bool cooperative = false;
   DeviceBuffer tiles;      // TileDesc[launched]
@@ -540,8 +551,8 @@ void check_tile_status(int32_t status) {
 }
 
 void validate_schedule(const fpmhip_batch_schedule& s) {
-  if (s.group_members != 1 && s.group_members != kGroupMembers)
-    fail(FPMHIP_INVALID_PARAMETER, "group_members must be 1 or 8", "group_members");
+  if (s.group_members > kMaxGroupMembers || (s.group_members > 1 && s.group_members < kGroupMembers))
+    fail(FPMHIP_INVALID_PARAMETER, "group_members must be 0, 1 or 8..32", "group_members");
   if (s.largest_first > 1) fail(FPMHIP_INVALID_PARAMETER, "largest_first must be 0 or 1", "largest_first");
   if (s.sweeps_per_launch > 1000)
     fail(FPMHIP_INVALID_PARAMETER, "sweeps_per_launch must be in 0..1000", "sweeps_per_launch");
@@ -679,7 +690,7 @@ int fpmhip_batch_tune(fpmhip_batch* b, uint32_t trial_sweeps, fpmhip_batch_tunin
     }
     // coordinate descent: group size, radix per tile size, launch order
     if (big)
-      for (uint32_t g : {1u, kGroupMembers})
+      for (uint32_t g : {0u, 1u, kGroupMembers})
         if (g != best.group_members) {
           fpmhip_batch_schedule c = best;
           c.group_members = g;

@@ -69,11 +69,14 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
                     uint32_t trace_len);
 // Launch order of a sweep's tiles: largest first (stable), so the longest
 // workgroups are dispatched first and spread over the XCDs, or session
-// order. With `members` > 1, tiles above kFusedLdsMaxN become cooperative
-// groups of group_members(n) descriptors as far as the whole grid stays
-// within `grid_limit` resident workgroups, tiles with more LEDs first; the
-// others run on one workgroup; a group's members are placed on one XCD
-// (positions device_xccs() apart). radix[4]: first-stage radix per tile size
+// order. Unless `members` is 1, tiles above kFusedLdsMaxN become cooperative
+// groups as far as the whole grid stays within `grid_limit` resident
+// workgroups, tiles with more LEDs first; the others run on one workgroup.
+// Group sizes: group_members(n) (512^2: 32); at 256^2 `members` (8..32), or
+// with `members` 0 group_members(256) plus an equal share of the resident
+// room left over, up to kMaxGroupMembers. A group's members are placed on
+// one XCD's dispatch slots (positions device_xccs() apart). radix[4]:
+// first-stage radix per tile size
 // 32, 64, 128, 256 (0: default; null: keep the descriptors' plans; 512^2
 // tiles have one plan).
 std::vector<TileDesc> order_tiles(const std::vector<TileDesc>& tiles, uint32_t members, const uint32_t* radix,

@@ -1099,43 +1099,6 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
   }
 }
 
-// Copies between a block of a row-major global array (rows of W elements at
-// a stride of `gstride`) and LDS (rows at a pitch of LD): every load of the
-// thread is issued before the first store.
-template <int TOTAL, int W, int LD, class MEM = PlainMem>
-__device__ __forceinline__ void copy_in(float2* lds, const float2* __restrict__ g, int gstride, int tid) {
-  constexpr int PER = TOTAL / kThreads;
-  static_assert(TOTAL % kThreads == 0, "copy shape");
-  float2 v[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    v[j] = MEM::ld(g + (i / W) * gstride + i % W);
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    lds[(i / W) * LD + i % W] = v[j];
-  }
-}
-
-template <int TOTAL, int W, int LD, class MEM = PlainMem>
-__device__ __forceinline__ void copy_out(const float2* lds, float2* __restrict__ g, int gstride, int tid) {
-  constexpr int PER = TOTAL / kThreads;
-  static_assert(TOTAL % kThreads == 0, "copy shape");
-  float2 v[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    v[j] = lds[(i / W) * LD + i % W];
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int i = tid + j * kThreads;
-    MEM::st(g + (i / W) * gstride + i % W, v[j]);
-  }
-}
-
 // Rows 0..nk-1 of an LDS row chunk (pitch LD, rows of N) to / from rows
 // row_of(j) of a row-major global array with N columns: every load of the
 // thread is issued before its first store.
@@ -1171,10 +1134,11 @@ __device__ __forceinline__ void rows_in(float2* lds, const float2* __restrict__ 
   }
 }
 
-// copy_in of a column block whose rows with live(row) false read as zero
-// (without loading them).
+// copy_in of a column block of width w <= W whose rows with live(row)
+// false read as zero (without loading them); LDS columns w..W-1 are not
+// written.
 template <int TOTAL, int W, int LD, class MEM, class Live>
-__device__ __forceinline__ void copy_in_live(float2* lds, const float2* __restrict__ g, int gstride, int tid,
+__device__ __forceinline__ void copy_in_live(float2* lds, const float2* __restrict__ g, int gstride, int w, int tid,
                                              Live live) {
   constexpr int PER = TOTAL / kThreads;
   static_assert(TOTAL % kThreads == 0, "copy shape");
@@ -1182,12 +1146,31 @@ __device__ __forceinline__ void copy_in_live(float2* lds, const float2* __restri
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     const int i = tid + j * kThreads;
-    v[j] = live(i / W) ? MEM::ld(g + (i / W) * gstride + i % W) : f2(0.0f, 0.0f);
+    v[j] = (i % W < w && live(i / W)) ? MEM::ld(g + (i / W) * gstride + i % W) : f2(0.0f, 0.0f);
   }
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     const int i = tid + j * kThreads;
-    lds[(i / W) * LD + i % W] = v[j];
+    if (i % W < w) lds[(i / W) * LD + i % W] = v[j];
+  }
+}
+
+// copy_out of the first w <= W columns of a column block.
+template <int TOTAL, int W, int LD, class MEM>
+__device__ __forceinline__ void copy_out_cols(const float2* lds, float2* __restrict__ g, int gstride, int w,
+                                              int tid) {
+  constexpr int PER = TOTAL / kThreads;
+  static_assert(TOTAL % kThreads == 0, "copy shape");
+  float2 v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    v[j] = lds[(i / W) * LD + i % W];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = tid + j * kThreads;
+    if (i % W < w) MEM::st(g + (i / W) * gstride + i % W, v[j]);
   }
 }
 
@@ -1313,7 +1296,10 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   constexpr bool COOP = MEM::kCoherent;
   static_assert((N & (N - 1)) == 0 && HALF % CH == 0, "big tile shape");
   // a cooperative group has one member per chunk; a lone workgroup runs them all
-  const int first = COOP ? (int)T.member : 0, step = COOP ? NCH : 1;
+  // a cooperative group has one member per chunk (8..32 members at 256^2,
+  // 32 at 512^2); a lone workgroup runs the N / CH chunks in turn
+  const int nch = COOP ? (int)__builtin_amdgcn_readfirstlane(T.members) : NCH;
+  const int first = COOP ? (int)T.member : 0, step = COOP ? nch : 1;
   const bool lead = !COOP || T.member == 0;
   uint64_t passed = COOP ? T.group[1] : 0;
   const float inv_n = 1.0f / (float)NN;
@@ -1342,7 +1328,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
   rlo = __builtin_amdgcn_readfirstlane(rlo);
   rhi = __builtin_amdgcn_readfirstlane(rhi);
   const int nl = rhi >= rlo ? rhi - rlo + 1 : 0;
-  const int crows = (nl + NCH - 1) / NCH;  // <= CH
+  const int crows = (nl + nch - 1) / nch;  // <= CH
   const int clast = nl ? (nl - 1) / crows : -1;  // last chunk with rows
   auto chunk_rows = [&](int k) { return k <= clast ? min(crows, nl - k * crows) : 0; };
   auto live_rows = [&](int nk) { return LiveRows{nk, 0, nk ? 1.0f / (float)nk : 0.0f}; };
@@ -1378,7 +1364,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       // A. chunk k: centred rows R0 .. R0 + nk - 1 in LDS rows 0 .. nk - 1 (with
       //    the ifftshifted columns), inverse row transforms, out to the natural
       //    rows (R0 + j) ^ HALF of T.field (ifftshift for even N)
-      for (int k = first; k < NCH; k += step) {
+      for (int k = first; k < nch; k += step) {
         tid = opaque_tid();
         const int nk = chunk_rows(k), R0 = rlo + k * crows;
         {
@@ -1401,40 +1387,45 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       double lsum = 0.0;
       int lcnt = 0;
       if (fetch) next_w = rec_word(T, next_f, threadIdx.x);
-      for (int jc = first; jc < NCH; jc += step) {
+      for (int jc = first; jc < nch; jc += step) {
         tid = opaque_tid();
-        const int c0 = jc * CH;
-        copy_in_live<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid(), [&](int nr) {
+        // the block's columns c0 .. c0 + w - 1 (w = CH unless the group has more
+        // members than N / CH chunks); only its w columns are transformed
+        const int c0 = (jc * N) / nch, w = ((jc + 1) * N) / nch - c0;
+        copy_in_live<N * CH, CH, LDC, MEM>(lds, fld + c0, N, w, opaque_tid(), [&](int nr) {
           const int cr = nr ^ HALF;  // centred row of natural row nr
           return cr >= rlo && cr <= rhi;
         });
         __syncthreads();
-        big_lines<N, CH, 1, LDC, true>(lds, twi, tid);
+        big_lines<N, CH, 1, LDC, true, true>(lds, twi, tid, live_rows(w));
         if (plain_frames) {
           // branch-free (as in sweep_tile): pixel (row0 + j RSTEP, c0 + col)
           constexpr int PP = N * CH / kThreads, RSTEP = kThreads / CH;
           const int tid = opaque_tid();
           const int row0 = tid / CH, col = tid % CH;
+          const bool in = col < w;
           float y[PP];
           const uint32_t p0 = (uint32_t)(row0 * N + c0 + col);
           if (T.is_u16) {
             const Rsrc rm = make_rsrc(static_cast<const uint16_t*>(T.frames) + fbase, NN * 2u);
 #pragma unroll
-            for (int j = 0; j < PP; ++j) y[j] = bld_u16f(rm, (p0 + j * RSTEP * N) * 2u);
+            for (int j = 0; j < PP; ++j) y[j] = bld_u16f(rm, in ? (p0 + j * RSTEP * N) * 2u : kSkip);
           } else {
             const Rsrc rm = make_rsrc(static_cast<const double*>(T.frames) + fbase, NN * 8u);
 #pragma unroll
-            for (int j = 0; j < PP; ++j) y[j] = bld_f64f(rm, (p0 + j * RSTEP * N) * 8u);
+            for (int j = 0; j < PP; ++j) y[j] = bld_f64f(rm, in ? (p0 + j * RSTEP * N) * 8u : kSkip);
           }
           float lsum_f = 0.0f;
+          if (in) {
 #pragma unroll
-          for (int j = 0; j < PP; ++j) {
-            const int lp = (row0 + j * RSTEP) * LDC + col;
-            const float2 fv = lds[lp];
-            lds[lp] = replace_plain(fv, y[j], sg, sig, eps_f, lsum_f);
+            for (int j = 0; j < PP; ++j) {
+              const int lp = (row0 + j * RSTEP) * LDC + col;
+              const float2 fv = lds[lp];
+              lds[lp] = replace_plain(fv, y[j], sg, sig, eps_f, lsum_f);
+            }
+            lcnt += PP;
           }
           lsum += (double)lsum_f;
-          lcnt += PP;
         } else {
           // measurement loads in groups of G issued before their use
           constexpr int PP = N * CH / kThreads;
@@ -1447,21 +1438,22 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
 #pragma unroll
             for (int j = 0; j < G; ++j) {
               const int i = tid + (g + j) * kThreads;
-              y[j] = measured(T, fbase + (uint64_t)(i / CH) * N + c0 + i % CH);
+              y[j] = i % CH < w ? measured(T, fbase + (uint64_t)(i / CH) * N + c0 + i % CH) : 0.0f;
             }
 #pragma unroll
             for (int j = 0; j < G; ++j) {
               const int i = tid + (g + j) * kThreads;
               const int lp = (i / CH) * LDC + i % CH;
-              lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, (i / CH) * N + c0 + i % CH, lds[lp], y[j], gain_f,
-                                          inv_gain, eps_f, lsum_f, lcnt);
+              if (i % CH < w)
+                lds[lp] = replace_amplitude(T, fbase, (uint64_t)NN, (i / CH) * N + c0 + i % CH, lds[lp], y[j],
+                                            gain_f, inv_gain, eps_f, lsum_f, lcnt);
             }
           }
           lsum += (double)lsum_f;
         }
         __syncthreads();
-        big_lines<N, CH, 1, LDC, false>(lds, twf, tid);
-        copy_out<N * CH, CH, LDC, MEM>(lds, fld + c0, N, opaque_tid());
+        big_lines<N, CH, 1, LDC, false, true>(lds, twf, tid, live_rows(w));
+        copy_out_cols<N * CH, CH, LDC, MEM>(lds, fld + c0, N, w, opaque_tid());
         __syncthreads();
       }
       if (fetch) rn[threadIdx.x] = next_w;
@@ -1500,7 +1492,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
       const double scale = fw * T.object_step;
       constexpr int CROWS = COOP ? CH + 1 : CH;  // rows in LDS
       constexpr int CLINES = COOP ? 2 * CH : CH;  // power-of-two line count of the live-row stages
-      for (int q = first; q < NCH; q += step) {
+      for (int q = first; q < nch; q += step) {
         tid = opaque_tid();
         const int nk = chunk_rows(q), R0 = rlo + q * crows;
         const int ex = (COOP && q > 0 && nk > 0) ? 1 : 0;  // the row above, transformed here
@@ -1547,7 +1539,7 @@ __device__ __forceinline__ void sweep_tile_big(const TileDesc& T, float2* lds, f
                               });
         }
         __syncthreads();
-        if (!COOP && q + 1 < NCH) {
+        if (!COOP && q + 1 < nch) {
           for (int i = tid; i < N; i += kThreads) prev[i] = nk > 0 ? lds[(nk - 1) * LDR + i] : f2(0.0f, 0.0f);
           __syncthreads();
         }

@@ -712,7 +712,8 @@ class Batch:
 
     def schedule(self) -> dict:
         """The sweep launch's schedule: group_members (workgroups per 256^2
-        tile), largest_first, radix (first-stage FFT radix for 32, 64, 128,
+        tile: 0 automatic, 1 none, 8..32 that many), largest_first, radix
+        (first-stage FFT radix for 32, 64, 128,
         256 pixel tiles; 0 = default), sweeps_per_launch (0 = default)."""
         sch = BatchSchedule()
         check(lib().fpmhip_batch_get_schedule(self.handle, ct.byref(sch)))

@@ -499,8 +499,10 @@ int fpmhip_batch_objects_pinned(fpmhip_batch* batch, const double** out);
 /* Schedule of the batch's sweep launch (no counterpart in the reference; the
  * reference runs one tile at a time on the host). */
 typedef struct fpmhip_batch_schedule {
-  uint32_t group_members; /* workgroups per 256^2 tile: 1, or 8 (a cooperative group of 32-line
-                             chunks; used only when the whole grid can be resident at once) */
+  uint32_t group_members; /* workgroups per 256^2 tile: 0 (default) cooperative groups of 8 plus
+                             an equal share of the co-resident room left over, up to 32; 8..32
+                             that many; 1 none. Groups (512^2 tiles: 32) are used only when the
+                             whole grid can be resident at once */
   uint32_t largest_first; /* 1: launch the largest tiles first; 0: session order */
   uint32_t radix[4];      /* first-stage FFT radix for 32, 64, 128, 256 pixel tiles (0: default;
                              32: 8|4, 64: 8|4|16, 128: 8|16, 256: 16) */

@@ -148,25 +148,29 @@ def test_batch_256_tile_identical_alone_and_in_mix(ctx):
         assert np.array_equal(mix[i].spectrum(), alone[0].spectrum())
 
 
-def test_cooperative_group_matches_single_workgroup(ctx):
-    """A 256^2 tile split over a cooperative group of 8 workgroups gives the
-    same spectrum, bit for bit, as one workgroup running all chunks (the
-    per-pixel arithmetic and insert order are the same); the objective sums
-    differ only by summation order."""
+@pytest.mark.parametrize("g", [8, 11, 0])
+def test_cooperative_group_matches_single_workgroup(ctx, g):
+    """A 256^2 tile split over a cooperative group (8 or 11 workgroups, or
+    the automatic size, which takes the co-resident room left over: 32 here)
+    gives the same spectrum, bit for bit, as one workgroup running all chunks
+    (the per-pixel arithmetic and insert order are the same whatever the row
+    chunks and column blocks); the objective sums differ only by summation
+    order."""
     alg = Mpie(iterations=3, momentum_interval=20, momentum_friction=0.5, momentum_feedback=0.5)
     tiles = [tile(256, 9, 91), tile(128, 7, 92), tile(256, 7, 93)]
     results = {}
-    for g in (8, 1):
+    for gm in (g, 1):
         ss = sessions_for(tiles, alg)
         b = runtime.Batch(ss)
-        b.set_schedule(group_members=g)
-        assert b.schedule()["group_members"] == g
+        b.set_schedule(group_members=gm)
+        assert b.schedule()["group_members"] == gm
         tr = b.run(3)
-        results[g] = ([s.spectrum() for s in ss], tr)
-        say("group", g, "sweep ms", b.timing()[0])
-    for a, c in zip(results[8][0], results[1][0]):
+        results[gm] = ([s.spectrum() for s in ss], tr)
+        say("group", gm, "sweep ms", b.timing()[0])
+        b.close()
+    for a, c in zip(results[g][0], results[1][0]):
         assert np.array_equal(a, c)
-    assert np.allclose(results[8][1], results[1][1], rtol=1e-12, atol=0)
+    assert np.allclose(results[g][1], results[1][1], rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize("radix", [[4, 4, 8, 0], [8, 16, 16, 0]])
@@ -303,11 +307,12 @@ def test_autotuner_keeps_a_valid_schedule(ctx):
     b = runtime.Batch(ss)
     rep = b.tune(1)
     say("tuning", rep)
-    # group size, radix of 32/64/128 and launch order: 1 + 1 + 1 + 2 + 1 + 1 candidates
-    assert rep["candidates"] == 7
+    # the default, then group size (none, 8), radix of 32/64/128 and launch
+    # order: 1 + 2 + 1 + 2 + 1 + 1 candidates
+    assert rep["candidates"] == 8
     assert rep["best_ms"] <= rep["first_ms"]
     sch = rep["schedule"]
-    assert sch["group_members"] in (1, 8) and sch["largest_first"] in (0, 1)
+    assert sch["group_members"] in (0, 1, 8) and sch["largest_first"] in (0, 1)
     b.run(2)
     for (model, meas), s in zip(tiles, ss):
         ref = oracle.run(model, meas, alg.to_c(), 2)
@@ -318,7 +323,8 @@ def test_autotuner_keeps_a_valid_schedule(ctx):
 def test_invalid_schedule_is_rejected(ctx):
     ss = sessions_for([tile(64, 5, 131)], Fpie(iterations=1))
     b = runtime.Batch(ss)
-    for kw in ({"group_members": 3}, {"radix": [0, 32, 0, 0]}, {"largest_first": 2}, {"sweeps_per_launch": 1001}):
+    for kw in ({"group_members": 3}, {"group_members": 33}, {"radix": [0, 32, 0, 0]}, {"largest_first": 2},
+               {"sweeps_per_launch": 1001}):
         with pytest.raises(runtime.FpmError) as e:
             b.set_schedule(**kw)
         assert e.value.kind == "InvalidParameter"
