@@ -570,6 +570,12 @@ def main():
         if args.sustained_rate > 0:
             log("serving leg at a sustainable rate")
             line["serving_sustained"] = serving(args.requests, args.sustained_rate, 30)
+        # 80 % of the capacity the saturated leg measured (its tiles/s over its
+        # tiles per request): a load the server keeps up with, near its limit
+        sat = line["serving"]
+        near = 0.8 * sat["tiles_per_s"] * sat["requests"] / sat["tiles"]
+        log(f"serving leg at 80 % of capacity ({near:.1f} req/s)")
+        line["serving_near_capacity"] = serving(args.requests, near, 30)
     if not args.no_cpu:
         threads = args.cpu_threads or host_threads()
         log(f"cpu baseline on {threads} threads")
