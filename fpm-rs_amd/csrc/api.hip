@@ -810,7 +810,11 @@ int fpmhip_session_create(const fpmhip_model* m, const fpmhip_measurements* me, 
       b.group.allocate(group_state_bytes());
       HIP_CHECK(hipMemsetAsync(b.group.ptr, 0, b.group.bytes, m->ctx->stream));
     }
-    if (precision == FPMHIP_MIXED_F32 && s->cfg.has_pupil_step) b.pupil64.allocate(N * sizeof(double2));
+    if (precision == FPMHIP_MIXED_F32 && s->cfg.has_pupil_step) {
+      b.pupil64.allocate(N * sizeof(double2));
+      b.bounds.allocate((size_t)((m->rh + kBoundBlock - 1) / kBoundBlock) * ((m->rw + kBoundBlock - 1) / kBoundBlock) *
+                        sizeof(uint64_t));
+    }
     set_session_pupil(*s, m->d_pupil64.as<double2>());
     if (precision == FPMHIP_MIXED_F32 && s->cfg.momentum && m->M() < (1ull << 25))
       b.mspan.allocate(momentum_span_capacity(*m) * sizeof(uint32_t));

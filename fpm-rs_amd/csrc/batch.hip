@@ -101,6 +101,8 @@ void fill_tile_desc(TileDesc& t, fpmhip_session& s, double* trace, double* frame
   t.recover_pupil = s.cfg.has_pupil_step ? 1 : 0;
   t.patch = t.recover_pupil ? s.b.patch.as<float2>() : nullptr;
   t.pupil64 = s.b.pupil64.as<double2>();
+  t.bounds = t.recover_pupil ? s.b.bounds.as<uint64_t>() : nullptr;
+  t.bcols = (m.rw + kBoundBlock - 1) / kBoundBlock;
   t.pupil_step = s.cfg.pupil_step;
 }
 
@@ -439,6 +441,9 @@ void build_tiles(fpmhip_batch& b, uint32_t trace_cap) {
   // capture the launch (the LDS attribute was set above, outside capture);
   // with a gauge, a launch runs one sweep and the gauge follows it
   HIP_CHECK(hipStreamBeginCapture(b.ctx->stream, hipStreamCaptureModeThreadLocal));
+  // EPRY tiles: exact block bounds of |O|^2 before the sweep (their inserts
+  // raise them during it)
+  if (b.host_tiles[0].recover_pupil) launch_patch_bounds(b.tiles.as<TileDesc>(), b.launched, b.ctx->stream);
   launch_sweep(b.tiles.as<TileDesc>(), b.launched, b.tw.as<float2>(), b.nmax, b.cooperative, b.ctx->stream,
                b.host_tiles[0].recover_pupil != 0);
   if (b.gauge_count) launch_gauge(b.gauges.as<GaugeTile>(), b.gauge_count, b.ctx->stream);

@@ -20,6 +20,8 @@ struct TileDesc {
   float2* field;          // [N] global field scratch (tiles above kFusedLdsMaxN)
   float2* patch;          // [N] the patch of the support pixels (EPRY pupil recovery)
   double2* pupil64;       // [N] FP64 master of the recovered pupil (EPRY pupil recovery)
+  uint64_t* bounds;       // [ceil(rh/8)][bcols] |O|^2 upper bound per 8 x 8 spectrum block, or null
+                          // (EPRY pupil recovery: exact before each sweep, raised by the inserts)
   const void* frames;     // [F*N] uint16 counts or f64 frames
   const uint8_t* masks;   // [F*N] or null
   const double* weights;  // [F]
@@ -46,7 +48,7 @@ struct TileDesc {
   uint32_t interval;
   int32_t is_u16, denom, momentum, loss;
   int32_t recover_pupil;  // EPRY pupil recovery (common.rs:271-300; tiles up to kFusedLdsMaxN)
-  int32_t pad_;
+  uint32_t bcols;         // blocks per spectrum row of `bounds`
   double object_step, alpha, eps, max_p, friction, feedback;
   double pupil_step;
 };
@@ -112,5 +114,10 @@ void prepare_sweep(uint32_t nmax);
 // that does (tiles up to kFusedLdsMaxN).
 void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, bool cooperative,
                   hipStream_t stream, bool epry = false);
+// Side of the spectrum blocks of TileDesc::bounds.
+constexpr uint32_t kBoundBlock = 8;
+// Recomputes TileDesc::bounds exactly (max |O|^2 per block) for every
+// descriptor that has them; one workgroup per descriptor.
+void launch_patch_bounds(const TileDesc* tiles_dev, uint32_t count, hipStream_t stream);
 
 }  // namespace fpmhip

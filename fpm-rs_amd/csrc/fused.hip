@@ -210,8 +210,11 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
+// Reduction slots in LDS: three doubles per wave (two sums, one maximum).
+constexpr int kRedSlots = 3 * (kThreads / 64);
+
 // Fixed-order workgroup sums of two doubles per thread (one pair of
-// barriers for both); red holds 2 * (kThreads / 64) doubles.
+// barriers for both); red holds kRedSlots doubles.
 __device__ __forceinline__ double2 wg_sum2(double a, double b, double* red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -529,12 +532,35 @@ __device__ __forceinline__ bool in_extent(uint32_t e, int c) {
   return c >= (int)(e & 0xFFFFu) && c < (int)(e >> 16);
 }
 
+// Upper bound of |O|^2 over the spectrum rows r0 .. r1 and columns c0 .. c1
+// (inclusive, clamped to the spectrum) from the block bounds (T.bounds), read
+// by atomic operations,
+// which the L2 serves, so that the raises of this workgroup's inserts are seen.
+__device__ __forceinline__ double block_bound(const TileDesc& T, uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1) {
+  r1 = r1 < T.rh ? r1 : T.rh - 1;  // a tap row or column past the spectrum is never read
+  c1 = c1 < T.rw ? c1 : T.rw - 1;
+  uint64_t m = 0;
+  for (uint32_t br = r0 / kBoundBlock; br <= r1 / kBoundBlock; ++br)
+    for (uint32_t bc = c0 / kBoundBlock; bc <= c1 / kBoundBlock; ++bc) {
+      const uint64_t v =
+          __hip_atomic_fetch_max(T.bounds + br * T.bcols + bc, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      m = v > m ? v : m;
+    }
+  return __longlong_as_double((long long)m);
+}
+
 // max |patch|^2 over the whole n x n bilinear patch of a source (the EPRY
 // pupil-step denominator, common.rs:271-277), in FP64: thread t takes column
 // t % N and N^2 / kThreads consecutive rows from row (t / N) N^2 / kThreads,
-// tap rows shared between consecutive pixels, in groups of G rows.
+// tap rows shared between consecutive pixels, in groups of G rows. With
+// block bounds (T.bounds), a group whose taps' blocks cannot exceed `lower`
+// (|patch|^2 of a patch pixel, so a lower bound of the maximum) is skipped:
+// a bilinear value's magnitude is at most its largest tap's (non-negative
+// weights summing to one), and the 1e-14 relative margin covers the FP64
+// rounding of both sides, so the maximum is the full pass's, bit for bit.
 template <int N>
-__device__ __forceinline__ double patch_max(Rsrc rspec, const SrcParam& sp, uint32_t rw, int tid) {
+__device__ __forceinline__ double patch_max(const TileDesc& T, Rsrc rspec, const SrcParam& sp, uint32_t rw, int tid,
+                                            double lower) {
   constexpr int PP = N * N / kThreads, G = PP < 8 ? PP : 8;
   static_assert(PP >= 1 && PP % G == 0, "patch rows per thread");
   const int c = tid % N, r0 = (tid / N) * PP;
@@ -546,22 +572,57 @@ __device__ __forceinline__ double patch_max(Rsrc rspec, const SrcParam& sp, uint
 #pragma unroll
   for (int g = 0; g < PP; g += G) {
     const uint32_t o = (sp.ar.lower + (uint32_t)(r0 + g)) * rw + sp.ac.lower + (uint32_t)c;
+    bool skip = false;
+    if (T.bounds) {
+      const uint32_t rr = sp.ar.lower + (uint32_t)(r0 + g), cc = sp.ac.lower + (uint32_t)c;
+      skip = block_bound(T, rr, rr + G, cc, cc + 1) * (1.0 + 1e-14) <= lower;
+    }
     double2 t0[G + 1], t1[G + 1];
 #pragma unroll
     for (int j = 0; j <= G; ++j) {
-      const bool need = j < G || fr;
+      const bool need = !skip && (j < G || fr);
       t0[j] = bld_d2(rspec, need ? (o + j * rw) * 16u : kSkip);
       t1[j] = bld_d2(rspec, need && fc ? (o + j * rw + 1) * 16u : kSkip);
     }
+    if (!skip) {
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-      const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
-      const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
-                                     fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
-      m = fmax(m, v.x * v.x + v.y * v.y);
+      for (int j = 0; j < G; ++j) {
+        const double2 ul = fr ? t0[j + 1] : zero, uu = fr ? t1[j + 1] : zero;
+        const double2 v = make_double2(fma(uu.x, wuu, fma(ul.x, wul, fma(t1[j].x, wlu, t0[j].x * wll))),
+                                       fma(uu.y, wuu, fma(ul.y, wul, fma(t1[j].y, wlu, t0[j].y * wll))));
+        m = fmax(m, v.x * v.x + v.y * v.y);
+      }
     }
   }
   return m;
+}
+
+// Exact block bounds of one tile: max |O|^2 over each kBoundBlock x
+// kBoundBlock block of the spectrum (partial blocks at the edges), as FP64
+// bit patterns. Wave w takes block rows w, w + waves, ...; lane l the columns
+// c0 + l of a 64-column strip, the 8 rows of the block row, then a maximum
+// over the 8 lanes of each block.
+__global__ void __launch_bounds__(kThreads) k_patch_bounds(const TileDesc* __restrict__ tiles) {
+  const TileDesc& T = tiles[blockIdx.x];
+  if (!T.bounds || T.member != 0) return;
+  const uint32_t rh = T.rh, rw = T.rw, brows = (rh + kBoundBlock - 1) / kBoundBlock;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = kThreads / 64;
+  const Rsrc rs = make_rsrc(T.spec, rh * rw * 16u);
+  for (uint32_t br = wave; br < brows; br += waves)
+    for (uint32_t c0 = 0; c0 < rw; c0 += 64) {
+      const uint32_t c = c0 + lane;
+      double m = 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < kBoundBlock; ++j) {
+        const uint32_t r = br * kBoundBlock + j;
+        const double2 v = bld_d2(rs, (r < rh && c < rw) ? (r * rw + c) * 16u : kSkip);
+        m = fmax(m, v.x * v.x + v.y * v.y);
+      }
+#pragma unroll
+      for (int off = 1; off < (int)kBoundBlock; off <<= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      if (lane % kBoundBlock == 0 && c < rw)
+        T.bounds[br * T.bcols + c / kBoundBlock] = (uint64_t)__double_as_longlong(m);
+    }
 }
 
 // Bilinear patch x pupil (crop.rs:129-167) over the tile's list of support
@@ -575,9 +636,9 @@ __device__ __forceinline__ double patch_max(Rsrc rspec, const SrcParam& sp, uint
 // without a fraction has an upper weight of exactly zero and its upper taps
 // are neither loaded nor used (they contribute zero).
 template <int N, bool EPRY = false>
-__device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
-                                              const uint32_t* sl, uint32_t ns, uint32_t en, float2* lds,
-                                              Rsrc rpatch) {
+__device__ __forceinline__ double gather_strips(Rsrc rspec, Rsrc rpup, const SrcParam& sp, uint32_t rw, int tid,
+                                                const uint32_t* sl, uint32_t ns, uint32_t en, float2* lds,
+                                                Rsrc rpatch) {
   constexpr int LD = pitch<N>(), HALF = N / 2, NN = N * N, S = (int)kGatherStrip;
   static_assert(NN % kThreads == 0, "pixels per thread");
   const Rsrc rl = make_rsrc(sl, ns * 4u);
@@ -591,6 +652,7 @@ __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcPa
   const double wll = sp.ar.wl * sp.ac.wl, wlu = sp.ar.wl * sp.ac.wu, wul = sp.ar.wu * sp.ac.wl,
                wuu = sp.ar.wu * sp.ac.wu;
   const double2 zero = make_double2(0.0, 0.0);
+  double vmax = 0.0;  // EPRY: max |patch|^2 over this thread's support pixels
   for (uint32_t i = (uint32_t)tid; i < ns; i += kThreads) {
     const uint32_t e = i == (uint32_t)tid ? en : (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rl, i * 4u, 0, 0);
     const uint32_t r0 = e & 0x3FFu, c = (e >> 10) & 0x3FFu, mask = e >> 20;
@@ -615,9 +677,13 @@ __device__ __forceinline__ void gather_strips(Rsrc rspec, Rsrc rpup, const SrcPa
       const float2 pv = f2((float)v.x, (float)v.y);
       if (mask >> j & 1u) lds[((r0 + j + HALF) % N) * LD + cc] = fmul(pv, P[j]);
       // EPRY keeps the patch of the support pixels for the pupil gradient
-      if (EPRY) bst_f2(rpatch, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip, pv);
+      if (EPRY) {
+        bst_f2(rpatch, (mask >> j & 1u) ? ((r0 + j) * N + c) * 8u : kSkip, pv);
+        if (mask >> j & 1u) vmax = fmax(vmax, v.x * v.x + v.y * v.y);
+      }
     }
   }
+  return vmax;
 }
 
 // gather_strips for one row chunk of a large tile: the strips of list
@@ -687,10 +753,13 @@ __device__ __forceinline__ void insert_list_first(const uint32_t* dl, uint32_t n
   }
 }
 
+// With `bounds` (EPRY's block bounds of |O|^2, bcols blocks per spectrum
+// row), every written value raises its block's bound when it exceeds it.
 template <int N, class MEM = PlainMem, class GetU>
 __device__ __forceinline__ void insert_list(double2* __restrict__ spec, uint32_t spec_elems, const SrcParam& sp,
                                             uint32_t rw, double scale, int tid, const uint32_t* dl, uint32_t nd,
-                                            uint32_t (&en)[kInsertGroup], GetU u) {
+                                            uint32_t (&en)[kInsertGroup], GetU u, uint64_t* bounds = nullptr,
+                                            uint32_t bcols = 0) {
   constexpr int AUX = kAux<MEM>;
   const bool fr = sp.ar.upper != sp.ar.lower, fc = sp.ac.upper != sp.ac.lower;
   const float wll = (float)(sp.ar.wl * sp.ac.wl), wlu = (float)(sp.ar.wl * sp.ac.wu),
@@ -741,9 +810,17 @@ __device__ __forceinline__ void insert_list(double2* __restrict__ spec, uint32_t
 #pragma unroll
     for (int k = 0; k < UNROLL; ++k) old[k] = bld_d2<AUX>(rs, off[k]);
 #pragma unroll
-    for (int k = 0; k < UNROLL; ++k)
-      bst_d2<AUX>(rs, off[k],
-                  make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y)));
+    for (int k = 0; k < UNROLL; ++k) {
+      const double2 nv = make_double2(fma(scale, (double)add[k].x, old[k].x), fma(scale, (double)add[k].y, old[k].y));
+      bst_d2<AUX>(rs, off[k], nv);
+      if (bounds && off[k] != kSkip) {
+        const uint32_t px = off[k] / 16u;  // spectrum pixel
+        uint64_t* bp = bounds + (px / rw / kBoundBlock) * bcols + (px % rw) / kBoundBlock;
+        const uint64_t n2 = (uint64_t)__double_as_longlong(nv.x * nv.x + nv.y * nv.y);
+        // a stale (lower) cached bound only costs a redundant atomic
+        if (n2 > *bp) __hip_atomic_fetch_max(bp, n2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
   }
 }
 
@@ -886,12 +963,27 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         den = make_denominator(T, fmax(maxp, eps));
         den.a = uniform(den.a);
         den.b = uniform(den.b);
-        const double maxo = wg_max(patch_max<N>(rspec, sp, T.rw, tid), red);
+      }
+      // 1. patch x pupil over the support pixels, written ifftshifted (EPRY:
+      //    with each wave's maximum of |patch|^2 over the support pixels in the
+      //    third set of reduction slots, read after the barrier)
+      {
+        double vm = gather_strips<N, EPRY>(rspec, rpup, sp, T.rw, tid, slist, nstrip, my_strip, lds, rpatch);
+        if constexpr (EPRY) {
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) vm = fmax(vm, __shfl_xor(vm, off, 64));
+          if ((threadIdx.x & 63) == 0) red[2 * (kThreads / 64) + (threadIdx.x >> 6)] = vm;
+        }
+      }
+      __syncthreads();
+      if constexpr (EPRY) {
+        // max |patch|^2 over the whole patch (common.rs:271-277); the support
+        // pixels' maximum prunes the pass
+        double lower = red[2 * (kThreads / 64)];
+        for (int w = 1; w < kThreads / 64; ++w) lower = fmax(lower, red[2 * (kThreads / 64) + w]);
+        const double maxo = wg_max(fmax(patch_max<N>(T, rspec, sp, T.rw, tid, lower), lower), red);
         pupil_k = uniform((float)(1.0 / (fmax(maxo, eps) + eps)) * inv_n);
       }
-      // 1. patch x pupil over the support pixels, written ifftshifted
-      gather_strips<N, EPRY>(rspec, rpup, sp, T.rw, tid, slist, nstrip, my_strip, lds, rpatch);
-      __syncthreads();
       // the next LED's frame index, then its record words (written to the
       // other slot after the forward transform)
       uint32_t next_f = 0, next_w = 0;
@@ -1082,9 +1174,12 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
       //    pixels that receive a non-zero update (U is zero outside the support
       //    extents, where it is not read)
       const double scale = fw * T.object_step;
-      insert_list<N>(spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 2, ndest, dfirst, [&](int sr_, int sc_) {
-        return in_extent(ext[sr_], sc_) ? lds[((sr_ + HALF) % N) * LD + (sc_ + HALF) % N] : f2(0.0f, 0.0f);
-      });
+      insert_list<N>(
+          spec, T.rh * T.rw, sp, T.rw, scale, tid, T.dest + 2, ndest, dfirst,
+          [&](int sr_, int sc_) {
+            return in_extent(ext[sr_], sc_) ? lds[((sr_ + HALF) % N) * LD + (sc_ + HALF) % N] : f2(0.0f, 0.0f);
+          },
+          EPRY ? T.bounds : nullptr, T.bcols);
       __syncthreads();
       // 7. mPIE momentum over the tile spectrum's changeable pixels (common.rs:308-387)
       if (T.momentum) {
@@ -1575,7 +1670,7 @@ __global__ void __launch_bounds__(kThreads)
   float2* twf = prev + (N > (int)kFusedLdsMaxN ? N : 0);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
-  int* bflag = reinterpret_cast<int*>(red + 2 * (kThreads / 64));
+  int* bflag = reinterpret_cast<int*>(red + kRedSlots);
   uint32_t* ext = reinterpret_cast<uint32_t*>(bflag + 4);
   uint32_t* rec = ext + N;  // two LED records
   const float2* src = tw_all + tw_offset(N);
@@ -1640,7 +1735,7 @@ __global__ void __launch_bounds__(kThreads)
   float2* twf = prev + (N > (int)kFusedLdsMaxN ? N : 0);
   float2* twi = twf + N;
   double* red = reinterpret_cast<double*>(twi + N);
-  int* bflag = reinterpret_cast<int*>(red + 2 * (kThreads / 64));
+  int* bflag = reinterpret_cast<int*>(red + kRedSlots);
   uint32_t* ext = reinterpret_cast<uint32_t*>(bflag + 4);
   uint32_t* rec = ext + N;
   const float2* src = tw_all + tw_offset(N);
@@ -1665,7 +1760,7 @@ __global__ void __launch_bounds__(kThreads)
 // LDS of one tile of n x n pixels (the layout of k_sweep).
 static size_t sweep_lds_bytes_one(uint32_t n) {
   return ((size_t)sweep_field_elems(n) + (n > kFusedLdsMaxN ? n : 0) + 2 * (size_t)n) * sizeof(float2) +
-         2 * (kThreads / 64) * sizeof(double) + 16 + (size_t)n * sizeof(uint32_t) + 2 * kRecPitch * sizeof(uint32_t);
+         kRedSlots * sizeof(double) + 16 + (size_t)n * sizeof(uint32_t) + 2 * kRecPitch * sizeof(uint32_t);
 }
 
 // A launch's LDS: the largest layout of any tile size up to nmax (a 256^2
@@ -1723,6 +1818,11 @@ uint32_t sweep_coresident_limit(uint32_t nmax) {
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel_for(nmax), kThreads,
                                                          sweep_lds_bytes(nmax)));
   return (uint32_t)(std::max(per_cu, 0) * std::max(cus, 0));
+}
+
+void launch_patch_bounds(const TileDesc* tiles_dev, uint32_t count, hipStream_t stream) {
+  hipLaunchKernelGGL(k_patch_bounds, dim3(count), dim3(kThreads), 0, stream, tiles_dev);
+  HIP_CHECK(hipGetLastError());
 }
 
 void launch_sweep(const TileDesc* tiles_dev, uint32_t count, const float2* tw_all, uint32_t nmax, bool cooperative,

@@ -63,6 +63,8 @@ struct SessionBuffers {
   DeviceBuffer pupil64;   // double2[N] FP64 master of the recovered pupil (MIXED_F32 EPRY pupil recovery):
                           // every pupil step accumulates into it and `pupil` is its complex64 copy
   DeviceBuffer patch;     // V[K*N]
+  DeviceBuffer bounds;    // uint64[ceil(R_h / 8) * ceil(R_w / 8)]: upper bounds of |O|^2 per 8 x 8 block of the
+                          // spectrum, as FP64 bit patterns (batched EPRY: prunes the whole-patch maximum)
   DeviceBuffer field;     // V[K*N]
   DeviceBuffer pred;      // double[N]
   DeviceBuffer proj;      // V[N]
