@@ -210,8 +210,13 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
-// Reduction slots in LDS: three doubles per wave (two sums, one maximum).
-constexpr int kRedSlots = 3 * (kThreads / 64);
+// Reduction slots in LDS, one double per wave in each of five sets: two
+// sums (frame objective), and three maxima of EPRY (|patch|^2 over the
+// support pixels, over the whole patch, and |P|^2 of the updated pupil), each
+// written before a barrier and read after it.
+constexpr int kWaves = kThreads / 64;
+constexpr int kSlotLower = 2 * kWaves, kSlotPatch = 3 * kWaves, kSlotPupil = 4 * kWaves;
+constexpr int kRedSlots = 5 * kWaves;
 
 // Fixed-order workgroup sums of two doubles per thread (one pair of
 // barriers for both); red holds kRedSlots doubles.
@@ -972,17 +977,22 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         if constexpr (EPRY) {
 #pragma unroll
           for (int off = 32; off > 0; off >>= 1) vm = fmax(vm, __shfl_xor(vm, off, 64));
-          if ((threadIdx.x & 63) == 0) red[2 * (kThreads / 64) + (threadIdx.x >> 6)] = vm;
+          if ((threadIdx.x & 63) == 0) red[kSlotLower + (threadIdx.x >> 6)] = vm;
         }
       }
       __syncthreads();
+      double lower = 0.0;
       if constexpr (EPRY) {
         // max |patch|^2 over the whole patch (common.rs:271-277); the support
-        // pixels' maximum prunes the pass
-        double lower = red[2 * (kThreads / 64)];
-        for (int w = 1; w < kThreads / 64; ++w) lower = fmax(lower, red[2 * (kThreads / 64) + w]);
-        const double maxo = wg_max(fmax(patch_max<N>(T, rspec, sp, T.rw, tid, lower), lower), red);
-        pupil_k = uniform((float)(1.0 / (fmax(maxo, eps) + eps)) * inv_n);
+        // pixels' maximum prunes the pass. Each wave's maximum goes to its
+        // slot, read after the barrier of the fused column stage (the pupil
+        // step needs it only at the update).
+        lower = red[kSlotLower];
+        for (int w = 1; w < kWaves; ++w) lower = fmax(lower, red[kSlotLower + w]);
+        double mo = patch_max<N>(T, rspec, sp, T.rw, tid, lower);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mo = fmax(mo, __shfl_xor(mo, off, 64));
+        if ((threadIdx.x & 63) == 0) red[kSlotPatch + (threadIdx.x >> 6)] = mo;
       }
       // the next LED's frame index, then its record words (written to the
       // other slot after the forward transform)
@@ -1076,6 +1086,11 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
         wave_sums2_to(red, (double)lsum_f, (double)lcnt_i);
       }
       __syncthreads();
+      if constexpr (EPRY) {
+        double maxo = lower;
+        for (int w = 0; w < kWaves; ++w) maxo = fmax(maxo, red[kSlotPatch + w]);
+        pupil_k = uniform((float)(1.0 / (fmax(maxo, eps) + eps)) * inv_n);
+      }
       if (tid == 0) {
         double ls = 0.0, lc = 0.0;
         for (int w = 0; w < kThreads / 64; ++w) {
@@ -1167,9 +1182,18 @@ __device__ __forceinline__ void sweep_tile(const TileDesc& T, float2* lds, const
             }
           }
         }
-        if (epry) maxp = wg_max(pmax, red);  // the next frame's max |P|^2 (zero off the support)
+        if constexpr (EPRY) {  // the next frame's max |P|^2 (zero off the support), read after the barrier
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) pmax = fmax(pmax, __shfl_xor(pmax, off, 64));
+          if ((threadIdx.x & 63) == 0) red[kSlotPupil + (threadIdx.x >> 6)] = pmax;
+        }
       }
       __syncthreads();
+      if constexpr (EPRY) {
+        double m = red[kSlotPupil];
+        for (int w = 1; w < kWaves; ++w) m = fmax(m, red[kSlotPupil + w]);
+        maxp = uniform(m);
+      }
       // 6. adjoint insert over the tile's live destinations: FP64 RMW of the
       //    pixels that receive a non-zero update (U is zero outside the support
       //    extents, where it is not read)
