@@ -267,6 +267,7 @@ def serving(requests, rate, iterations, seed=5, max_tiles=256):
     return {"requests": requests, "tiles": tiles, "rate_per_s": rate, "iterations": iterations,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "max_ms": float(lat.max()), "tiles_per_s": tiles / wall, "ticks": stats["ticks"],
+            "pool_reuses": stats["pool_reuses"], "pool_allocations": stats["pool_allocations"],
             "sizes": "64^2,128^2,256^2", "channels": "1-3 of 450/532/630 nm", "leds": "9x9..15x15"}
 
 

@@ -3,10 +3,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/fpmhip.h"
@@ -28,25 +32,80 @@ int capture_bad_alloc();
                      std::string(#expr) + ": " + hipGetErrorString(e_));                       \
   } while (0)
 
+// ---- device memory pool ---------------------------------------------------------
+// Cache of device blocks for an owner that allocates and frees many buffers
+// of recurring sizes on one device (the server's jobs: about 40 buffers each,
+// whose hipMalloc / hipFree pairs otherwise cost ~40 us and sit between the
+// server's sweep launches). A DeviceBuffer allocated on a thread inside a
+// PoolScope takes its block from the pool and gives it back on release.
+// Blocks come in size classes: multiples of 1/8 of the request's power of
+// two (at least 256 B), so a block is at most 1/8 larger than asked for.
+// A block given back is pending and is handed out again only after
+// recycle(), which the owner calls where no device work queued before that
+// point can still touch it (every stream the blocks are used on
+// synchronised): that replaces hipFree's implicit device synchronisation.
+// Idle blocks above `max_cached` bytes are freed at recycle(); a failed
+// hipMalloc frees every idle block and tries once more.
+class DevicePool {
+ public:
+  explicit DevicePool(size_t max_cached) : max_cached_(max_cached) {}
+  ~DevicePool();  // frees every idle and pending block
+  DevicePool(const DevicePool&) = delete;
+  DevicePool& operator=(const DevicePool&) = delete;
+  void* acquire(size_t bytes, size_t* cls);
+  void give_back(void* p, size_t cls);
+  void recycle();
+  uint64_t reuses() const { return reuses_; }
+  uint64_t allocations() const { return allocations_; }
+
+ private:
+  void free_idle_locked(size_t keep);
+  std::mutex mu_;
+  std::map<size_t, std::vector<void*>> idle_;  // size class -> blocks
+  std::vector<std::pair<void*, size_t>> pending_;
+  size_t idle_bytes_ = 0;
+  size_t max_cached_;
+  std::atomic<uint64_t> reuses_{0}, allocations_{0};
+};
+// Size class of a request (the block size a pool hands out for it).
+size_t pool_size_class(size_t bytes);
+// Pool of the DeviceBuffers allocated on this thread (null: hipMalloc).
+DevicePool*& thread_pool();
+struct PoolScope {
+  DevicePool* saved;
+  explicit PoolScope(DevicePool* p) : saved(thread_pool()) { thread_pool() = p; }
+  ~PoolScope() { thread_pool() = saved; }
+  PoolScope(const PoolScope&) = delete;
+  PoolScope& operator=(const PoolScope&) = delete;
+};
+
 // ---- device buffer ------------------------------------------------------------
 struct DeviceBuffer {
   void* ptr = nullptr;
   size_t bytes = 0;
+  DevicePool* pool = nullptr;  // the block's pool (null: from hipMalloc)
+  size_t cls = 0;              // its size class in that pool
   DeviceBuffer() = default;
   explicit DeviceBuffer(size_t n) { allocate(n); }
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes), pool(o.pool), cls(o.cls) {
     o.ptr = nullptr;
     o.bytes = 0;
+    o.pool = nullptr;
+    o.cls = 0;
   }
   DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
     if (this != &o) {
       release();
       ptr = o.ptr;
       bytes = o.bytes;
+      pool = o.pool;
+      cls = o.cls;
       o.ptr = nullptr;
       o.bytes = 0;
+      o.pool = nullptr;
+      o.cls = 0;
     }
     return *this;
   }
@@ -54,13 +113,25 @@ struct DeviceBuffer {
   void allocate(size_t n) {
     release();
     if (n == 0) return;
-    HIP_CHECK(hipMalloc(&ptr, n));
+    if (DevicePool* p = thread_pool()) {
+      ptr = p->acquire(n, &cls);
+      pool = p;
+    } else {
+      HIP_CHECK(hipMalloc(&ptr, n));
+    }
     bytes = n;
   }
   void release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) {
+      if (pool)
+        pool->give_back(ptr, cls);
+      else
+        (void)hipFree(ptr);
+    }
     ptr = nullptr;
     bytes = 0;
+    pool = nullptr;
+    cls = 0;
   }
   template <class T>
   T* as() const {

@@ -9,7 +9,10 @@
 // resident. A job that reaches its iteration count is finished (result
 // transform, download) and leaves; the queue is re-examined between ticks.
 // Admissions and the downloads of finished jobs run on a second stream while
-// the next tick's sweep runs on the first.
+// the next tick's sweep runs on the first. Job buffers come from a device
+// memory pool of the server, recycled between ticks, so that the frees of
+// finished jobs and the allocations of new ones do not call hipFree /
+// hipMalloc between sweep launches.
 // Tiles are independent workgroups and sweeps are sequential within a tile,
 // so a job's result does not depend on which other jobs share its ticks.
 #include <algorithm>
@@ -98,6 +101,9 @@ struct Job {
 
 struct fpmhip_server {
   static constexpr int kAdmitThreads = 4;
+  // Device blocks of the jobs' buffers (declared first: it outlives every
+  // job). Blocks given back become reusable at recycle_pool().
+  std::unique_ptr<DevicePool> pool;
   fpmhip_context own;                   // private stream of the ticks: never the caller's stream
   fpmhip_context stage[kAdmitThreads];  // streams of the admission threads, while a tick runs
   fpmhip_context drain;                 // stream of the result downloads, while a tick runs
@@ -129,6 +135,9 @@ struct fpmhip_server {
   // device objects are handed over to it instead of released (a free could
   // wait for the running tick) and stay counted in resident_bytes.
   void finish_all(std::vector<Job*>& done, std::vector<Job>* held);
+  // Makes the blocks given back to the pool reusable. Called between ticks,
+  // after every stream that job buffers are used on has been synchronised.
+  void recycle_pool();
 };
 
 namespace {
@@ -189,6 +198,7 @@ bool fpmhip_server::admit(Job& j, fpmhip_context& c) {
 }
 
 void fpmhip_server::finish(Job& j) {
+  PoolScope scope(pool.get());
   fpmhip_session& s = *j.sess;
   s.ctx = &drain;  // the result transform and downloads run on the drain stream
   const uint64_t M = s.model->M();
@@ -283,6 +293,7 @@ void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
   const size_t t = std::min<size_t>(n, kAdmitThreads);
   auto work = [&](size_t k) {
     (void)hipSetDevice(ctx->device);
+    PoolScope scope(pool.get());
     for (size_t i = k; i < n; i += t) admit_one(*to_admit[i], stage[k]);
   };
   std::vector<std::thread> pool;
@@ -290,6 +301,14 @@ void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
   if (n) work(0);
   for (std::thread& th : pool) th.join();
   to_admit.clear();
+}
+
+void fpmhip_server::recycle_pool() {
+  // an admission that failed part-way may have left work on its stream
+  (void)hipStreamSynchronize(own.stream);
+  (void)hipStreamSynchronize(drain.stream);
+  for (fpmhip_context& c : stage) (void)hipStreamSynchronize(c.stream);
+  pool->recycle();
 }
 
 void fpmhip_server::run() {
@@ -327,6 +346,7 @@ void fpmhip_server::run() {
     if (active.empty()) {
       finish_all(finished, nullptr);
       admit_all(to_admit);
+      recycle_pool();
       if (busy_since >= 0.0) {
         std::lock_guard<std::mutex> lock(mu);
         stats.busy_seconds += now_s() - busy_since;
@@ -403,24 +423,28 @@ void fpmhip_server::run() {
       if (drainer.joinable()) drainer.join();
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
       release_held();
+      recycle_pool();
     } catch (const Failure& f) {
       // no-ops for the lists handled before the failure
       release_held();
       finish_all(finished, nullptr);
       admit_all(to_admit);
-      std::lock_guard<std::mutex> lock(mu);
-      for (Job* j : active) {
-        j->error_code = f.code;
-        j->error = "device failure during a sweep";
-        j->release_device();
-        j->state = kFailed;
-        j->t_done = now_s();
-        stats.failed++;
-        resident.erase(std::find(resident.begin(), resident.end(), j));
-        resident_bytes -= j->bytes;
-        resident_wgs -= j->wgs;
+      {
+        std::lock_guard<std::mutex> lock(mu);
+        for (Job* j : active) {
+          j->error_code = f.code;
+          j->error = "device failure during a sweep";
+          j->release_device();
+          j->state = kFailed;
+          j->t_done = now_s();
+          stats.failed++;
+          resident.erase(std::find(resident.begin(), resident.end(), j));
+          resident_bytes -= j->bytes;
+          resident_wgs -= j->wgs;
+        }
+        cv_done.notify_all();
       }
-      cv_done.notify_all();
+      recycle_pool();
       continue;
     }
     {
@@ -463,6 +487,8 @@ int fpmhip_server_create(fpmhip_context* c, const fpmhip_server_config* config, 
     }
     s->ctx = &s->own;
     s->cfg = *config;
+    // idle pooled blocks kept: a quarter of the admission budget, at most 4 GiB
+    s->pool = std::make_unique<DevicePool>(std::min<uint64_t>(config->hbm_budget_bytes / 4, 4ull << 30));
     s->wg_limit = std::max<uint32_t>(sweep_coresident_limit(kFusedMaxN), kGroupMembers);  // tiles up to 256^2
     make_sweep_twiddles(s->tw, s->own.stream);
     s->tiles.allocate(config->max_tiles * sizeof(TileDesc));
@@ -606,6 +632,8 @@ int fpmhip_server_stats(fpmhip_server* s, fpmhip_server_statistics* out) {
     *out = s->stats;
     out->resident_tiles = s->resident.size();
     out->resident_bytes = s->resident_bytes;
+    out->pool_reuses = s->pool->reuses();
+    out->pool_allocations = s->pool->allocations();
   });
 }
 

@@ -550,6 +550,8 @@ typedef struct fpmhip_server_statistics {
   uint64_t resident_tiles;   /* at the time of the call */
   uint64_t resident_bytes;
   double busy_seconds;       /* wall time the worker spent with resident tiles */
+  uint64_t pool_reuses;      /* job buffers served from the server's device memory pool */
+  uint64_t pool_allocations; /* job buffers the pool had to allocate (hipMalloc) */
 } fpmhip_server_statistics;
 
 int fpmhip_server_create(fpmhip_context* context, const fpmhip_server_config* config,
