@@ -7,12 +7,14 @@
 // quota of sweeps inversely proportional to its per-sweep work (LEDs x
 // pixels), so that a small tile is not held to the pace of the largest one
 // resident. A job that reaches its iteration count is finished (result
-// transform, download) and leaves; the queue is re-examined between ticks.
-// Admissions and the downloads of finished jobs run on a second stream while
-// the next tick's sweep runs on the first. Job buffers come from a device
-// memory pool of the server, recycled between ticks, so that the frees of
-// finished jobs and the allocations of new ones do not call hipFree /
-// hipMalloc between sweep launches.
+// transform, download) and leaves. Each tick is planned before it is
+// launched; the jobs it completes are known then, so queued jobs are
+// selected into the room they leave and admitted (uploaded, initialised)
+// on streams of their own while the tick runs, together with the downloads
+// of the jobs the previous tick finished, and join the next tick. Job
+// buffers come from a device memory pool of the server, recycled between
+// ticks, so that the frees of finished jobs and the allocations of new ones
+// do not call hipFree / hipMalloc between sweep launches.
 // Tiles are independent workgroups and sweeps are sequential within a tile,
 // so a job's result does not depend on which other jobs share its ticks.
 #include <algorithm>
@@ -77,15 +79,6 @@ struct Job {
   double t_submit = 0, t_admit = 0, t_done = 0;
   std::vector<double> object, spectrum, trace;
 
-  // Moves the device objects into `to` (released later, by its own
-  // release_device), leaving this job without device state.
-  void hand_over(Job& to) {
-    std::swap(sess, to.sess);
-    std::swap(meas, to.meas);
-    std::swap(model, to.model);
-    std::swap(scratch, to.scratch);
-    to.bytes = bytes;
-  }
   void release_device() {
     if (sess) fpmhip_session_destroy(sess);
     if (meas) fpmhip_measurements_destroy(meas);
@@ -115,7 +108,7 @@ struct fpmhip_server {
   std::map<uint64_t, std::unique_ptr<Job>> jobs;
   std::vector<Job*> resident;
   uint64_t resident_bytes = 0;
-  uint64_t resident_wgs = 0;  // workgroups of the resident, unfinished tiles
+  uint64_t resident_wgs = 0;  // workgroups of the resident tiles
   uint32_t wg_limit = 0;      // workgroups of k_sweep resident at once on the device
   uint64_t next_id = 1;
   bool stop = false;
@@ -131,10 +124,15 @@ struct fpmhip_server {
   // Admits jobs on up to kAdmitThreads threads and empties the list.
   void admit_all(std::vector<Job*>& to_admit);
   void finish(Job& j);
-  // Finishes jobs (finish) and marks them done or failed; with `held`, their
-  // device objects are handed over to it instead of released (a free could
-  // wait for the running tick) and stay counted in resident_bytes.
-  void finish_all(std::vector<Job*>& done, std::vector<Job>* held);
+  // Finishes jobs (finish), releases their device objects and marks them
+  // done or failed; empties the list.
+  void finish_all(std::vector<Job*>& done);
+  // Moves queued jobs to `to_admit` while the next tick has tile slots,
+  // co-resident workgroups and HBM budget for them. The tiles of finishing
+  // jobs and of `leaving` (the jobs the tick being planned completes) take
+  // no part in the next tick, so their slots and workgroups count as free;
+  // their bytes still count until they are released.
+  void select_admissions(std::vector<Job*>& to_admit, const std::vector<Job*>& leaving);
   // Makes the blocks given back to the pool reusable. Called between ticks,
   // after every stream that job buffers are used on has been synchronised.
   void recycle_pool();
@@ -226,7 +224,7 @@ void fpmhip_server::finish(Job& j) {
   j.error_code = rc;
 }
 
-void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) {
+void fpmhip_server::finish_all(std::vector<Job*>& done) {
   for (Job* j : done) {
     try {
       finish(*j);
@@ -237,12 +235,7 @@ void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) 
       j->error_code = FPMHIP_DEVICE_ERROR;
       j->error = "result download failed";
     }
-    if (held) {
-      held->emplace_back();
-      j->hand_over(held->back());
-    } else {
-      j->release_device();
-    }
+    j->release_device();  // into the pool: no hipFree while a tick runs
     std::lock_guard<std::mutex> lock(mu);
     j->t_done = now_s();
     j->state = j->error_code ? kFailed : kDone;
@@ -251,11 +244,37 @@ void fpmhip_server::finish_all(std::vector<Job*>& done, std::vector<Job>* held) 
     else
       stats.completed++;
     resident.erase(std::find(resident.begin(), resident.end(), j));
-    if (!held) resident_bytes -= j->bytes;
+    resident_bytes -= j->bytes;
     resident_wgs -= j->wgs;
     cv_done.notify_all();
   }
   done.clear();
+}
+
+void fpmhip_server::select_admissions(std::vector<Job*>& to_admit, const std::vector<Job*>& leaving) {
+  std::lock_guard<std::mutex> lock(mu);
+  uint64_t bytes = resident_bytes, wgs = resident_wgs;
+  size_t slots = resident.size();
+  auto free_up = [&](const Job* j) {
+    wgs -= j->wgs;
+    --slots;
+  };
+  for (const Job* j : resident)
+    if (j->finishing) free_up(j);
+  for (const Job* j : leaving) free_up(j);
+  // the resident tiles of the next tick, large ones as cooperative groups,
+  // fit on the device at once, so that no large tile falls back to a single
+  // workgroup (a job over the HBM or workgroup budget is still admitted when
+  // nothing else would run)
+  while (!queue.empty() && slots < cfg.max_tiles) {
+    Job* j = jobs[queue.front()].get();
+    if (slots > 0 && (bytes + j->bytes > cfg.hbm_budget_bytes || wgs + j->wgs > wg_limit)) break;
+    queue.pop_front();
+    to_admit.push_back(j);
+    bytes += j->bytes;
+    wgs += j->wgs;
+    ++slots;
+  }
 }
 
 void fpmhip_server::admit_one(Job& j, fpmhip_context& c) {
@@ -316,35 +335,18 @@ void fpmhip_server::run() {
   double busy_since = -1.0;
   std::vector<Job*> finished;  // jobs whose last tick has run
   for (;;) {
-    std::vector<Job*> to_admit;
+    std::vector<Job*> active;
     {
       std::unique_lock<std::mutex> lock(mu);
       cv_work.wait(lock, [&] { return stop || !queue.empty() || !resident.empty(); });
       if (stop) break;
-      // admission by slots, HBM footprint and workgroups: the resident
-      // tiles, large ones as cooperative groups, fit on the device at once,
-      // so that no large tile falls back to a single workgroup (a job over
-      // the HBM or workgroup budget is still admitted when nothing else is)
-      uint64_t bytes = resident_bytes, wgs = resident_wgs;
-      size_t slots = resident.size();
-      while (!queue.empty() && slots < cfg.max_tiles) {
-        Job* j = jobs[queue.front()].get();
-        if (slots > 0 && (bytes + j->bytes > cfg.hbm_budget_bytes || wgs + j->wgs > wg_limit)) break;
-        queue.pop_front();
-        to_admit.push_back(j);
-        bytes += j->bytes;
-        wgs += j->wgs;
-        ++slots;
-      }
-    }
-    std::vector<Job*> active;
-    {
-      std::lock_guard<std::mutex> lock(mu);
       for (Job* j : resident)
         if (!j->finishing) active.push_back(j);
     }
+    std::vector<Job*> to_admit;
     if (active.empty()) {
-      finish_all(finished, nullptr);
+      select_admissions(to_admit, {});
+      finish_all(finished);
       admit_all(to_admit);
       recycle_pool();
       if (busy_since >= 0.0) {
@@ -357,17 +359,7 @@ void fpmhip_server::run() {
     if (busy_since < 0.0) busy_since = now_s();
     // one tick: every resident tile advances by its quota of sweeps
     std::vector<uint32_t> quota(active.size());
-    std::vector<Job> held;
-    auto release_held = [&] {
-      uint64_t freed = 0;
-      for (Job& h : held) {
-        h.release_device();
-        freed += h.bytes;
-      }
-      held.clear();
-      std::lock_guard<std::mutex> lock(mu);
-      resident_bytes -= freed;
-    };
+    uint64_t grid = 0;
     try {
       std::vector<TileDesc> desc(active.size());
       uint32_t nmax = 0;
@@ -393,26 +385,33 @@ void fpmhip_server::run() {
       for (const TileDesc& t : launch)
         if (t.members > 1 && t.member == 0) work[index[t.spec]] /= kGroupSpeedup;
       const double wmax = *std::max_element(work.begin(), work.end());
+      std::vector<Job*> leaving;  // jobs this tick completes
       for (size_t i = 0; i < active.size(); ++i) {
         const double share = std::max(std::floor(kTickSweeps * wmax / work[i]), 1.0);
-        quota[i] = (uint32_t)std::min<double>(share, (double)(active[i]->iterations - active[i]->done_iterations));
+        const uint64_t left = active[i]->iterations - active[i]->done_iterations;
+        quota[i] = (uint32_t)std::min<double>(share, (double)left);
+        if (quota[i] >= left) leaving.push_back(active[i]);
       }
       for (TileDesc& t : launch) t.sweeps = quota[index[t.spec]];
       bool cooperative = false;
       for (const TileDesc& t : launch) cooperative |= t.members > 1;
+      // the jobs admitted while this tick runs join the next one, in the
+      // room the finished and the leaving jobs make
+      select_admissions(to_admit, leaving);
       if (tiles.bytes < launch.size() * sizeof(TileDesc))
         tiles.allocate(std::max<size_t>(launch.size(), cfg.max_tiles * kGroupMembers) * sizeof(TileDesc));
       HIP_CHECK(hipMemcpyAsync(tiles.ptr, launch.data(), launch.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                ctx->stream));
       launch_sweep(tiles.as<TileDesc>(), (uint32_t)launch.size(), tw.as<float2>(), nmax, cooperative, ctx->stream);
+      grid = launch.size();
       // while the tick runs: the results of the jobs finished by the last
-      // tick are downloaded, and the queued jobs are admitted (uploads and
-      // initialisation), on the second stream
+      // tick are downloaded, and the selected jobs are admitted (uploads and
+      // initialisation), on streams of their own
       std::thread drainer;
       if (!finished.empty())
         drainer = std::thread([&] {
           (void)hipSetDevice(ctx->device);
-          finish_all(finished, &held);
+          finish_all(finished);
         });
       try {
         admit_all(to_admit);
@@ -422,12 +421,10 @@ void fpmhip_server::run() {
       }
       if (drainer.joinable()) drainer.join();
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      release_held();
       recycle_pool();
     } catch (const Failure& f) {
       // no-ops for the lists handled before the failure
-      release_held();
-      finish_all(finished, nullptr);
+      finish_all(finished);
       admit_all(to_admit);
       {
         std::lock_guard<std::mutex> lock(mu);
@@ -450,6 +447,7 @@ void fpmhip_server::run() {
     {
       std::lock_guard<std::mutex> lock(mu);
       stats.ticks++;
+      stats.peak_tick_workgroups = std::max(stats.peak_tick_workgroups, grid);
       for (size_t i = 0; i < active.size(); ++i) {
         Job* j = active[i];
         j->done_iterations += quota[i];

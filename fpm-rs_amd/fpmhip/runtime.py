@@ -755,7 +755,8 @@ class ServerStatistics(ct.Structure):
     _fields_ = [("submitted", ct.c_uint64), ("admitted", ct.c_uint64), ("completed", ct.c_uint64),
                 ("failed", ct.c_uint64), ("ticks", ct.c_uint64), ("resident_tiles", ct.c_uint64),
                 ("resident_bytes", ct.c_uint64), ("busy_seconds", ct.c_double),
-                ("pool_reuses", ct.c_uint64), ("pool_allocations", ct.c_uint64)]
+                ("pool_reuses", ct.c_uint64), ("pool_allocations", ct.c_uint64),
+                ("peak_tick_workgroups", ct.c_uint64)]
 
 
 class Server:

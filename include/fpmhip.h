@@ -552,6 +552,8 @@ typedef struct fpmhip_server_statistics {
   double busy_seconds;       /* wall time the worker spent with resident tiles */
   uint64_t pool_reuses;      /* job buffers served from the server's device memory pool */
   uint64_t pool_allocations; /* job buffers the pool had to allocate (hipMalloc) */
+  uint64_t peak_tick_workgroups; /* largest grid of one tick (at most the device's co-resident
+                                    k_sweep workgroups) */
 } fpmhip_server_statistics;
 
 int fpmhip_server_create(fpmhip_context* context, const fpmhip_server_config* config,
