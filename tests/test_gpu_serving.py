@@ -129,6 +129,36 @@ def test_small_job_is_not_paced_by_a_large_one(ctx):
     assert np.array_equal(alone[1], rs[1]) and np.array_equal(alone[2], rs[2])
 
 
+def test_pooled_buffers_are_reused_and_results_unchanged(ctx):
+    """A served job's device buffers come from the server's memory pool.
+    Served a second time, after the first runs have finished, the same jobs
+    allocate nothing new: every buffer is a block of the first runs, stale
+    contents and all. The objects, spectra and traces are bitwise those of
+    the first runs."""
+    jobs = [job(128, 9, 21), job(256, 7, 22, 630e-9)]
+    srv = runtime.Server(max_tiles=4)
+
+    def serve_each():
+        out = []
+        for p in jobs:
+            jid = srv.submit(p, ALG)
+            srv.wait(jid, 120)
+            out.append(srv.result(jid))
+        return out, srv.stats()
+
+    first, st1 = serve_each()
+    again, st2 = serve_each()
+    srv.close()
+    say("pool after the first runs", st1, "after the second", st2)
+    assert st2["completed"] == 4 and st2["failed"] == 0
+    assert st1["pool_allocations"] > 0
+    assert st2["pool_allocations"] == st1["pool_allocations"]
+    assert st2["pool_reuses"] - st1["pool_reuses"] == st1["pool_allocations"] + st1["pool_reuses"]
+    for x, y in zip(first, again):
+        for k in range(3):
+            assert np.array_equal(x[k], y[k])
+
+
 def test_admission_keeps_large_tiles_as_cooperative_groups(ctx):
     """A 256^2 tile is admitted as a cooperative group of 8 workgroups: with
     every workgroup of a tick held on the device at once (one k_sweep
