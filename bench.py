@@ -302,9 +302,10 @@ def uneven_tiles(seed=11, only=None):
 
 
 def run_uneven(iterations=40, steps=1, warmup=1):
-    """The uneven 96-tile batch, FP32 mPIE `iterations` sweeps, one fused
-    launch per sweep (graph replay). Returns tiles/s, the sweep-kernel
-    figures and the algorithmic bytes."""
+    """The uneven 96-tile batch, FP32 mPIE `iterations` sweeps, replayed
+    from the batch's captured graph: one persistent fused launch runs the
+    schedule's sweeps_per_launch (default 10) sweeps of every tile. Returns
+    tiles/s, the sweep-kernel figures and the algorithmic bytes."""
     log("building the uneven 96-tile batch")
     probs = uneven_tiles()
     ctx = runtime.default_context()
