@@ -315,10 +315,10 @@ void fpmhip_server::admit_all(std::vector<Job*>& to_admit) {
     PoolScope scope(pool.get());
     for (size_t i = k; i < n; i += t) admit_one(*to_admit[i], stage[k]);
   };
-  std::vector<std::thread> pool;
-  for (size_t k = 1; k < t; ++k) pool.emplace_back(work, k);
+  std::vector<std::thread> helpers;
+  for (size_t k = 1; k < t; ++k) helpers.emplace_back(work, k);
   if (n) work(0);
-  for (std::thread& th : pool) th.join();
+  for (std::thread& th : helpers) th.join();
   to_admit.clear();
 }
 
