@@ -4,9 +4,9 @@
 // admitted while tile slots and the HBM budget allow, initialised on the
 // device, and then advanced together with every other resident tile by one
 // launch of the fused sweep kernel per tick. Within a tick every tile runs a
-// quota of sweeps inversely proportional to its per-sweep work (LEDs x
-// pixels), so that a small tile is not held to the pace of the largest one
-// resident. A job that reaches its iteration count is finished (result
+// quota of sweeps inversely proportional to its per-sweep work (LEDs x the
+// measured time of one LED update at its size), so that a small tile is not
+// held to the pace of the largest one resident. A job that reaches its iteration count is finished (result
 // transform, download) and leaves. Each tick is planned before it is
 // launched; the jobs it completes are known then, so queued jobs are
 // selected into the room they leave and admitted (uploaded, initialised)
@@ -148,19 +148,30 @@ uint64_t job_bytes(const fpmhip_model_desc& d, uint64_t iterations) {
   return 2 * M * 16 + 2 * M * 8 + 12 * N * 16 + F * N * 2 + N * 25 + F * 64 + 16 + (iterations + F) * 8;
 }
 
-// Speed-up of a cooperative group (kGroupMembers workgroups) over one
-// workgroup on a 256^2 tile (measured: about 5x), used only to balance
-// per-tick sweep quotas.
-constexpr double kGroupSpeedup = 5.0;
 // A tick lasts about this many sweeps of its heaviest tile: every other tile
 // runs floor(kTickSweeps x heaviest work / its work) sweeps, so the rounding
 // leaves a tile idle for less than 1 / kTickSweeps of the tick.
 constexpr double kTickSweeps = 2.0;
 
-// Work of one sweep of a job's tile: LED updates x pixels per update.
-double sweep_work(const Job& j) {
+// Time of one LED update of the fused sweep by tile size, in microseconds:
+// batches of one tile size at a time, 13 x 13 LEDs, on an MI355X (256^2
+// tiles as cooperative groups of kGroupMembers workgroups, or on one
+// workgroup). It is not proportional to the pixels: a small tile's LED is
+// bound by the latency of its dependent phases. Used only to balance the
+// per-tick sweep quotas.
+double led_cost_us(uint32_t n, bool grouped) {
+  switch (n) {
+    case 32: return 9.1;
+    case 64: return 12.2;
+    case 128: return 26.2;
+    default: return grouped ? 35.7 : 199.0;  // 256^2
+  }
+}
+
+// Work of one sweep of a job's tile: LED updates x the cost of one.
+double sweep_work(const Job& j, bool grouped) {
   const double F = (double)(j.desc.mux_row_ptr ? j.desc.frame_count : j.desc.source_count);
-  return F * (double)(j.desc.image_h * j.desc.image_w);
+  return F * led_cost_us((uint32_t)j.desc.image_h, grouped);
 }
 
 }  // namespace
@@ -378,12 +389,13 @@ void fpmhip_server::run() {
         prepared_nmax = nmax;
       }
       std::vector<TileDesc> launch = order_tiles(desc, kGroupMembers, nullptr, true, sweep_coresident_limit(nmax));
-      // quotas from each tile's per-sweep work, a cooperative group counting
-      // as kGroupSpeedup times faster than one workgroup
-      std::vector<double> work(active.size());
-      for (size_t i = 0; i < active.size(); ++i) work[i] = std::max(sweep_work(*active[i]), 1.0);
+      // quotas from each tile's per-sweep work (as a cooperative group when
+      // the launch made it one)
+      std::vector<char> grouped(active.size(), 0);
       for (const TileDesc& t : launch)
-        if (t.members > 1 && t.member == 0) work[index[t.spec]] /= kGroupSpeedup;
+        if (t.members > 1) grouped[index[t.spec]] = 1;
+      std::vector<double> work(active.size());
+      for (size_t i = 0; i < active.size(); ++i) work[i] = std::max(sweep_work(*active[i], grouped[i] != 0), 1e-3);
       const double wmax = *std::max_element(work.begin(), work.end());
       std::vector<Job*> leaving;  // jobs this tick completes
       for (size_t i = 0; i < active.size(); ++i) {
