@@ -539,8 +539,10 @@ __device__ __forceinline__ bool in_extent(uint32_t e, int c) {
 
 // Upper bound of |O|^2 over the spectrum rows r0 .. r1 and columns c0 .. c1
 // (inclusive, clamped to the spectrum) from the block bounds (T.bounds), read
-// by atomic operations, which the L2 serves, so that the raises of this
-// workgroup's inserts are seen.
+// as relaxed workgroup-scope atomics (the maximum with 0 compiles to a plain
+// load). The raises of this workgroup's earlier inserts (atomic maxima of the
+// same scope) are seen: the workgroup barriers between an insert and these
+// reads order them (one CU; its vector L1 is coherent for the workgroup).
 __device__ __forceinline__ double block_bound(const TileDesc& T, uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1) {
   r1 = r1 < T.rh ? r1 : T.rh - 1;  // a tap row or column past the spectrum is never read
   c1 = c1 < T.rw ? c1 : T.rw - 1;
