@@ -186,7 +186,7 @@ def run_batched(tiles, iterations, steps, warmup, alg=None):
         ctx.synchronize()
         secs = time.perf_counter() - t0
         # EPRY batches run one sweep per launch (the gauge follows each sweep)
-        per_launch = 1 if isinstance(alg, Epry) else (batch.schedule()["sweeps_per_launch"] or 10)
+        per_launch = 1 if isinstance(alg, Epry) else (batch.schedule()["sweeps_per_launch"] or 50)
         return secs, float(np.mean(sweep_ms)), min(per_launch, iterations), download_ms(batch)
     finally:
         close_all(batch, sessions, dmes, dms)
@@ -304,7 +304,7 @@ def uneven_tiles(seed=11, only=None):
 def run_uneven(iterations=40, steps=1, warmup=1):
     """The uneven 96-tile batch, FP32 mPIE `iterations` sweeps, replayed
     from the batch's captured graph: one persistent fused launch runs the
-    schedule's sweeps_per_launch (default 10) sweeps of every tile. Returns
+    schedule's sweeps_per_launch (default 50) sweeps of every tile. Returns
     tiles/s, the sweep-kernel figures and the algorithmic bytes."""
     log("building the uneven 96-tile batch")
     probs = uneven_tiles()

@@ -506,7 +506,7 @@ typedef struct fpmhip_batch_schedule {
   uint32_t largest_first; /* 1: launch the largest tiles first; 0: session order */
   uint32_t radix[4];      /* first-stage FFT radix for 32, 64, 128, 256 pixel tiles (0: default;
                              32: 8|4, 64: 8|4|16, 128: 8|16, 256: 16) */
-  uint32_t sweeps_per_launch; /* sweeps one launch runs (1..1000; 0: default, 10): fewer launches
+  uint32_t sweeps_per_launch; /* sweeps one launch runs (1..1000; 0: default, 50): fewer launches
                                  and host synchronisations per run */
 } fpmhip_batch_schedule;
 int fpmhip_batch_get_schedule(const fpmhip_batch* batch, fpmhip_batch_schedule* out);
